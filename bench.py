@@ -1,0 +1,115 @@
+"""Benchmark of the MI355X path-tracing hot path (one JSON line on stdout).
+
+Workload: the box lit by a 1024x512 sky+sun environment map sampled by the steerable sampler,
+GGX dielectric + Lambert spheres, 512x512 pixels, 64 samples per pixel, max path length 10
+(-sb 1 -iic 0).  One step = one full render of that image (all 64 iterations of RunIteration).
+`value` is Msamples/s with every input already resident in HBM; the roofline object reports the
+dominant kernel against MI355X peaks, and `cpu_baseline` times the CPU oracle (a restatement of
+the reference's OpenMP renderer) on a bounded slice of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 (vector) peak
+TRI_FLOP, SPH_FLOP, SHADE_FLOP = 62, 35, 400
+BYTES_PER_SAMPLE, BYTES_PER_SEGMENT = 112, 304
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=2, help="2: steerable-EM box 512^2 x 64 spp")
+    ap.add_argument("--kernel", choices=["megakernel", "wavefront"], default="megakernel")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(sb, spp, target_s):
+    """Times the CPU oracle (all host threads) on a bounded band of image rows at full spp."""
+    from oracle import oracle as O
+    w, h = sb.resolution
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    orc = O.OracleScene(sb)
+    # calibrate on 4 rows, then size the band to about target_s seconds
+    probe_rows = 4
+    r0 = h // 2 - probe_rows // 2
+    _, t = orc.render(0, spp, threads=threads, pixel_begin=r0 * w, pixel_end=(r0 + probe_rows) * w)
+    rows = max(probe_rows, min(h, int(probe_rows * target_s / max(t, 1e-3))))
+    r0 = max(0, h // 2 - rows // 2)
+    _, t = orc.render(0, spp, threads=threads, pixel_begin=r0 * w, pixel_end=(r0 + rows) * w)
+    samples = rows * w * spp
+    return {"value": samples / t / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"rows {r0}-{r0 + rows - 1} of {h} ({rows * w} pixels) x {spp} spp, {t:.1f} s wall"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build", file=sys.stderr)
+        sys.exit(2)
+    from pg3render_amd import Context, SceneBuild, _ffi
+
+    sb = SceneBuild.workload(args.workload, seed=2024)
+    spp = sb.spp
+    w, h = sb.resolution
+    d = sb.desc
+    ctx = Context(0)
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL if args.kernel == "megakernel" else _ffi.KERNEL_WAVEFRONT)
+    ctx.upload(sb)
+
+    for i in range(args.warmup):
+        ctx.render(i * spp, spp)
+    ctx.synchronize()
+    ctx.clear(0)
+    ctx.reset_counters()
+
+    kernel_ms = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        kernel_ms.append(ctx.render_timed(i * spp, spp))
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    segments, samples = ctx.counters()
+
+    ms_per_step = wall * 1000.0 / args.steps
+    total_samples = w * h * spp * args.steps
+    msps = total_samples / wall / 1e6
+    # algorithmic work model (one path segment = one extension ray + its shadow ray)
+    ray_flop = TRI_FLOP * d.num_triangles + SPH_FLOP * d.num_spheres
+    flops = samples * ray_flop + segments * (2 * ray_flop + SHADE_FLOP)
+    bytes_alg = BYTES_PER_SAMPLE * samples + BYTES_PER_SEGMENT * segments
+    kernel_s = sum(kernel_ms) / 1000.0
+    tflops = flops / kernel_s / 1e12
+    gbs = bytes_alg / kernel_s / 1e9
+    rays_per_s = (samples + 2 * segments) / wall
+    roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "compute": "fp32 VALU (no matrix work on this path); peak = MI355X FP32 157.3 TFLOP/s",
+                "hbm_model_gbs": round(gbs, 1), "hbm_model_frac": round(gbs / HBM_PEAK_GBS, 4),
+                "kernel_ms_per_step": round(sum(kernel_ms) / len(kernel_ms), 3),
+                "flop_per_segment": 2 * ray_flop + SHADE_FLOP, "segments_per_step": segments // args.steps}
+    out = {"metric": "Msamples/s", "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "box + 1024x512 sky/sun EM (steerable, levels<=4), GGX dielectric + Lambert "
+                                  "spheres, pt -sb 1 -iic 0 -maxpl 10", "resolution": [w, h], "spp": spp,
+                      "kernel": args.kernel},
+           "rays_per_s": round(rays_per_s, 1), "roofline": roofline}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sb, spp, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

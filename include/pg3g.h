@@ -1,0 +1,225 @@
+/*
+ * pg3g.h -- C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * The library replaces the inner work of PG3Render's CPU renderers behind the
+ * reference's own renderer boundary, `AbstractRenderer` (src/renderer.hxx:232-278):
+ *
+ *   AbstractRenderer::RunIteration(Algorithm, uint32_t)  -> pg3g_render()
+ *       (src/path_tracer_base.hxx:57-90, one sample per pixel per iteration)
+ *   AbstractRenderer::GetFramebuffer(Framebuffer&)         -> pg3g_read_framebuffer()
+ *       (src/renderer.hxx:255-261; FP64 sums, src/frame_buffer.hxx:30-77)
+ *   Scene (geometry, materials, lights, camera, EM)        -> pg3g_upload_scene()
+ *       (src/scene.hxx:42-113 queries, :176-956 builder output)
+ *
+ * Every call returns an int status (0 = PG3G_OK) and throws nothing across the
+ * ABI; pg3g_last_error() returns the message of the last failure on the calling
+ * host thread.  A context is used from one host thread at a time.  All pointer
+ * arguments are host pointers unless the name ends in `_dev`.
+ */
+#ifndef PG3G_H
+#define PG3G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------------------- */
+#define PG3G_OK               0
+#define PG3G_ERR_INVALID_ARG  1
+#define PG3G_ERR_HIP          2
+#define PG3G_ERR_UNSUPPORTED  3
+#define PG3G_ERR_OUT_OF_MEMORY 4
+#define PG3G_ERR_NO_SCENE     5
+
+/* ---- algorithms: same numbering as `enum Algorithm` (src/config.hxx:23-34) ------------- */
+#define PG3G_ALG_EYE_LIGHT            0
+#define PG3G_ALG_NORMAL_VISUALISER    1
+#define PG3G_ALG_DIRECT_BSDF          2  /* dbs  */
+#define PG3G_ALG_DIRECT_LIGHT_ALL     3  /* dlsa */
+#define PG3G_ALG_DIRECT_LIGHT_SINGLE  4  /* dlss */
+#define PG3G_ALG_DIRECT_MIS           5  /* dmis */
+#define PG3G_ALG_PATH_TRACING_NAIVE   6  /* ptn  */
+#define PG3G_ALG_PATH_TRACING         7  /* pt   */
+
+/* ---- materials (src/materials.hxx) ---------------------------------------------------- */
+#define PG3G_MAT_LAMBERT           0  /* LambertMaterial                 :253-350  */
+#define PG3G_MAT_PHONG             1  /* PhongMaterial                   :353-601  */
+#define PG3G_MAT_GGX_CONDUCTOR     2  /* MicrofacetGGXConductorMaterial  :788-989  */
+#define PG3G_MAT_GGX_DIELECTRIC    3  /* MicrofacetGGXDielectricMaterial :992-1298 */
+#define PG3G_MAT_SMOOTH_CONDUCTOR  4  /* SmoothConductorMaterial         :657-710  */
+#define PG3G_MAT_SMOOTH_DIELECTRIC 5  /* SmoothDielectricMaterial        :713-785  */
+
+typedef struct pg3g_material {
+    int32_t type;              /* PG3G_MAT_* */
+    int32_t light_id;          /* emitter this material belongs to, -1 if none
+                                  (the material->light map, src/scene.hxx:51-55) */
+    float   diffuse[3];        /* Lambert reflectance; Phong diffuse reflectance as passed to the
+                                  PhongMaterial ctor (halved by the library when glossy is on) */
+    float   glossy[3];         /* Phong glossy reflectance; GGX specular tint (1,1,1 = reference) */
+    float   phong_exponent;    /* Phong exponent */
+    int32_t phong_diffuse_on;  /* PhongMaterial ctor flags aDiffuse / aGlossy */
+    int32_t phong_glossy_on;
+    float   alpha;             /* GGX roughness, clamped to [0.001, 1] as in the ctors */
+    float   inner_ior;         /* n inside */
+    float   outer_ior;         /* n outside */
+    float   absorbance;        /* conductor k */
+    int32_t back_side_light_sampling; /* GGX dielectric ctor flag (materials.hxx:1000-1007) */
+} pg3g_material;
+
+/* ---- lights (src/lights.hxx) ---------------------------------------------------------- */
+#define PG3G_LIGHT_AREA        0  /* AreaLight: one-sided triangle p0,p1,p2 */
+#define PG3G_LIGHT_POINT       1  /* PointLight at p0 */
+#define PG3G_LIGHT_BACKGROUND  2  /* BackgroundLight: constant radiance or the EM of the scene */
+
+typedef struct pg3g_light {
+    int32_t type;
+    float   p0[3], p1[3], p2[3];
+    float   power[3];          /* flux as given to SetPower(); for the background: constant
+                                  radiance (SetConstantRadiance) when the scene has no EM image */
+} pg3g_light;
+
+/* ---- environment map ------------------------------------------------------------------ */
+#define PG3G_EM_NONE        0
+#define PG3G_EM_COSINE      1  /* EnvironmentMapCosineSampler      src/em_cosine_sampler.hxx  */
+#define PG3G_EM_SIMPLE      2  /* EnvironmentMapSimpleSphericalSampler (2-D luminance CDF)    */
+#define PG3G_EM_STEERABLE   3  /* EnvironmentMapSteerableSampler   src/em_steerable_sampler.hxx */
+
+typedef struct pg3g_em_desc {
+    uint32_t     width, height;  /* lat-long RGB image; 0 = no image */
+    const float *texels;         /* width*height*3 FP32, row-major (em_image.hxx:185-199) */
+    int32_t      bilinear;       /* tent filtering (default on, lights.hxx:372) */
+    int32_t      sampler;        /* PG3G_EM_* */
+    /* steerable BuildParameters (em_steerable_sampler.hxx:27-57); negative = reference default */
+    float        st_max_approx_error;   /* <= 0: default 5.0 */
+    int32_t      st_min_subdiv_level;   /* < 0: default 4 */
+    int32_t      st_max_subdiv_level;   /* < 0: default 9 */
+} pg3g_em_desc;
+
+/* ---- camera (src/camera.hxx:13-59) ---------------------------------------------------- */
+typedef struct pg3g_camera_desc {
+    float    position[3];
+    float    forward[3];
+    float    up[3];
+    float    hfov_deg;
+    uint32_t res_x, res_y;
+} pg3g_camera_desc;
+
+/* ---- render parameters (Config, src/config.hxx:556-578) ------------------------------- */
+typedef struct pg3g_render_params {
+    int32_t  algorithm;          /* PG3G_ALG_* */
+    uint32_t min_path_length;    /* -minpl (default 1) */
+    uint32_t max_path_length;    /* -maxpl (0 = Russian roulette) */
+    float    indirect_clipping;  /* -iic (0 = off) */
+    float    splitting_budget;   /* -sb (the GPU path supports 1) */
+    float    splitting_level;    /* -slbr (default 1) */
+    float    light_to_bsdf_ratio;/* -sl  (default 1) */
+    uint64_t seed;               /* counter-based RNG key */
+    uint32_t scene_id;           /* RNG stream id of the scene (batched renders) */
+} pg3g_render_params;
+
+/* ---- scene ------------------------------------------------------------------------------ */
+typedef struct pg3g_scene_desc {
+    uint32_t             num_triangles;
+    const float         *triangles;       /* num_triangles * 9: p0, p1, p2 (scene_graph.hxx:88-101) */
+    const int32_t       *triangle_mat;    /* material index per triangle */
+    uint32_t             num_spheres;
+    const float         *spheres;         /* num_spheres * 4: centre, radius (scene_graph.hxx:164-174) */
+    const int32_t       *sphere_mat;
+    uint32_t             num_materials;
+    const pg3g_material *materials;
+    uint32_t             num_lights;
+    const pg3g_light    *lights;          /* Scene::mLights order; at most one background light */
+    pg3g_em_desc         em;              /* image of the background light, if any */
+    pg3g_camera_desc     camera;
+    pg3g_render_params   params;
+} pg3g_scene_desc;
+
+/* ---- context -------------------------------------------------------------------------- */
+typedef struct pg3g_context pg3g_context;
+
+/* Creates a context on HIP device `device`.  Replaces the per-thread renderer construction in
+ * CreateRenderer (src/pg3render.cxx:24-50). */
+int pg3g_create(pg3g_context **ctx, int device);
+int pg3g_destroy(pg3g_context *ctx);
+/* Message of the last failed call on this host thread ("" if none). */
+const char *pg3g_last_error(void);
+
+/* Uploads a scene into slot 0 and builds its EM sampling tables (FP64 luminance CDF prefix sums
+ * on the GPU; steerable triangulation + 6-ary tree on the host).  Replaces Scene::LoadCornellBox's
+ * output (src/scene.hxx:176-956) as seen by the renderers and EnvironmentMap's sampler Init
+ * (src/em.hxx:25-69).  Clears the framebuffer of the slot. */
+int pg3g_upload_scene(pg3g_context *ctx, const pg3g_scene_desc *desc);
+/* Same, into a numbered slot (0..PG3G_MAX_SCENES-1) for batched renders. */
+#define PG3G_MAX_SCENES 256
+int pg3g_upload_scene_slot(pg3g_context *ctx, uint32_t slot, const pg3g_scene_desc *desc);
+/* Replaces the render parameters of a slot without rebuilding tables. */
+int pg3g_set_params(pg3g_context *ctx, uint32_t slot, const pg3g_render_params *params);
+
+/* Renders iterations [first_iter, first_iter + n_iter) of slot 0: each iteration is one sample per
+ * pixel, exactly PathTracerBase::RunIteration (src/path_tracer_base.hxx:57-90), accumulated into the
+ * slot's FP64 framebuffer.  Asynchronous on the context stream. */
+int pg3g_render(pg3g_context *ctx, uint32_t first_iter, uint32_t n_iter);
+int pg3g_render_slot(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter);
+
+/* Same as pg3g_render_slot, bracketed by HIP events recorded on the context stream (the stream
+ * the kernels run on); waits for completion and returns the elapsed GPU time in milliseconds. */
+int pg3g_render_timed(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
+                      float *elapsed_ms);
+
+/* Kernel organisation of pg3g_render*: wavefront queues (default) or one fused kernel. */
+#define PG3G_KERNEL_WAVEFRONT 0
+#define PG3G_KERNEL_MEGAKERNEL 1
+int pg3g_set_kernel_mode(pg3g_context *ctx, int mode);
+
+/* Copies the FP64 per-pixel RGB sums (res_x*res_y*3) and the iteration count of a slot.
+ * The image the reference writes is sum / iterations (renderer.hxx:255-261). */
+int pg3g_read_framebuffer(pg3g_context *ctx, double *sum_rgb, uint64_t *iterations);
+int pg3g_read_framebuffer_slot(pg3g_context *ctx, uint32_t slot, double *sum_rgb, uint64_t *iterations);
+int pg3g_clear_framebuffer(pg3g_context *ctx, uint32_t slot);
+/* Device address of a slot's framebuffer (res_x*res_y*3 doubles) for zero-copy consumers. */
+int pg3g_framebuffer_device_ptr(pg3g_context *ctx, uint32_t slot, double **fb_dev);
+int pg3g_synchronize(pg3g_context *ctx);
+
+/* Number of path segments (extension ray + its shadow ray) traced since the last reset. */
+int pg3g_read_counters(pg3g_context *ctx, uint64_t *segments, uint64_t *camera_samples);
+int pg3g_reset_counters(pg3g_context *ctx);
+
+/* ---- function-level entry points (tier-1 parity; host arrays, n records) ----------------- */
+/* AbstractMaterial::EvalBsdf with kOptSamplingProbs (materials.hxx:228-229): wil/wol local
+ * directions (n*3) -> f (n*3), pdfW, compProb. */
+int pg3g_bsdf_eval_batch(pg3g_context *ctx, const pg3g_material *mat, uint32_t n,
+                         const float *wil, const float *wol,
+                         float *f, float *pdf, float *comp_prob);
+/* AbstractMaterial::SampleBsdf (materials.hxx:231-234) with explicit uniforms u (n*3):
+ * u[0],u[1] = direction sample, u[2] = component / reflect-refract choice.
+ * Outputs wil (n*3), attenuation (n*3), pdfW, compProb. */
+int pg3g_bsdf_sample_batch(pg3g_context *ctx, const pg3g_material *mat, uint32_t n,
+                           const float *wol, const float *u,
+                           float *wil, float *f, float *pdf, float *comp_prob);
+/* EnvironmentMapSampler::Sample / PdfW and EnvironmentMapImage::Evaluate of slot 0's EM
+ * (em_sampler.hxx:49-64, em_image.hxx:95-150).  normals: n*3 world surface normals (frame from
+ * Frame::SetFromZ), side_flags: bit0 front, bit1 back; u: n*3 uniforms.
+ * Sample outputs: dir (n*3), pdf, radiance*cos (n*3), ok. */
+int pg3g_em_sample_batch(pg3g_context *ctx, uint32_t n, const float *normals,
+                         const int32_t *side_flags, const float *u,
+                         float *dir, float *pdf, float *radiance_cos, int32_t *ok);
+int pg3g_em_pdf_batch(pg3g_context *ctx, uint32_t n, const float *dirs, const float *normals,
+                      const int32_t *side_flags, float *pdf);
+int pg3g_em_eval_batch(pg3g_context *ctx, uint32_t n, const float *dirs, float *rgb);
+
+/* Introspection of the steerable tables of slot 0 (for parity with the reference build). */
+int pg3g_steerable_info(pg3g_context *ctx, uint32_t *num_leaves, uint32_t *num_vertices,
+                        uint32_t *num_levels);
+/* Copies leaf triangles in tree order: vertex directions (num_leaves*9). */
+int pg3g_steerable_leaves(pg3g_context *ctx, float *leaf_dirs);
+/* Root weight (9 SH floats) = sum of all leaf weights grouped as the 6-ary tree does. */
+int pg3g_steerable_root_weight(pg3g_context *ctx, float *weight9);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PG3G_H */

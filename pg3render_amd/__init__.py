@@ -1,0 +1,10 @@
+"""MI355X (gfx950) path-tracing hot path of PG3Render behind its renderer boundary.
+
+The GPU work runs in libpg3g.so (hand-written HIP kernels + C ABI, include/pg3g.h); this package
+only binds it.  There is no CPU fallback: if the library is not built, importing the renderer
+raises.
+"""
+from . import _ffi
+from .renderer import Context, GpuPathTracer, SceneBuild
+
+__all__ = ["Context", "GpuPathTracer", "SceneBuild", "_ffi"]

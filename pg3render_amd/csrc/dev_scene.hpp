@@ -1,0 +1,545 @@
+// Device scene layer: flat SoA-friendly scene records in HBM, brute-force intersection,
+// lights, the environment-map image and its two table-driven samplers (2-D luminance CDF and the
+// steerable 6-ary triangle tree).  Behaviour follows the cited PG3Render files.
+#pragma once
+
+#include "dev_math.hpp"
+
+namespace pg3 {
+
+// Triangle record: 64 B, read with wave-uniform (scalar) loads in the brute-force loop.
+struct alignas(16) DevTri {
+    float p0[3], p1[3], p2[3];
+    float n[3];          // normalize(cross(p1-p0, p2-p0)), scene_graph.hxx:100
+    int32_t mat;
+    int32_t pad[3];
+};
+struct alignas(16) DevSphere {
+    float c[3], r;
+    int32_t mat;
+    int32_t pad[3];
+};
+enum : int32_t { kLightArea = 0, kLightPoint = 1, kLightBackground = 2 };
+struct alignas(16) DevLight {
+    int32_t type;
+    float invArea;
+    float pad[2];
+    F3 p0, e1, e2;       // area triangle (point light: position in p0)
+    F3 n;                // AreaLight frame normal
+    F3 radiance;         // area: Le = power*invArea/pi; point: intensity; background: constant radiance
+};
+
+enum : int32_t { kEmNone = 0, kEmCosine = 1, kEmSimple = 2, kEmSteerable = 3 };
+constexpr int kMaxTreeLevels = 16;
+
+// Environment map and its sampling tables.
+struct DevEm {
+    uint32_t w, h;
+    int32_t bilinear;
+    int32_t sampler;
+    const float4 *texels;      // w*h RGB texels padded to 16 B (one load per texel)
+    // simple spherical sampler (em_simple_spherical_sampler.hxx): FP32 CDFs built from FP64 sums
+    const float *margCdf;      // h+1
+    const float *condCdf;      // h*(w+1)
+    float plan2ang;
+    // steerable sampler (em_steerable_sampler.hxx): layered 6-ary tree
+    uint32_t numLevels;        // levels[0] = leaves ... levels[numLevels-1] = root
+    uint32_t levelBase[kMaxTreeLevels];
+    uint32_t levelCount[kMaxTreeLevels];
+    const float *nodeW;        // 12 floats per node (9 SH weights + pad), all levels concatenated
+    const float *leafVerts;    // per leaf: 3 vertices x (dir 3 + SH weight 9) = 36 floats
+    float rootW[9];
+};
+
+struct SceneView {
+    const DevTri *tris;
+    const DevSphere *sphs;
+    const DevMaterial *mats;
+    const DevLight *lights;
+    uint32_t numTris, numSphs, numLights;
+    int32_t bgLight;
+    DevEm em;
+    float r2w[16];             // raster->world (row-major), camera.hxx:47-55
+    F3 camPos;
+    uint32_t resX, resY;
+    // Config
+    int32_t algorithm;
+    uint32_t minPL, maxPL;
+    float iic;
+    uint32_t k0, k1, sceneId;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Intersection (scene_graph.hxx, scene.hxx)
+struct Hit {
+    float dist;
+    int32_t mat;
+    F3 n;
+};
+
+PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, float tmin, Hit &h) {       // :103-134
+    const F3 ao = mk(t.p0[0], t.p0[1], t.p0[2]) - org;
+    const F3 bo = mk(t.p1[0], t.p1[1], t.p1[2]) - org;
+    const F3 co = mk(t.p2[0], t.p2[1], t.p2[2]) - org;
+    const float v0d = Dot(Cross(co, bo), dir);
+    const float v1d = Dot(Cross(bo, ao), dir);
+    const float v2d = Dot(Cross(ao, co), dir);
+    if (((v0d < 0.f) && (v1d < 0.f) && (v2d < 0.f)) || ((v0d >= 0.f) && (v1d >= 0.f) && (v2d >= 0.f))) {
+        const F3 n = mk(t.n[0], t.n[1], t.n[2]);
+        const float d = Dot(n, ao) / Dot(n, dir);
+        if ((d > tmin) & (d < h.dist)) { h.dist = d; h.mat = t.mat; h.n = n; return true; }
+    }
+    return false;
+}
+// :176-245.  B*B - 4*A*C is evaluated in FP32 (all float operands) and widened; the root, q and
+// both roots are FP64 as in the reference.
+PG3_DEV bool SphHit(const DevSphere &s, F3 org, F3 dir, float tmin, Hit &h) {
+    const F3 o = org - mk(s.c[0], s.c[1], s.c[2]);
+    const float A = Dot(dir, dir);
+    const float B = 2 * Dot(dir, o);
+    const float C = Dot(o, o) - (s.r * s.r);
+    const float discF = B * B - 4 * A * C;
+    if (discF < 0.f) return false;
+    const double ds = sqrt((double)discF);
+    const double q = (B >= 0) ? ((-(double)B - ds) / 2.0) : ((-(double)B + ds) / 2.0);
+    double t0 = q / (double)A;
+    double t1 = (fabs(q) > 0.1) ? ((double)C / q) : t0;
+    if (t0 > t1) { const double t = t0; t0 = t1; t1 = t; }
+    float resT;
+    if (t0 > tmin && t0 < h.dist) resT = (float)t0;
+    else if (t1 > tmin && t1 < h.dist) resT = (float)t1;
+    else return false;
+    h.dist = resT;
+    h.mat = s.mat;
+    h.n = Normalize(o + mk(resT) * dir);
+    return true;
+}
+PG3_DEV bool Intersect(const SceneView &S, F3 org, F3 dir, float tmin, Hit &h) {  // scene.hxx:42-59
+    bool any = false;
+    for (uint32_t i = 0; i < S.numTris; ++i) any |= TriHit(S.tris[i], org, dir, tmin, h);
+    for (uint32_t i = 0; i < S.numSphs; ++i) any |= SphHit(S.sphs[i], org, dir, tmin, h);
+    return any;
+}
+PG3_DEV bool Occluded(const SceneView &S, F3 p, F3 dir, float tmax) {              // scene.hxx:61-74
+    const F3 org = p + dir * kEpsRay;
+    Hit h;
+    h.dist = tmax - 2 * kEpsRay;
+    for (uint32_t i = 0; i < S.numTris; ++i) if (TriHit(S.tris[i], org, dir, 0.f, h)) return true;
+    for (uint32_t i = 0; i < S.numSphs; ++i) if (SphHit(S.sphs[i], org, dir, 0.f, h)) return true;
+    return false;
+}
+
+// Camera::GenerateRay, camera.hxx:91-100 (TransformPoint, math.hxx:734-752)
+PG3_DEV void GenerateRay(const SceneView &S, float x, float y, F3 &org, F3 &dir) {
+    const float *m = S.r2w;
+    float w = m[15];
+    w += m[12] * x;
+    w += m[13] * y;
+    w += m[14] * 0.f;
+    const float invW = 1.f / w;
+    F3 p;
+    p.x = (((m[3] + x * m[0]) + y * m[1]) + 0.f * m[2]) * invW;
+    p.y = (((m[7] + x * m[4]) + y * m[5]) + 0.f * m[6]) * invW;
+    p.z = (((m[11] + x * m[8]) + y * m[9]) + 0.f * m[10]) * invW;
+    org = S.camPos;
+    dir = Normalize(p - S.camPos);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Environment map image (em_image.hxx)
+PG3_DEV F3 Texel(const DevEm &E, uint32_t x, uint32_t y) {
+    const float4 t = E.texels[(size_t)E.w * y + x];
+    return mk(t.x, t.y, t.z);
+}
+PG3_DEV void EmNorm(const DevEm &E, int32_t x, int32_t y, uint32_t &ox, uint32_t &oy) {   // :245-278
+    int32_t rx, ry;
+    if (y >= (int32_t)E.h) { rx = x + (int32_t)(E.w / 2); ry = (int32_t)E.h - 1; }
+    else if (y < 0) { rx = x + (int32_t)(E.w / 2); ry = 0; }
+    else { rx = x; ry = y; }
+    int32_t m = rx % (int32_t)E.w;
+    if (m < 0) m += (int32_t)E.w;
+    ox = (uint32_t)m;
+    oy = (uint32_t)ry;
+}
+PG3_DEV F3 EmEvalUV(const DevEm &E, float u, float v) {                             // :95-142
+    const float xf = u * (float)E.w, yf = v * (float)E.h;
+    if (!E.bilinear) {
+        const uint32_t x0 = min((uint32_t)xf, E.w - 1u), y0 = min((uint32_t)yf, E.h - 1u);
+        return Texel(E, x0, y0);
+    }
+    const int32_t cx = (int32_t)(xf + 0.5f), cy = (int32_t)(yf + 0.5f);
+    const int32_t x0 = cx - 1, y0 = cy - 1;
+    const float xl = xf - ((float)x0 + 0.5f);
+    const float yl = yf - ((float)y0 + 0.5f);
+    uint32_t ax, ay, bx, by;
+    EmNorm(E, x0, y0, ax, ay);
+    EmNorm(E, cx, cy, bx, by);
+    const F3 a = Texel(E, ax, ay) * (1 - xl) + Texel(E, bx, ay) * xl;
+    const F3 b = Texel(E, ax, by) * (1 - xl) + Texel(E, bx, by) * xl;
+    return a * (1 - yl) + b * yl;
+}
+PG3_DEV F3 EmEval(const DevEm &E, F3 dir) {
+    float u, v;
+    Dir2LatLong(dir, u, v);
+    return EmEvalUV(E, u, v);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Simple spherical sampler (em_simple_spherical_sampler.hxx) over the 2-D CDF
+// (distribution.hxx:391-455, 510-541): upper_bound over the full CDF selects the same segment as
+// the reference's 16-float hierarchical blocks.
+PG3_DEV uint32_t UpperBound(const float *cdf, uint32_t n1, float x) {   // first index with cdf > x
+    uint32_t lo = 0, len = n1;
+    while (len > 0) {
+        const uint32_t half = len >> 1;
+        if (cdf[lo + half] <= x) { lo += half + 1; len -= half + 1; }
+        else len = half;
+    }
+    return lo;
+}
+PG3_DEV void Sample1D(const float *cdf, uint32_t count, float u, float &x, uint32_t &seg, float &pdf) {
+    const float ut = u * 0.999999f;
+    int32_t s = (int32_t)UpperBound(cdf, count + 1, ut) - 1;
+    s = max(0, min(s, (int32_t)count - 1));
+    seg = (uint32_t)s;
+    const float c0 = cdf[seg], c1 = cdf[seg + 1];
+    const float p = c1 - c0;
+    const float off = (ut - c0) / p;
+    pdf = p * count;
+    x = (seg + off) / count;
+}
+PG3_DEV float SinMidTheta(uint32_t h, uint32_t sy) { return sinf(kPiF * (sy + 0.5f) / h); }  // :177-189
+PG3_DEV void SimpleSample(const DevEm &E, const Frame &fr, uint32_t sides, float u0, float u1, F3 &dir,
+                          float &pdf, F3 &radCos) {                                      // :27-81
+    float u, v, mp, cp;
+    uint32_t sx, sy;
+    Sample1D(E.margCdf, E.h, u0, v, sy, mp);
+    Sample1D(E.condCdf + (size_t)sy * (E.w + 1), E.w, u1, u, sx, cp);
+    dir = LatLong2Dir(u, v);
+    pdf = (mp * cp) * E.plan2ang / SinMidTheta(E.h, sy);
+    const F3 rad = !E.bilinear ? Texel(E, sx, sy) : EmEvalUV(E, u, v);
+    const float c = Dot(dir, fr.z);
+    if (((sides & kSideFront) && c > 0.0f) || ((sides & kSideBack) && c < 0.0f)) radCos = rad * fabsf(c);
+    else radCos = mk(0.f);
+}
+PG3_DEV float SimplePdf(const DevEm &E, F3 dir) {                                    // :84-98, distribution.hxx:530-540
+    float u, v;
+    Dir2LatLong(dir, u, v);
+    const uint32_t iu = min((uint32_t)fmaxf(0.f, u * E.w), E.w - 1);
+    const uint32_t iv = min((uint32_t)fmaxf(0.f, v * E.h), E.h - 1);
+    const float *cc = E.condCdf + (size_t)iv * (E.w + 1);
+    const float pc = (cc[iu + 1] - cc[iu]) * E.w;
+    const float pm = (E.margCdf[iv + 1] - E.margCdf[iv]) * E.h;
+    const uint32_t seg = min((uint32_t)(v * E.h), E.h - 1u);
+    return (pc * pm) * E.plan2ang / SinMidTheta(E.h, seg);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Steerable sampler run time (em_steerable_sampler.hxx)
+struct SH9 {
+    float c[9];
+};
+PG3_DEV SH9 ClampedCos(F3 n) {                                 // GenerateForClampedCos(n, true), :657-688
+    const float c1 = .429043f, c2 = .511644f, c3 = .743125f, c4 = .886227f, c5 = .247708f;
+    SH9 s;
+    s.c[0] = c4 + 0.15f;
+    s.c[1] = 2 * c2 * n.y;
+    s.c[2] = 2 * c2 * n.z;
+    s.c[3] = 2 * c2 * n.x;
+    s.c[4] = 2 * c1 * n.x * n.y;
+    s.c[5] = 2 * c1 * n.y * n.z;
+    s.c[6] = c3 * Sqr(n.z) - c5;
+    s.c[7] = 2 * c1 * n.x * n.z;
+    s.c[8] = c1 * (Sqr(n.x) - Sqr(n.y));
+    return s;
+}
+PG3_DEV float Dot9(const float *w, const SH9 &c) {            // :135-146 (sequential sum)
+    float r = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r += w[i] * c.c[i];
+    return r;
+}
+PG3_DEV float ShDotCoeffs(F3 d, const SH9 &c) {               // Dot(GenerateSphHarm(d), coeffs), :181-202
+    const float b[9] = {0.282095f, 0.488603f * d.y, 0.488603f * d.z, 0.488603f * d.x,
+                        1.092548f * d.x * d.y, 1.092548f * d.y * d.z, 0.315392f * (3.f * d.z * d.z - 1.f),
+                        1.092548f * d.x * d.z, 0.546274f * (d.x * d.x - d.y * d.y)};
+    float r = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r += b[i] * c.c[i];
+    return r;
+}
+PG3_DEV float WholeIntegral(const DevEm &E, const SH9 &c) { return Dot9(E.rootW, c); }  // :2416-2432
+PG3_DEV float BasePdf(const DevEm &E, F3 dir, const SH9 &c) {                          // :2387-2413
+    const float lum = Luminance(EmEval(E, dir));
+    const float cc = ShDotCoeffs(dir, c);
+    const float whole = WholeIntegral(E, c);
+    return IsTiny(whole) ? 0.f : (lum * cc) / whole;
+}
+// PickTriangle, :3798-3875.  Level l node i has children [6i, min(6i+6, count(l-1))) on level l-1.
+PG3_DEV uint32_t PickLeaf(const DevEm &E, const SH9 &c, float &u) {
+    uint32_t node = 0;
+    for (int32_t lvl = (int32_t)E.numLevels - 1; lvl > 0; --lvl) {
+        const uint32_t first = node * 6u;
+        const uint32_t cc = min(6u, E.levelCount[lvl - 1] - first);
+        const float *w = E.nodeW + (size_t)(E.levelBase[lvl - 1] + first) * 12u;
+        float cs[7];
+        cs[0] = 0.f;
+#pragma unroll
+        for (uint32_t i = 0; i < 6; ++i) {
+            const float ci = (i < cc) ? Dot9(w + i * 12u, c) : 0.f;
+            cs[i + 1] = cs[i] + ci;
+        }
+        float whole = cs[0];
+#pragma unroll
+        for (uint32_t i = 1; i <= 6; ++i) if (i == cc) whole = cs[i];
+        if (whole > 1E-10f) {
+            const float inv = 1.f / whole;
+#pragma unroll
+            for (uint32_t i = 1; i <= 6; ++i) cs[i] *= inv;
+        }
+        const float ut = u * 0.999999f;
+        // upper_bound over cs[0..cc]; past-the-end clamps to the last element
+        uint32_t ub = cc;
+#pragma unroll
+        for (int32_t i = 6; i >= 0; --i) if ((uint32_t)i <= cc && cs[i] > ut) ub = (uint32_t)i;
+        if (ub == 0) ub = 1;   // cs[0] = 0 <= ut, unreachable; kept for safety
+        const uint32_t child = ub - 1u;
+        float low = 0.f, high = 0.f;
+#pragma unroll
+        for (uint32_t i = 0; i <= 6; ++i) {
+            if (i == ub - 1u) low = cs[i];
+            if (i == ub) high = cs[i];
+        }
+        const float nu = (ut - low) / fmaxf(high - low, 0.00001f);
+        u = fminf(nu, 1.f);
+        node = first + child;
+    }
+    return node;
+}
+// SampleTriangleFFunction / GFunction / Bilinear, :3709-3747, :3882-3905 (math.hxx:284-342)
+PG3_DEV void SampleTriBilinear(float ux, float uy, float va, float vb, float vc, float &b0, float &b1) {
+    const float x = (vb - va) / 3 + (vc - vb) / 6;
+    const float y = va / 2;
+    const float a = x / (x + y), b = y / (x + y);
+    float s = 0.5f;
+    bool bail = false;
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const float f = s * (s * (s * a + b) + 0.f) + (-ux);
+        const float d = s * (s * 3.f * a + 2.f * b) + 0.f;
+        if (!bail) {
+            if (fabsf(d) < 0.0001f) { bail = true; }
+            else s = Clampf(s - f / d, 0.f, 1.f);
+        }
+    }
+    if (bail) s = 0.5f;
+    const float t0 = (s * (vc - vb)) + (2 * (1 - s) * va) + (s * vb);
+    const float gamma = s * (vc - vb) / t0;
+    const float rho = 2 * ((1 - s) * va + s * vb) / t0;
+    const float t = (2 * uy) / (rho + SafeSqrt(rho * rho + 4 * gamma * uy));
+    b0 = 1 - s;
+    b1 = s * (1 - t);
+}
+// SampleHemisphereImpl, :2162-2217
+PG3_DEV bool SteerSampleHemisphere(const DevEm &E, F3 n, const SH9 &c, float u0, float u1, F3 &dir,
+                                   float &pdf, F3 &radCos) {
+    float ux = u0;
+    const uint32_t leaf = PickLeaf(E, c, ux);
+    const float *lv = E.leafVerts + (size_t)leaf * 36u;
+    const float v0 = Dot9(lv + 3, c), v1 = Dot9(lv + 15, c), v2 = Dot9(lv + 27, c);
+    float b0, b1;
+    SampleTriBilinear(ux, u1, v0, v1, v2, b0, b1);
+    const float b2 = 1.0f - b0 - b1;
+    const float val = b0 * v0 + b1 * v1 + b2 * v2;
+    const F3 p = mk(lv[0], lv[1], lv[2]) * b0 + mk(lv[12], lv[13], lv[14]) * b1 + mk(lv[24], lv[25], lv[26]) * b2;
+    dir = p / Length(p);
+    const float whole = WholeIntegral(E, c);
+    pdf = IsTiny(whole) ? 0.f : val / whole;
+    const F3 fl = -dir;
+    pdf += BasePdf(E, fl, c);
+    float cs = Dot(dir, n);
+    if (cs < 0.f) { dir = fl; cs *= -1.f; }
+    radCos = EmEval(E, dir) * cs;
+    return true;
+}
+// SampleImpl, :2084-2157
+PG3_DEV bool SteerSample(const DevEm &E, F3 n, uint32_t sides, float uSide, float u0, float u1, F3 &dir,
+                         float &pdf, F3 &radCos) {
+    const bool front = sides & kSideFront, back = sides & kSideBack;
+    if (front && back) {
+        const SH9 uc = ClampedCos(n), lc = ClampedCos(-n);
+        const float ui = WholeIntegral(E, uc), li = WholeIntegral(E, lc), wi = ui + li;
+        if (IsTiny(wi)) return false;
+        if (uSide * wi < ui) {
+            SteerSampleHemisphere(E, n, uc, u0, u1, dir, pdf, radCos);
+            pdf *= ui / wi;
+        } else {
+            SteerSampleHemisphere(E, -n, lc, u0, u1, dir, pdf, radCos);
+            pdf *= li / wi;
+        }
+        return true;
+    }
+    if (front) return SteerSampleHemisphere(E, n, ClampedCos(n), u0, u1, dir, pdf, radCos);
+    if (back) return SteerSampleHemisphere(E, -n, ClampedCos(-n), u0, u1, dir, pdf, radCos);
+    return false;
+}
+PG3_DEV float SteerPdfHemisphere(const DevEm &E, F3 dir, F3 n, const SH9 &c) {       // :2281-2299
+    if (Dot(dir, n) < 0.f) return 0.f;
+    return BasePdf(E, dir, c) + BasePdf(E, -dir, c);
+}
+PG3_DEV float SteerPdf(const DevEm &E, F3 dir, F3 n, uint32_t sides) {               // :2220-2278
+    const bool front = sides & kSideFront, back = sides & kSideBack;
+    if (front && back) {
+        const SH9 uc = ClampedCos(n), lc = ClampedCos(-n);
+        const float ui = WholeIntegral(E, uc), li = WholeIntegral(E, lc), wi = ui + li;
+        if (IsTiny(wi)) return 0.f;
+        if (Dot(dir, n) >= 0.f) return SteerPdfHemisphere(E, dir, n, uc) * (ui / wi);
+        return SteerPdfHemisphere(E, dir, -n, lc) * (li / wi);
+    }
+    if (front) return SteerPdfHemisphere(E, dir, n, ClampedCos(n));
+    if (back) return SteerPdfHemisphere(E, dir, -n, ClampedCos(-n));
+    return 0.f;
+}
+PG3_DEV float SteerIrradiance(const DevEm &E, F3 n, uint32_t sides) {               // :2304-2358
+    const float ui = WholeIntegral(E, ClampedCos(n)), li = WholeIntegral(E, ClampedCos(-n));
+    const bool front = sides & kSideFront, back = sides & kSideBack;
+    if (front && back) return ui + li;
+    if (front) return ui;
+    if (back) return li;
+    return 0.f;
+}
+
+// EnvironmentMap::PdfW of the active sampler (em.hxx:113-120)
+PG3_DEV float EmPdf(const DevEm &E, F3 dir, const Frame &fr, uint32_t sides) {
+    if (E.sampler == kEmSimple) return SimplePdf(E, dir);
+    if (E.sampler == kEmSteerable) return SteerPdf(E, dir, fr.z, sides);
+    return CosSpherePdfW(sides & kSideFront, sides & kSideBack, ToLocal(fr, dir));
+}
+// EnvironmentMapSampler::Sample of the active sampler (em_sampler.hxx:49-64)
+PG3_DEV bool EmSample(const DevEm &E, const Frame &fr, uint32_t sides, float u0, float u1, float u2, F3 &dir,
+                      float &pdf, F3 &radCos) {
+    if (E.sampler == kEmSimple) { SimpleSample(E, fr, sides, u0, u1, dir, pdf, radCos); return true; }
+    if (E.sampler == kEmSteerable) return SteerSample(E, fr.z, sides, u2, u0, u1, dir, pdf, radCos);
+    const F3 w = SampleCosSphereParamPdfW(u0, u1, u2, sides & kSideFront, sides & kSideBack, pdf);
+    dir = ToWorld(fr, w);
+    radCos = EmEval(E, dir) * fabsf(w.z);
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Lights (lights.hxx)
+struct LightSample {
+    F3 sample, wig;
+    float pdfW, prob, dist;
+};
+
+PG3_DEV float SideCos(uint32_t sides, float c) {                                    // :188-204
+    if ((sides & kSideFront) && (sides & kSideBack)) return fabsf(c);
+    if (sides & kSideFront) return fmaxf(c, 0.0f);
+    if (sides & kSideBack) return fmaxf(-c, 0.0f);
+    return 0.0f;
+}
+PG3_DEV void AreaComputeSample(const DevLight &L, F3 sp, F3 pt, F3 n, uint32_t sides, LightSample &o) {  // :171-219
+    o.wig = pt - sp;
+    const float d2 = LenSqr(o.wig);
+    o.dist = sqrtf(d2);
+    o.wig = o.wig / o.dist;
+    const float cosOut = -Dot(L.n, o.wig);
+    const float cosIn = SideCos(sides, Dot(n, o.wig));
+    o.sample = (cosOut > 0.f) ? L.radiance * cosIn : mk(0.f);
+    o.pdfW = fmaxf(L.invArea * (d2 / fabsf(cosOut)), kEpsDist);
+    o.prob = 1.0f;
+}
+PG3_DEV void PointCompute(const DevLight &L, F3 sp, F3 n, uint32_t sides, LightSample &o) {  // :296-331
+    o.wig = L.p0 - sp;
+    const float d2 = LenSqr(o.wig);
+    o.dist = sqrtf(d2);
+    o.wig = o.wig / o.dist;
+    const float cosIn = SideCos(sides, Dot(n, o.wig));
+    o.sample = (cosIn > 0.f) ? L.radiance * cosIn / d2 : mk(0.f);
+    o.pdfW = kInf;
+    o.prob = 1.0f;
+}
+// BackgroundLight::GetEmmision(dir), lights.hxx:379-407 (radiance only)
+PG3_DEV F3 BgRadiance(const SceneView &S, F3 dir) {
+    return (S.em.w > 0) ? EmEval(S.em, dir) : S.lights[S.bgLight].radiance;
+}
+// ... and its PDF w.r.t. a shading frame/material
+PG3_DEV float BgPdf(const SceneView &S, F3 dir, const Frame &fr, uint32_t sides) {
+    if (S.em.w > 0) return EmPdf(S.em, dir, fr, sides);
+    return CosSpherePdfW(sides & kSideFront, sides & kSideBack, ToLocal(fr, dir));
+}
+// AreaLight::GetEmmision PDF part, lights.hxx:88-99
+PG3_DEV float AreaEmissionPdf(const DevLight &L, F3 lightPt, F3 surfPt) {
+    F3 wig = lightPt - surfPt;
+    const float d2 = LenSqr(wig);
+    wig = wig / sqrtf(d2);
+    return fmaxf(L.invArea * (d2 / fabsf(Dot(L.n, wig))), kEpsDist);
+}
+
+// EnvironmentMap::EstimateIrradiance MC fallback (em.hxx:123-170), simple + cosine samplers
+PG3_DEV float EmEstimateMC(const SceneView &S, const Frame &fr, uint32_t sides, const RngKey &key, uint32_t vtx) {
+    const bool front = sides & kSideFront, back = sides & kSideBack;
+    float sum = 0.f;
+    for (uint32_t round = 0; round < 10; round++) {
+        const uint32_t d = kDimEstimateBase + round * 8;
+        float p1c;
+        const F3 w1 = SampleCosSphereParamPdfW(RngGet(key, vtx, 0, d), RngGet(key, vtx, 0, d + 1),
+                                               RngGet(key, vtx, 0, d + 2), front, back, p1c);
+        const F3 g1 = ToWorld(fr, w1);
+        const F3 s1 = EmEval(S.em, g1) * fabsf(w1.z);
+        const float p1s = SimplePdf(S.em, g1);
+        F3 g2, s2;
+        float p2s;
+        SimpleSample(S.em, fr, sides, RngGet(key, vtx, 0, d + 3), RngGet(key, vtx, 0, d + 4), g2, p2s, s2);
+        const float p2c = CosSpherePdfW(front, back, ToLocal(fr, g2));
+        sum += Luminance(s1) / (p1c + p1s) + Luminance(s2) / (p2c + p2s);
+    }
+    return sum / 10;
+}
+// AbstractLight::EstimateContribution
+PG3_DEV float EstimateContribution(const SceneView &S, uint32_t li, F3 sp, const Frame &fr, uint32_t sides,
+                                   const RngKey &key, uint32_t vtx) {
+    const DevLight &L = S.lights[li];
+    if (L.type == kLightArea) {                                                        // :133-167
+        const F3 P1 = L.p0 + L.e1, P2 = L.p0 + L.e2;
+        const F3 P3 = L.p0 + 0.33f * L.e1 + 0.33f * L.e2;
+        LightSample a, b, c, d;
+        AreaComputeSample(L, sp, L.p0, fr.z, sides, a);
+        AreaComputeSample(L, sp, P1, fr.z, sides, b);
+        AreaComputeSample(L, sp, P2, fr.z, sides, c);
+        AreaComputeSample(L, sp, P3, fr.z, sides, d);
+        return (Luminance(a.sample) / a.pdfW + Luminance(b.sample) / b.pdfW + Luminance(c.sample) / c.pdfW +
+                Luminance(d.sample) / d.pdfW) / 4.f;
+    }
+    if (L.type == kLightPoint) {                                                       // :277-290
+        LightSample s;
+        PointCompute(L, sp, fr.z, sides, s);
+        return Luminance(s.sample);
+    }
+    if (S.em.w > 0) {                                                                  // :451-467
+        if (S.em.sampler == kEmSteerable) return SteerIrradiance(S.em, fr.z, sides);
+        return EmEstimateMC(S, fr, sides, key, vtx);
+    }
+    return Luminance(L.radiance) * kPiF;
+}
+// AbstractLight::SampleIllumination
+PG3_DEV bool SampleIllumination(const SceneView &S, uint32_t li, F3 sp, const Frame &fr, uint32_t sides,
+                                float u0, float u1, float u2, LightSample &o) {
+    const DevLight &L = S.lights[li];
+    if (L.type == kLightArea) {                                                        // :114-128
+        const F3 pt = SampleUniformTriangle(L.p0, L.p0 + L.e1, L.p0 + L.e2, u0, u1);
+        AreaComputeSample(L, sp, pt, fr.z, sides, o);
+        return true;
+    }
+    if (L.type == kLightPoint) { PointCompute(L, sp, fr.z, sides, o); return true; }
+    o.prob = 1.0f;
+    o.dist = 3.402823466e+38f;   // em_sampler.hxx:57
+    if (S.em.w > 0) return EmSample(S.em, fr, sides, u0, u1, u2, o.wig, o.pdfW, o.sample);
+    const F3 w = SampleCosSphereParamPdfW(u0, u1, u2, sides & kSideFront, sides & kSideBack, o.pdfW);  // em_cosine_sampler.hxx:12-33
+    o.wig = ToWorld(fr, w);
+    o.sample = L.radiance * fabsf(w.z);
+    return true;
+}
+
+}  // namespace pg3

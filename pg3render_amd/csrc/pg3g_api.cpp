@@ -1,0 +1,692 @@
+// C ABI of libpg3g: contexts, scene upload with EM table builds, renders, framebuffer access and
+// the function-level entry points.  No exception crosses the ABI; failures return a status and
+// leave a message for pg3g_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pg3g.h"
+#include "pg3g_internal.hpp"
+
+using namespace pg3;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int Fail(int code, const std::string &msg) {
+    g_lastError = msg;
+    return code;
+}
+
+#define PG3_HIP(expr)                                                                         \
+    do {                                                                                      \
+        const hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                                 \
+            return Fail(e_ == hipErrorOutOfMemory ? PG3G_ERR_OUT_OF_MEMORY : PG3G_ERR_HIP,    \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { Free(); }
+    void Free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t Alloc(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        Free();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct SceneSlot {
+    bool valid = false;
+    SceneView view{};
+    pg3g_render_params params{};
+    DevBuf tris, sphs, mats, lights, texels, margCdf, condCdf, rowInt, nodeW, leafVerts, fb;
+    uint64_t iterations = 0;
+    SteerableTables steer;
+    bool hasSteer = false;
+};
+
+}  // namespace
+
+struct pg3g_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<SceneSlot>> slots;
+    DevBuf counters;   // [segments, camera samples]
+    int kernelMode = PG3G_KERNEL_MEGAKERNEL;
+    DevBuf scratch[8];
+};
+
+namespace {
+
+// Material constructors (materials.hxx): the record stores what the constructors leave behind.
+DevMaterial ToDev(const pg3g_material &d) {
+    DevMaterial m{};
+    m.type = d.type;
+    m.lightId = d.light_id;
+    m.diffuse = mk(d.diffuse[0], d.diffuse[1], d.diffuse[2]);
+    m.glossy = mk(d.glossy[0], d.glossy[1], d.glossy[2]);
+    m.phongExp = d.phong_exponent;
+    m.alpha = 0.1f;
+    m.eta = 1.f;
+    m.etaInv = 1.f;
+    m.k = d.absorbance;
+    const float inner = d.inner_ior, outer = d.outer_ior;
+    switch (d.type) {
+    case PG3G_MAT_LAMBERT:                                   // :253-266
+        m.glossy = mk(0.f);
+        m.sides = kSideFront;
+        break;
+    case PG3G_MAT_PHONG:                                     // :364-379
+        m.diffuse = d.phong_diffuse_on ? m.diffuse : mk(0.f);
+        m.glossy = d.phong_glossy_on ? m.glossy : mk(0.f);
+        if (d.phong_glossy_on) m.diffuse = m.diffuse / 2;
+        m.sides = kSideFront;
+        break;
+    case PG3G_MAT_GGX_CONDUCTOR:                             // :791-806
+        m.eta = !IsTiny(outer) ? inner / outer : 1.0f;
+        m.alpha = Clampf(d.alpha, 0.001f, 1.0f);
+        m.sides = kSideFront;
+        break;
+    case PG3G_MAT_GGX_DIELECTRIC:                            // :996-1021
+        if (!IsTiny(outer)) { m.eta = inner / outer; m.etaInv = outer / inner; }
+        m.alpha = Clampf(d.alpha, 0.001f, 1.0f);
+        m.sides = kSideFront | (d.back_side_light_sampling ? kSideBack : 0u);
+        break;
+    case PG3G_MAT_SMOOTH_CONDUCTOR:                          // :660-669
+        m.eta = !IsTiny(outer) ? inner / outer : 1.0f;
+        m.sides = 0;
+        break;
+    case PG3G_MAT_SMOOTH_DIELECTRIC:                         // :716-727
+        m.eta = !IsTiny(outer) ? inner / outer : 1.0f;
+        m.etaInv = !IsTiny(inner) ? outer / inner : 1.0f;
+        m.sides = 0;
+        break;
+    }
+    return m;
+}
+
+// Camera::Setup (camera.hxx:13-59), composed in FP64 and stored as FP32.
+void CameraMatrix(const pg3g_camera_desc &c, float out[16]) {
+    const double pi = 3.14159265358979323846;
+    double f[3] = {c.forward[0], c.forward[1], c.forward[2]};
+    const double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (double &v : f) v /= fl;
+    const double b[3] = {-f[0], -f[1], -f[2]};
+    double up[3] = {c.up[1] * b[2] - c.up[2] * b[1], c.up[2] * b[0] - c.up[0] * b[2], c.up[0] * b[1] - c.up[1] * b[0]};
+    const double ul = std::sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+    for (double &v : up) v /= ul;
+    const double left[3] = {b[1] * up[2] - b[2] * up[1], b[2] * up[0] - b[0] * up[2], b[0] * up[1] - b[1] * up[0]};
+    const double P[3] = {c.position[0], c.position[1], c.position[2]};
+    auto dot = [](const double *a, const double *q) { return a[0] * q[0] + a[1] * q[1] + a[2] * q[2]; };
+    double M[4][4] = {{up[0], up[1], up[2], -dot(up, P)},
+                      {left[0], left[1], left[2], -dot(left, P)},
+                      {b[0], b[1], b[2], -dot(b, P)},
+                      {0, 0, 0, 1}};
+    const double half = 0.5 * c.hfov_deg * pi / 180.0;
+    const double pixelSize = std::tan(half) / (0.5 * c.res_x);
+    const double vhalf = std::atan(pixelSize * 0.5 * c.res_y);
+    const double fx = 1.0 / std::tan(half), fy = 1.0 / std::tan(vhalf);
+    const double n = 0.1, fr = 10000.0, d = 1.0 / (n - fr);
+    const double Pm[4][4] = {{fx, 0, 0, 0}, {0, -fy, 0, 0}, {0, 0, (n + fr) * d, 2.0 * n * fr * d}, {0, 0, -1, 0}};
+    double W[4][8];   // [P*M | I] reduced to [I | (P*M)^-1]
+    for (int r = 0; r < 4; ++r)
+        for (int col = 0; col < 4; ++col) {
+            double s = 0;
+            for (int i = 0; i < 4; ++i) s += Pm[r][i] * M[i][col];
+            W[r][col] = s;
+            W[r][col + 4] = (r == col) ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r) if (std::fabs(W[r][col]) > std::fabs(W[piv][col])) piv = r;
+        for (int k = 0; k < 8; ++k) std::swap(W[col][k], W[piv][k]);
+        const double dv = W[col][col];
+        for (int k = 0; k < 8; ++k) W[col][k] /= dv;
+        for (int r = 0; r < 4; ++r) {
+            if (r == col) continue;
+            const double fct = W[r][col];
+            for (int k = 0; k < 8; ++k) W[r][k] -= fct * W[col][k];
+        }
+    }
+    // raster -> NDC: x' = 2x/res - 1, y' = 2y/res - 1, z' = 0
+    const double S[4][4] = {{2.0 / c.res_x, 0, 0, -1}, {0, 2.0 / c.res_y, 0, -1}, {0, 0, 0, 0}, {0, 0, 0, 1}};
+    for (int r = 0; r < 4; ++r)
+        for (int col = 0; col < 4; ++col) {
+            double s = 0;
+            for (int i = 0; i < 4; ++i) s += W[r][i + 4] * S[i][col];
+            out[r * 4 + col] = (float)s;
+        }
+}
+
+template <class T>
+hipError_t Upload(DevBuf &buf, const T *src, size_t count, hipStream_t s) {
+    hipError_t e = buf.Alloc(sizeof(T) * std::max<size_t>(count, 1));
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpyAsync(buf.p, src, sizeof(T) * count, hipMemcpyHostToDevice, s);
+}
+
+int ValidateDesc(const pg3g_scene_desc *d) {
+    if (!d) return Fail(PG3G_ERR_INVALID_ARG, "scene desc is null");
+    if (d->num_triangles && (!d->triangles || !d->triangle_mat)) return Fail(PG3G_ERR_INVALID_ARG, "triangle arrays are null");
+    if (d->num_spheres && (!d->spheres || !d->sphere_mat)) return Fail(PG3G_ERR_INVALID_ARG, "sphere arrays are null");
+    if (d->num_materials == 0 || !d->materials) return Fail(PG3G_ERR_INVALID_ARG, "scene has no materials");
+    if (d->num_lights && !d->lights) return Fail(PG3G_ERR_INVALID_ARG, "light array is null");
+    for (uint32_t i = 0; i < d->num_triangles; ++i)
+        if (d->triangle_mat[i] < 0 || (uint32_t)d->triangle_mat[i] >= d->num_materials)
+            return Fail(PG3G_ERR_INVALID_ARG, "triangle material index out of range");
+    for (uint32_t i = 0; i < d->num_spheres; ++i) {
+        if (d->sphere_mat[i] < 0 || (uint32_t)d->sphere_mat[i] >= d->num_materials)
+            return Fail(PG3G_ERR_INVALID_ARG, "sphere material index out of range");
+        if (!(d->spheres[i * 4 + 3] > 0.f)) return Fail(PG3G_ERR_INVALID_ARG, "sphere radius must be positive");
+    }
+    for (uint32_t i = 0; i < d->num_materials; ++i) {
+        const pg3g_material &m = d->materials[i];
+        if (m.type < PG3G_MAT_LAMBERT || m.type > PG3G_MAT_SMOOTH_DIELECTRIC) return Fail(PG3G_ERR_INVALID_ARG, "unknown material type");
+        if (m.light_id >= (int32_t)d->num_lights) return Fail(PG3G_ERR_INVALID_ARG, "material light id out of range");
+        if (m.light_id >= 0 && d->lights[m.light_id].type != PG3G_LIGHT_AREA)
+            return Fail(PG3G_ERR_INVALID_ARG, "only area lights can own geometry");
+        const float vals[] = {m.diffuse[0], m.diffuse[1], m.diffuse[2], m.glossy[0], m.glossy[1], m.glossy[2],
+                              m.alpha, m.inner_ior, m.outer_ior, m.absorbance, m.phong_exponent};
+        for (float v : vals) if (!std::isfinite(v)) return Fail(PG3G_ERR_INVALID_ARG, "material parameter is not finite");
+    }
+    int bg = 0;
+    for (uint32_t i = 0; i < d->num_lights; ++i) {
+        const int t = d->lights[i].type;
+        if (t < PG3G_LIGHT_AREA || t > PG3G_LIGHT_BACKGROUND) return Fail(PG3G_ERR_INVALID_ARG, "unknown light type");
+        if (t == PG3G_LIGHT_BACKGROUND) ++bg;
+    }
+    if (bg > 1) return Fail(PG3G_ERR_INVALID_ARG, "at most one background light");
+    if (d->num_lights > 1 && d->num_lights > 8)
+        return Fail(PG3G_ERR_UNSUPPORTED, "more than 8 lights with light picking are not supported");
+    if (d->em.width > 0) {
+        if (!d->em.texels || d->em.height == 0) return Fail(PG3G_ERR_INVALID_ARG, "EM image arrays are null");
+        if (d->em.sampler < PG3G_EM_COSINE || d->em.sampler > PG3G_EM_STEERABLE) return Fail(PG3G_ERR_INVALID_ARG, "unknown EM sampler");
+    }
+    if (d->camera.res_x == 0 || d->camera.res_y == 0) return Fail(PG3G_ERR_INVALID_ARG, "camera resolution is zero");
+    if ((uint64_t)d->camera.res_x * d->camera.res_y > (1ull << 28)) return Fail(PG3G_ERR_INVALID_ARG, "resolution too large");
+    return PG3G_OK;
+}
+
+int ValidateParams(const pg3g_render_params &p) {
+    if (p.algorithm < PG3G_ALG_DIRECT_BSDF || p.algorithm > PG3G_ALG_PATH_TRACING)
+        return Fail(PG3G_ERR_UNSUPPORTED, "algorithm is not on the GPU path (supported: dbs dlsa dlss dmis ptn pt)");
+    if (p.algorithm == PG3G_ALG_PATH_TRACING) {
+        if (std::lround(p.splitting_budget) != 1 || p.splitting_budget != 1.f)
+            return Fail(PG3G_ERR_UNSUPPORTED, "the GPU path tracer supports splitting budget 1 (-sb 1)");
+        if (p.indirect_clipping > 0.f)
+            return Fail(PG3G_ERR_UNSUPPORTED, "the GPU path tracer supports -iic 0 (indirect clipping off)");
+    }
+    if (p.max_path_length > 0 && p.min_path_length > p.max_path_length)
+        return Fail(PG3G_ERR_INVALID_ARG, "min path length exceeds max path length");
+    return PG3G_OK;
+}
+
+void ApplyParams(SceneView &v, const pg3g_render_params &p) {
+    v.algorithm = p.algorithm;
+    v.minPL = p.min_path_length;
+    v.maxPL = p.max_path_length;
+    v.iic = p.indirect_clipping;
+    v.k0 = (uint32_t)p.seed;
+    v.k1 = (uint32_t)(p.seed >> 32);
+    v.sceneId = p.scene_id & 0xFFFFu;
+}
+
+SceneSlot *GetSlot(pg3g_context *ctx, uint32_t slot) {
+    if (!ctx || slot >= ctx->slots.size() || !ctx->slots[slot] || !ctx->slots[slot]->valid) return nullptr;
+    return ctx->slots[slot].get();
+}
+
+int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
+    if (int rc = ValidateDesc(d)) return rc;
+    if (int rc = ValidateParams(d->params)) return rc;
+    PG3_HIP(hipSetDevice(ctx->device));
+    if (slotIdx >= PG3G_MAX_SCENES) return Fail(PG3G_ERR_INVALID_ARG, "scene slot out of range");
+    if (ctx->slots.size() <= slotIdx) ctx->slots.resize(slotIdx + 1);
+    if (!ctx->slots[slotIdx]) ctx->slots[slotIdx].reset(new (std::nothrow) SceneSlot());
+    if (!ctx->slots[slotIdx]) return Fail(PG3G_ERR_OUT_OF_MEMORY, "host allocation failed");
+    SceneSlot &S = *ctx->slots[slotIdx];
+    S.valid = false;
+    hipStream_t st = ctx->stream;
+    SceneView &v = S.view;
+    v = SceneView{};
+
+    std::vector<DevTri> tris(d->num_triangles);
+    for (uint32_t i = 0; i < d->num_triangles; ++i) {
+        DevTri &t = tris[i];
+        std::memset(&t, 0, sizeof(t));
+        const float *p = d->triangles + i * 9;
+        for (int k = 0; k < 3; ++k) { t.p0[k] = p[k]; t.p1[k] = p[3 + k]; t.p2[k] = p[6 + k]; }
+        const F3 n = Normalize(Cross(mk(p[3] - p[0], p[4] - p[1], p[5] - p[2]), mk(p[6] - p[0], p[7] - p[1], p[8] - p[2])));
+        t.n[0] = n.x; t.n[1] = n.y; t.n[2] = n.z;
+        t.mat = d->triangle_mat[i];
+    }
+    std::vector<DevSphere> sphs(d->num_spheres);
+    for (uint32_t i = 0; i < d->num_spheres; ++i) {
+        std::memset(&sphs[i], 0, sizeof(DevSphere));
+        for (int k = 0; k < 4; ++k) (k < 3 ? sphs[i].c[k] : sphs[i].r) = d->spheres[i * 4 + k];
+        sphs[i].mat = d->sphere_mat[i];
+    }
+    std::vector<DevMaterial> mats(d->num_materials);
+    for (uint32_t i = 0; i < d->num_materials; ++i) mats[i] = ToDev(d->materials[i]);
+    std::vector<DevLight> lights(d->num_lights);
+    v.bgLight = -1;
+    for (uint32_t i = 0; i < d->num_lights; ++i) {
+        const pg3g_light &l = d->lights[i];
+        DevLight &L = lights[i];
+        std::memset(&L, 0, sizeof(L));
+        L.type = l.type;
+        const F3 pw = mk(l.power[0], l.power[1], l.power[2]);
+        if (l.type == PG3G_LIGHT_AREA) {                     // AreaLight ctor + SetPower, lights.hxx:62-112
+            L.p0 = mk(l.p0[0], l.p0[1], l.p0[2]);
+            L.e1 = mk(l.p1[0], l.p1[1], l.p1[2]) - L.p0;
+            L.e2 = mk(l.p2[0], l.p2[1], l.p2[2]) - L.p0;
+            const F3 n = Cross(L.e1, L.e2);
+            const float len = Length(n);
+            if (!(len > 0.f)) return Fail(PG3G_ERR_INVALID_ARG, "degenerate area light");
+            L.invArea = 2.f / len;
+            L.n = FrameFromZ(n / Length(n)).z;
+            L.radiance = pw * (L.invArea / kPiF);
+        } else if (l.type == PG3G_LIGHT_POINT) {             // PointLight::SetPower, lights.hxx:242-245
+            L.p0 = mk(l.p0[0], l.p0[1], l.p0[2]);
+            L.radiance = pw / (4 * kPiF);
+        } else {
+            L.radiance = pw;
+            v.bgLight = (int32_t)i;
+        }
+    }
+    PG3_HIP(Upload(S.tris, tris.data(), tris.size(), st));
+    PG3_HIP(Upload(S.sphs, sphs.data(), sphs.size(), st));
+    PG3_HIP(Upload(S.mats, mats.data(), mats.size(), st));
+    PG3_HIP(Upload(S.lights, lights.data(), lights.size(), st));
+    v.tris = S.tris.as<DevTri>();
+    v.sphs = S.sphs.as<DevSphere>();
+    v.mats = S.mats.as<DevMaterial>();
+    v.lights = S.lights.as<DevLight>();
+    v.numTris = d->num_triangles;
+    v.numSphs = d->num_spheres;
+    v.numLights = d->num_lights;
+
+    // Environment map + sampling tables
+    DevEm &E = v.em;
+    S.hasSteer = false;
+    if (d->em.width > 0 && v.bgLight >= 0) {
+        E.w = d->em.width;
+        E.h = d->em.height;
+        E.bilinear = d->em.bilinear ? 1 : 0;
+        E.sampler = d->em.sampler;
+        E.plan2ang = 1.0f / (2.0f * kPiF * kPiF);
+        std::vector<float4> tex((size_t)E.w * E.h);
+        for (size_t i = 0; i < tex.size(); ++i)
+            tex[i] = make_float4(d->em.texels[i * 3], d->em.texels[i * 3 + 1], d->em.texels[i * 3 + 2], 0.f);
+        PG3_HIP(Upload(S.texels, tex.data(), tex.size(), st));
+        E.texels = S.texels.as<float4>();
+        if (E.sampler == PG3G_EM_SIMPLE || E.sampler == PG3G_EM_STEERABLE) {
+            // the simple sampler's CDF is also used by EstimateIrradiance's MC fallback
+            PG3_HIP(S.margCdf.Alloc(sizeof(float) * (E.h + 1)));
+            PG3_HIP(S.condCdf.Alloc(sizeof(float) * (size_t)E.h * (E.w + 1)));
+            PG3_HIP(S.rowInt.Alloc(sizeof(float) * E.h));
+            E.margCdf = S.margCdf.as<float>();
+            E.condCdf = S.condCdf.as<float>();
+            PG3_HIP(LaunchCdfBuild(E, S.condCdf.as<float>(), S.margCdf.as<float>(), S.rowInt.as<float>(), st));
+        }
+        if (E.sampler == PG3G_EM_STEERABLE) {
+            std::string err;
+            const uint32_t mn = d->em.st_min_subdiv_level >= 0 ? (uint32_t)d->em.st_min_subdiv_level : 4u;
+            const uint32_t mx = d->em.st_max_subdiv_level >= 0 ? (uint32_t)d->em.st_max_subdiv_level : 9u;
+            if (!BuildSteerable(d->em.texels, E.w, E.h, E.bilinear != 0, d->em.st_max_approx_error, mn, mx, S.steer, err))
+                return Fail(PG3G_ERR_INVALID_ARG, err);
+            S.hasSteer = true;
+            PG3_HIP(Upload(S.nodeW, S.steer.nodeW.data(), S.steer.nodeW.size(), st));
+            PG3_HIP(Upload(S.leafVerts, S.steer.leafVerts.data(), S.steer.leafVerts.size(), st));
+            E.nodeW = S.nodeW.as<float>();
+            E.leafVerts = S.leafVerts.as<float>();
+            E.numLevels = S.steer.numLevels;
+            for (uint32_t l = 0; l < S.steer.numLevels; ++l) {
+                E.levelBase[l] = S.steer.levelBase[l];
+                E.levelCount[l] = S.steer.levelCount[l];
+            }
+            for (int j = 0; j < 9; ++j) E.rootW[j] = S.steer.rootW[j];
+        }
+    }
+
+    CameraMatrix(d->camera, v.r2w);
+    v.camPos = mk(d->camera.position[0], d->camera.position[1], d->camera.position[2]);
+    v.resX = d->camera.res_x;
+    v.resY = d->camera.res_y;
+    S.params = d->params;
+    ApplyParams(v, d->params);
+
+    const size_t fbBytes = sizeof(double) * 3 * (size_t)v.resX * v.resY;
+    PG3_HIP(S.fb.Alloc(fbBytes));
+    PG3_HIP(hipMemsetAsync(S.fb.p, 0, fbBytes, st));
+    S.iterations = 0;
+    PG3_HIP(hipStreamSynchronize(st));
+    S.valid = true;
+    return PG3G_OK;
+}
+
+// Host staging for the function-level entry points
+struct Staging {
+    pg3g_context *ctx;
+    size_t next = 0;
+    void *Get(size_t bytes, hipError_t &err) {
+        DevBuf &b = ctx->scratch[next++];
+        err = b.Alloc(bytes);
+        return b.p;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *pg3g_last_error(void) { return g_lastError.c_str(); }
+
+int pg3g_create(pg3g_context **ctx, int device) {
+    if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    *ctx = nullptr;
+    int count = 0;
+    PG3_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return Fail(PG3G_ERR_INVALID_ARG, "no such HIP device");
+    PG3_HIP(hipSetDevice(device));
+    std::unique_ptr<pg3g_context> c(new (std::nothrow) pg3g_context());
+    if (!c) return Fail(PG3G_ERR_OUT_OF_MEMORY, "host allocation failed");
+    c->device = device;
+    PG3_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    PG3_HIP(c->counters.Alloc(sizeof(unsigned long long) * 2));
+    PG3_HIP(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned long long) * 2, c->stream));
+    PG3_HIP(hipStreamSynchronize(c->stream));
+    *ctx = c.release();
+    return PG3G_OK;
+}
+
+int pg3g_destroy(pg3g_context *ctx) {
+    if (!ctx) return PG3G_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->slots.clear();
+    ctx->counters.Free();
+    for (auto &b : ctx->scratch) b.Free();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PG3G_OK;
+}
+
+int pg3g_upload_scene(pg3g_context *ctx, const pg3g_scene_desc *desc) { return pg3g_upload_scene_slot(ctx, 0, desc); }
+
+int pg3g_upload_scene_slot(pg3g_context *ctx, uint32_t slot, const pg3g_scene_desc *desc) {
+    if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    return UploadScene(ctx, slot, desc);
+}
+
+int pg3g_set_params(pg3g_context *ctx, uint32_t slot, const pg3g_render_params *p) {
+    SceneSlot *S = GetSlot(ctx, slot);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
+    if (!p) return Fail(PG3G_ERR_INVALID_ARG, "params is null");
+    if (int rc = ValidateParams(*p)) return rc;
+    S->params = *p;
+    ApplyParams(S->view, *p);
+    return PG3G_OK;
+}
+
+int pg3g_set_kernel_mode(pg3g_context *ctx, int mode) {
+    if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    if (mode != PG3G_KERNEL_MEGAKERNEL && mode != PG3G_KERNEL_WAVEFRONT) return Fail(PG3G_ERR_INVALID_ARG, "unknown kernel mode");
+    ctx->kernelMode = mode;
+    return PG3G_OK;
+}
+
+int pg3g_render(pg3g_context *ctx, uint32_t first_iter, uint32_t n_iter) { return pg3g_render_slot(ctx, 0, first_iter, n_iter); }
+
+int pg3g_render_slot(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter) {
+    SceneSlot *S = GetSlot(ctx, slot);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
+    if ((uint64_t)first_iter + n_iter > 0xFFFFFFFFull) return Fail(PG3G_ERR_INVALID_ARG, "iteration range overflows");
+    PG3_HIP(hipSetDevice(ctx->device));
+    if (ctx->kernelMode == PG3G_KERNEL_WAVEFRONT) {
+        PG3_HIP(LaunchWavefront(S->view, first_iter, n_iter, S->fb.as<double>(),
+                                ctx->counters.as<unsigned long long>(), nullptr, ctx->stream));
+    } else {
+        PG3_HIP(LaunchMegakernel(S->view, first_iter, n_iter, S->fb.as<double>(), ctx->counters.as<unsigned long long>(),
+                                 ctx->stream));
+    }
+    S->iterations += n_iter;
+    return PG3G_OK;
+}
+
+int pg3g_render_timed(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, float *elapsed_ms) {
+    if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    PG3_HIP(hipSetDevice(ctx->device));
+    hipEvent_t e0, e1;
+    PG3_HIP(hipEventCreate(&e0));
+    PG3_HIP(hipEventCreate(&e1));
+    PG3_HIP(hipEventRecord(e0, ctx->stream));
+    const int rc = pg3g_render_slot(ctx, slot, first_iter, n_iter);
+    hipError_t e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != PG3G_OK) return rc;
+    PG3_HIP(e);
+    if (elapsed_ms) *elapsed_ms = ms;
+    return PG3G_OK;
+}
+
+int pg3g_read_framebuffer(pg3g_context *ctx, double *sum_rgb, uint64_t *iterations) {
+    return pg3g_read_framebuffer_slot(ctx, 0, sum_rgb, iterations);
+}
+
+int pg3g_read_framebuffer_slot(pg3g_context *ctx, uint32_t slot, double *sum_rgb, uint64_t *iterations) {
+    SceneSlot *S = GetSlot(ctx, slot);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
+    PG3_HIP(hipSetDevice(ctx->device));
+    if (sum_rgb) PG3_HIP(hipMemcpyAsync(sum_rgb, S->fb.p, sizeof(double) * 3 * (size_t)S->view.resX * S->view.resY,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    if (iterations) *iterations = S->iterations;
+    return PG3G_OK;
+}
+
+int pg3g_clear_framebuffer(pg3g_context *ctx, uint32_t slot) {
+    SceneSlot *S = GetSlot(ctx, slot);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(hipMemsetAsync(S->fb.p, 0, S->fb.bytes, ctx->stream));
+    S->iterations = 0;
+    return PG3G_OK;
+}
+
+int pg3g_framebuffer_device_ptr(pg3g_context *ctx, uint32_t slot, double **fb_dev) {
+    SceneSlot *S = GetSlot(ctx, slot);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
+    if (!fb_dev) return Fail(PG3G_ERR_INVALID_ARG, "fb_dev is null");
+    *fb_dev = S->fb.as<double>();
+    return PG3G_OK;
+}
+
+int pg3g_synchronize(pg3g_context *ctx) {
+    if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    return PG3G_OK;
+}
+
+int pg3g_read_counters(pg3g_context *ctx, uint64_t *segments, uint64_t *camera_samples) {
+    if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    unsigned long long h[2] = {0, 0};
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(hipMemcpyAsync(h, ctx->counters.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    if (segments) *segments = h[0];
+    if (camera_samples) *camera_samples = h[1];
+    return PG3G_OK;
+}
+
+int pg3g_reset_counters(pg3g_context *ctx) {
+    if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(hipMemsetAsync(ctx->counters.p, 0, ctx->counters.bytes, ctx->stream));
+    return PG3G_OK;
+}
+
+// ---- function-level entry points ----------------------------------------------------------------
+int pg3g_bsdf_eval_batch(pg3g_context *ctx, const pg3g_material *mat, uint32_t n, const float *wil, const float *wol,
+                         float *f, float *pdf, float *comp_prob) {
+    if (!ctx || !mat || (n && (!wil || !wol || !f || !pdf || !comp_prob))) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return PG3G_OK;
+    PG3_HIP(hipSetDevice(ctx->device));
+    Staging sg{ctx};
+    hipError_t e1, e2, e3, e4, e5;
+    float *dWil = (float *)sg.Get(12ull * n, e1), *dWol = (float *)sg.Get(12ull * n, e2), *dF = (float *)sg.Get(12ull * n, e3);
+    float *dPdf = (float *)sg.Get(4ull * n, e4), *dComp = (float *)sg.Get(4ull * n, e5);
+    PG3_HIP(e1); PG3_HIP(e2); PG3_HIP(e3); PG3_HIP(e4); PG3_HIP(e5);
+    PG3_HIP(hipMemcpyAsync(dWil, wil, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dWol, wol, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(LaunchBsdfEval(ToDev(*mat), n, dWil, dWol, dF, dPdf, dComp, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(f, dF, 12ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(pdf, dPdf, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(comp_prob, dComp, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    return PG3G_OK;
+}
+
+int pg3g_bsdf_sample_batch(pg3g_context *ctx, const pg3g_material *mat, uint32_t n, const float *wol, const float *u,
+                           float *wil, float *f, float *pdf, float *comp_prob) {
+    if (!ctx || !mat || (n && (!wol || !u || !wil || !f || !pdf || !comp_prob))) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return PG3G_OK;
+    PG3_HIP(hipSetDevice(ctx->device));
+    Staging sg{ctx};
+    hipError_t e[6];
+    float *dWol = (float *)sg.Get(12ull * n, e[0]), *dU = (float *)sg.Get(12ull * n, e[1]), *dWil = (float *)sg.Get(12ull * n, e[2]);
+    float *dF = (float *)sg.Get(12ull * n, e[3]), *dPdf = (float *)sg.Get(4ull * n, e[4]), *dComp = (float *)sg.Get(4ull * n, e[5]);
+    for (hipError_t x : e) PG3_HIP(x);
+    PG3_HIP(hipMemcpyAsync(dWol, wol, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dU, u, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(LaunchBsdfSample(ToDev(*mat), n, dWol, dU, dWil, dF, dPdf, dComp, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(wil, dWil, 12ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(f, dF, 12ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(pdf, dPdf, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(comp_prob, dComp, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    return PG3G_OK;
+}
+
+int pg3g_em_sample_batch(pg3g_context *ctx, uint32_t n, const float *normals, const int32_t *side_flags, const float *u,
+                         float *dir, float *pdf, float *radiance_cos, int32_t *ok) {
+    SceneSlot *S = GetSlot(ctx, 0);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot 0");
+    if (S->view.em.w == 0) return Fail(PG3G_ERR_INVALID_ARG, "scene has no EM image");
+    if (n && (!normals || !side_flags || !u || !dir || !pdf || !radiance_cos || !ok)) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return PG3G_OK;
+    PG3_HIP(hipSetDevice(ctx->device));
+    Staging sg{ctx};
+    hipError_t e[7];
+    float *dN = (float *)sg.Get(12ull * n, e[0]);
+    int32_t *dS = (int32_t *)sg.Get(4ull * n, e[1]);
+    float *dU = (float *)sg.Get(12ull * n, e[2]), *dD = (float *)sg.Get(12ull * n, e[3]), *dP = (float *)sg.Get(4ull * n, e[4]);
+    float *dR = (float *)sg.Get(12ull * n, e[5]);
+    int32_t *dOk = (int32_t *)sg.Get(4ull * n, e[6]);
+    for (hipError_t x : e) PG3_HIP(x);
+    PG3_HIP(hipMemcpyAsync(dN, normals, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dS, side_flags, 4ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dU, u, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(LaunchEmSample(S->view, n, dN, dS, dU, dD, dP, dR, dOk, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dir, dD, 12ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(pdf, dP, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(radiance_cos, dR, 12ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(ok, dOk, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    return PG3G_OK;
+}
+
+int pg3g_em_pdf_batch(pg3g_context *ctx, uint32_t n, const float *dirs, const float *normals, const int32_t *side_flags,
+                      float *pdf) {
+    SceneSlot *S = GetSlot(ctx, 0);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot 0");
+    if (S->view.em.w == 0) return Fail(PG3G_ERR_INVALID_ARG, "scene has no EM image");
+    if (n && (!dirs || !normals || !side_flags || !pdf)) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return PG3G_OK;
+    PG3_HIP(hipSetDevice(ctx->device));
+    Staging sg{ctx};
+    hipError_t e[4];
+    float *dD = (float *)sg.Get(12ull * n, e[0]), *dN = (float *)sg.Get(12ull * n, e[1]);
+    int32_t *dS = (int32_t *)sg.Get(4ull * n, e[2]);
+    float *dP = (float *)sg.Get(4ull * n, e[3]);
+    for (hipError_t x : e) PG3_HIP(x);
+    PG3_HIP(hipMemcpyAsync(dD, dirs, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dN, normals, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dS, side_flags, 4ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(LaunchEmPdf(S->view, n, dD, dN, dS, dP, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(pdf, dP, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    return PG3G_OK;
+}
+
+int pg3g_em_eval_batch(pg3g_context *ctx, uint32_t n, const float *dirs, float *rgb) {
+    SceneSlot *S = GetSlot(ctx, 0);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot 0");
+    if (S->view.em.w == 0) return Fail(PG3G_ERR_INVALID_ARG, "scene has no EM image");
+    if (n && (!dirs || !rgb)) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return PG3G_OK;
+    PG3_HIP(hipSetDevice(ctx->device));
+    Staging sg{ctx};
+    hipError_t e[2];
+    float *dD = (float *)sg.Get(12ull * n, e[0]), *dR = (float *)sg.Get(12ull * n, e[1]);
+    for (hipError_t x : e) PG3_HIP(x);
+    PG3_HIP(hipMemcpyAsync(dD, dirs, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(LaunchEmEval(S->view, n, dD, dR, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(rgb, dR, 12ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    return PG3G_OK;
+}
+
+int pg3g_steerable_info(pg3g_context *ctx, uint32_t *num_leaves, uint32_t *num_vertices, uint32_t *num_levels) {
+    SceneSlot *S = GetSlot(ctx, 0);
+    if (!S || !S->hasSteer) return Fail(PG3G_ERR_NO_SCENE, "slot 0 has no steerable sampler");
+    if (num_leaves) *num_leaves = S->steer.numLeaves;
+    if (num_vertices) *num_vertices = S->steer.numVertices;
+    if (num_levels) *num_levels = S->steer.numLevels;
+    return PG3G_OK;
+}
+
+int pg3g_steerable_leaves(pg3g_context *ctx, float *leaf_dirs) {
+    SceneSlot *S = GetSlot(ctx, 0);
+    if (!S || !S->hasSteer) return Fail(PG3G_ERR_NO_SCENE, "slot 0 has no steerable sampler");
+    if (!leaf_dirs) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    for (uint32_t i = 0; i < S->steer.numLeaves; ++i)
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) leaf_dirs[i * 9 + k * 3 + c] = S->steer.leafVerts[(size_t)i * 36 + k * 12 + c];
+    return PG3G_OK;
+}
+
+int pg3g_steerable_root_weight(pg3g_context *ctx, float *weight9) {
+    SceneSlot *S = GetSlot(ctx, 0);
+    if (!S || !S->hasSteer) return Fail(PG3G_ERR_NO_SCENE, "slot 0 has no steerable sampler");
+    if (!weight9) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    for (int j = 0; j < 9; ++j) weight9[j] = S->steer.rootW[j];
+    return PG3G_OK;
+}
+
+}  // extern "C"

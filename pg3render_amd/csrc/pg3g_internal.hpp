@@ -1,0 +1,42 @@
+// Internal declarations shared by the host side of libpg3g (C ABI, table builders) and the
+// kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dev_scene.hpp"
+
+namespace pg3 {
+
+hipError_t LaunchMegakernel(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
+                            unsigned long long *counters, hipStream_t stream);
+hipError_t LaunchWavefront(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
+                           unsigned long long *counters, void *queues, hipStream_t stream);
+hipError_t LaunchBsdfEval(const DevMaterial &m, uint32_t n, const float *wil, const float *wol, float *f, float *pdf,
+                          float *comp, hipStream_t s);
+hipError_t LaunchBsdfSample(const DevMaterial &m, uint32_t n, const float *wol, const float *u, float *wil, float *f,
+                            float *pdf, float *comp, hipStream_t s);
+hipError_t LaunchEmSample(const SceneView &S, uint32_t n, const float *nrm, const int32_t *sides, const float *u,
+                          float *dir, float *pdf, float *rc, int32_t *ok, hipStream_t s);
+hipError_t LaunchEmPdf(const SceneView &S, uint32_t n, const float *dirs, const float *nrm, const int32_t *sides,
+                       float *pdf, hipStream_t s);
+hipError_t LaunchEmEval(const SceneView &S, uint32_t n, const float *dirs, float *rgb, hipStream_t s);
+hipError_t LaunchCdfBuild(const DevEm &E, float *condCdf, float *margCdf, float *rowIntegral, hipStream_t s);
+
+// Host-built steerable sampler tables (em_steerable_sampler.hxx build path), flattened:
+struct SteerableTables {
+    uint32_t numLeaves = 0, numVertices = 0, numLevels = 0;
+    std::vector<uint32_t> levelCount, levelBase;   // levels[0] = leaves, last = root
+    std::vector<float> nodeW;                      // 12 floats per node, all levels
+    std::vector<float> leafVerts;                  // 36 floats per leaf (3 x (dir, SH weight))
+    float rootW[9];
+};
+// Builds the tables from an RGB image (w*h*3 floats) with tent or box filtering.
+bool BuildSteerable(const float *texels, uint32_t w, uint32_t h, bool bilinear, float maxErr, uint32_t minLevel,
+                    uint32_t maxLevel, SteerableTables &out, std::string &err);
+
+}  // namespace pg3

@@ -1,0 +1,275 @@
+// gfx950 kernels of the path-tracing hot path and their launchers.
+#include <hip/hip_runtime.h>
+
+#include "pg3g_internal.hpp"
+#include "pt_kernels.hpp"
+
+namespace pg3 {
+
+// ---------------------------------------------------------------------------------------------
+// Wave-level helpers (64-wide wavefronts)
+__device__ __forceinline__ uint32_t WaveSum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused path-tracing kernel: one thread traces `itersPerThread` consecutive iterations of one pixel
+// (PathTracerBase::RunIteration, path_tracer_base.hxx:57-90, over a range of iterations) and adds
+// its FP64 sum to the framebuffer (frame_buffer.hxx:30-46).  Consecutive lanes are consecutive
+// pixels of one 8x8 tile so a wavefront traces spatially coherent rays.
+__global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t firstIter, uint32_t nIter,
+                                                         uint32_t itersPerThread, uint32_t numChunks,
+                                                         double *__restrict__ fb,
+                                                         unsigned long long *__restrict__ counters) {
+    const uint32_t npix = S.resX * S.resY;
+    const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
+    const uint32_t numTiles = tilesX * tilesY;
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = (uint32_t)(gid & 63u);
+    const uint64_t waveId = gid >> 6;
+    const uint32_t tile = (uint32_t)(waveId % numTiles);
+    const uint32_t chunk = (uint32_t)(waveId / numTiles);
+    const uint32_t x = (tile % tilesX) * 8u + (lane & 7u);
+    const uint32_t y = (tile / tilesX) * 8u + (lane >> 3);
+    uint32_t segments = 0, samples = 0;
+    if (chunk < numChunks && x < S.resX && y < S.resY) {
+        const uint32_t pixel = y * S.resX + x;
+        const uint32_t itBegin = firstIter + chunk * itersPerThread;
+        const uint32_t itEnd = min(firstIter + nIter, itBegin + itersPerThread);
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        for (uint32_t it = itBegin; it < itEnd; ++it) {
+            const F3 L = RadianceSample(S, pixel, it, segments);
+            acc0 += L.x;
+            acc1 += L.y;
+            acc2 += L.z;
+            ++samples;
+        }
+        if (itEnd > itBegin) {
+            atomicAdd(fb + (size_t)pixel * 3 + 0, acc0);
+            atomicAdd(fb + (size_t)pixel * 3 + 1, acc1);
+            atomicAdd(fb + (size_t)pixel * 3 + 2, acc2);
+        }
+    }
+    (void)npix;
+    const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
+    if (lane == 0 && counters) {
+        atomicAdd(counters + 0, (unsigned long long)segSum);
+        atomicAdd(counters + 1, (unsigned long long)smpSum);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Function-level kernels (tier-1 parity of BSDF and EM functions)
+__global__ void BsdfEvalKernel(DevMaterial m, uint32_t n, const float *__restrict__ wil, const float *__restrict__ wol,
+                               float *__restrict__ f, float *__restrict__ pdf, float *__restrict__ comp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    BsdfRec r;
+    EvalBsdf(m, mk(wil[i * 3], wil[i * 3 + 1], wil[i * 3 + 2]), mk(wol[i * 3], wol[i * 3 + 1], wol[i * 3 + 2]), r);
+    f[i * 3] = r.att.x; f[i * 3 + 1] = r.att.y; f[i * 3 + 2] = r.att.z;
+    pdf[i] = r.pdfW; comp[i] = r.compProb;
+}
+__global__ void BsdfSampleKernel(DevMaterial m, uint32_t n, const float *__restrict__ wol, const float *__restrict__ u,
+                                 float *__restrict__ wil, float *__restrict__ f, float *__restrict__ pdf,
+                                 float *__restrict__ comp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    BsdfRec r;
+    SampleBsdf(m, mk(wol[i * 3], wol[i * 3 + 1], wol[i * 3 + 2]), u[i * 3], u[i * 3 + 1], u[i * 3 + 2], r);
+    wil[i * 3] = r.wil.x; wil[i * 3 + 1] = r.wil.y; wil[i * 3 + 2] = r.wil.z;
+    f[i * 3] = r.att.x; f[i * 3 + 1] = r.att.y; f[i * 3 + 2] = r.att.z;
+    pdf[i] = r.pdfW; comp[i] = r.compProb;
+}
+__global__ void EmSampleKernel(SceneView S, uint32_t n, const float *__restrict__ nrm, const int32_t *__restrict__ sides,
+                               const float *__restrict__ u, float *__restrict__ dir, float *__restrict__ pdf,
+                               float *__restrict__ rc, int32_t *__restrict__ ok) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Frame fr = FrameFromZ(mk(nrm[i * 3], nrm[i * 3 + 1], nrm[i * 3 + 2]));
+    F3 d = mk(0.f), radCos = mk(0.f);
+    float p = 0.f;
+    const bool good = EmSample(S.em, fr, (uint32_t)sides[i], u[i * 3], u[i * 3 + 1], u[i * 3 + 2], d, p, radCos);
+    ok[i] = good ? 1 : 0;
+    dir[i * 3] = d.x; dir[i * 3 + 1] = d.y; dir[i * 3 + 2] = d.z;
+    pdf[i] = p;
+    rc[i * 3] = radCos.x; rc[i * 3 + 1] = radCos.y; rc[i * 3 + 2] = radCos.z;
+}
+__global__ void EmPdfKernel(SceneView S, uint32_t n, const float *__restrict__ dirs, const float *__restrict__ nrm,
+                            const int32_t *__restrict__ sides, float *__restrict__ pdf) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Frame fr = FrameFromZ(mk(nrm[i * 3], nrm[i * 3 + 1], nrm[i * 3 + 2]));
+    pdf[i] = EmPdf(S.em, mk(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]), fr, (uint32_t)sides[i]);
+}
+__global__ void EmEvalKernel(SceneView S, uint32_t n, const float *__restrict__ dirs, float *__restrict__ rgb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const F3 v = EmEval(S.em, mk(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]));
+    rgb[i * 3] = v.x; rgb[i * 3 + 1] = v.y; rgb[i * 3 + 2] = v.z;
+}
+
+// ---------------------------------------------------------------------------------------------
+// EM luminance CDF build (em_simple_spherical_sampler.hxx:117-152, distribution.hxx:45-74,
+// 484-506): f(x,y) = sin(theta_mid(y)) * AveragePixelLuminance(x,y), per-row FP64 prefix sums of
+// f/w, normalised rows stored as FP32; the marginal is built from the FP32 row integrals.
+// One 256-thread workgroup per row; the row is scanned in 256-element FP64 tiles.
+__device__ __forceinline__ float AvgPixelLum(const DevEm &E, uint32_t x, uint32_t y) {   // em_image.hxx:152-183
+    if (!E.bilinear) return Luminance(Texel(E, x, y));
+    uint32_t x0, y0, x2, y2;
+    EmNorm(E, (int32_t)x - 1, (int32_t)y - 1, x0, y0);
+    EmNorm(E, (int32_t)x + 1, (int32_t)y + 1, x2, y2);
+    const float e = 1 / 24.f, c = 2 / 3.f;
+    float s = Luminance(Texel(E, x0, y0)) * e;
+    s = s + Luminance(Texel(E, x0, y)) * e;
+    s = s + Luminance(Texel(E, x0, y2)) * e;
+    s = s + Luminance(Texel(E, x, y0)) * e;
+    s = s + Luminance(Texel(E, x, y)) * c;
+    s = s + Luminance(Texel(E, x, y2)) * e;
+    s = s + Luminance(Texel(E, x2, y0)) * e;
+    s = s + Luminance(Texel(E, x2, y)) * e;
+    s = s + Luminance(Texel(E, x2, y2)) * e;
+    return s;
+}
+
+__device__ __forceinline__ double BlockScanInclusive(double v, double *lds) {
+    // Hillis-Steele over 64 lanes with shuffles, then across the 4 waves through LDS.
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double t = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    if (lane == 63) lds[wave] = v;
+    __syncthreads();
+    double off = 0.0;
+    for (uint32_t w = 0; w < wave; ++w) off += lds[w];
+    __syncthreads();
+    return v + off;
+}
+
+__global__ void __launch_bounds__(256) CdfRowsKernel(DevEm E, float *__restrict__ condCdf, float *__restrict__ rowIntegral) {
+    __shared__ double lds[4];
+    __shared__ double carry;
+    const uint32_t row = blockIdx.x;
+    const uint32_t w = E.w;
+    float *cdf = condCdf + (size_t)row * (w + 1);
+    const float sinMid = SinMidTheta(E.h, row);
+    if (threadIdx.x == 0) carry = 0.0;
+    __syncthreads();
+    // pass 1: total (FP64)
+    for (uint32_t base = 0; base < w; base += 256) {
+        const uint32_t x = base + threadIdx.x;
+        const double v = (x < w) ? (double)(sinMid * AvgPixelLum(E, x, row)) / (double)w : 0.0;
+        const double incl = BlockScanInclusive(v, lds);
+        const double c = carry;
+        __syncthreads();
+        if (threadIdx.x == 255) carry = c + incl;
+        __syncthreads();
+    }
+    const double total = carry;
+    __syncthreads();
+    if (threadIdx.x == 0) carry = 0.0;
+    __syncthreads();
+    // pass 2: normalised prefix sums
+    for (uint32_t base = 0; base < w; base += 256) {
+        const uint32_t x = base + threadIdx.x;
+        const double v = (x < w) ? (double)(sinMid * AvgPixelLum(E, x, row)) / (double)w : 0.0;
+        const double incl = BlockScanInclusive(v, lds);
+        const double c = carry;
+        if (x < w) cdf[x + 1] = (total == 0.0) ? (float)(x + 1) / w : (float)((c + incl) / total);
+        __syncthreads();
+        if (threadIdx.x == 255) carry = c + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cdf[0] = 0.f;
+        rowIntegral[row] = (float)total;
+    }
+}
+__global__ void __launch_bounds__(256) CdfMarginalKernel(uint32_t h, const float *__restrict__ rowIntegral,
+                                                         float *__restrict__ margCdf) {
+    __shared__ double lds[4];
+    __shared__ double carry;
+    if (threadIdx.x == 0) carry = 0.0;
+    __syncthreads();
+    for (uint32_t base = 0; base < h; base += 256) {
+        const uint32_t y = base + threadIdx.x;
+        const double v = (y < h) ? (double)rowIntegral[y] / (double)h : 0.0;
+        const double incl = BlockScanInclusive(v, lds);
+        const double c = carry;
+        __syncthreads();
+        if (threadIdx.x == 255) carry = c + incl;
+        __syncthreads();
+    }
+    const double total = carry;
+    __syncthreads();
+    if (threadIdx.x == 0) carry = 0.0;
+    __syncthreads();
+    for (uint32_t base = 0; base < h; base += 256) {
+        const uint32_t y = base + threadIdx.x;
+        const double v = (y < h) ? (double)rowIntegral[y] / (double)h : 0.0;
+        const double incl = BlockScanInclusive(v, lds);
+        const double c = carry;
+        if (y < h) margCdf[y + 1] = (total == 0.0) ? (float)(y + 1) / h : (float)((c + incl) / total);
+        __syncthreads();
+        if (threadIdx.x == 255) carry = c + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) margCdf[0] = 0.f;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers (host)
+hipError_t LaunchMegakernel(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
+                            unsigned long long *counters, hipStream_t stream) {
+    if (nIter == 0) return hipSuccess;
+    const uint32_t npix = S.resX * S.resY;
+    // Enough lanes to fill 256 CUs several times over; each thread keeps >= 1 iteration.
+    const uint64_t targetThreads = 1u << 21;
+    uint32_t numChunks = (uint32_t)((targetThreads + npix - 1) / npix);
+    numChunks = max(1u, min(numChunks, nIter));
+    const uint32_t ipt = (nIter + numChunks - 1) / numChunks;
+    numChunks = (nIter + ipt - 1) / ipt;
+    const uint32_t tiles = ((S.resX + 7u) / 8u) * ((S.resY + 7u) / 8u);
+    const uint64_t threads = (uint64_t)tiles * 64u * numChunks;
+    const uint32_t blocks = (uint32_t)((threads + 255) / 256);
+    hipLaunchKernelGGL(RenderMegakernel, dim3(blocks), dim3(256), 0, stream, S, firstIter, nIter, ipt, numChunks, fb,
+                       counters);
+    return hipGetLastError();
+}
+
+hipError_t LaunchBsdfEval(const DevMaterial &m, uint32_t n, const float *wil, const float *wol, float *f, float *pdf,
+                          float *comp, hipStream_t s) {
+    hipLaunchKernelGGL(BsdfEvalKernel, dim3((n + 255) / 256), dim3(256), 0, s, m, n, wil, wol, f, pdf, comp);
+    return hipGetLastError();
+}
+hipError_t LaunchBsdfSample(const DevMaterial &m, uint32_t n, const float *wol, const float *u, float *wil, float *f,
+                            float *pdf, float *comp, hipStream_t s) {
+    hipLaunchKernelGGL(BsdfSampleKernel, dim3((n + 255) / 256), dim3(256), 0, s, m, n, wol, u, wil, f, pdf, comp);
+    return hipGetLastError();
+}
+hipError_t LaunchEmSample(const SceneView &S, uint32_t n, const float *nrm, const int32_t *sides, const float *u,
+                          float *dir, float *pdf, float *rc, int32_t *ok, hipStream_t s) {
+    hipLaunchKernelGGL(EmSampleKernel, dim3((n + 255) / 256), dim3(256), 0, s, S, n, nrm, sides, u, dir, pdf, rc, ok);
+    return hipGetLastError();
+}
+hipError_t LaunchEmPdf(const SceneView &S, uint32_t n, const float *dirs, const float *nrm, const int32_t *sides,
+                       float *pdf, hipStream_t s) {
+    hipLaunchKernelGGL(EmPdfKernel, dim3((n + 255) / 256), dim3(256), 0, s, S, n, dirs, nrm, sides, pdf);
+    return hipGetLastError();
+}
+hipError_t LaunchEmEval(const SceneView &S, uint32_t n, const float *dirs, float *rgb, hipStream_t s) {
+    hipLaunchKernelGGL(EmEvalKernel, dim3((n + 255) / 256), dim3(256), 0, s, S, n, dirs, rgb);
+    return hipGetLastError();
+}
+hipError_t LaunchCdfBuild(const DevEm &E, float *condCdf, float *margCdf, float *rowIntegral, hipStream_t s) {
+    hipLaunchKernelGGL(CdfRowsKernel, dim3(E.h), dim3(256), 0, s, E, condCdf, rowIntegral);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(CdfMarginalKernel, dim3(1), dim3(256), 0, s, E.h, rowIntegral, margCdf);
+    return hipGetLastError();
+}
+
+}  // namespace pg3

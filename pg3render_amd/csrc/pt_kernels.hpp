@@ -1,0 +1,339 @@
+// Per-sample radiance estimators of the reference renderers, written as iterative device code.
+// PathTracer::EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with splitting budget 1 is a
+// chain: pixel = emitted(1) + sum_v B_v * (NEE_v + E_v), where B_v is the path throughput up to
+// vertex v, NEE_v the MIS-weighted light sample (path_tracer_base.hxx:350-425) and E_v the
+// MIS-weighted emission found by the BSDF sample of v (path_tracer.hxx:327-369).  The sums are
+// accumulated forward; the indirect clipping of path_tracer.hxx:400-402 needs the whole suffix and
+// is therefore rejected by the host when -iic > 0.
+#pragma once
+
+#include "dev_scene.hpp"
+
+namespace pg3 {
+
+constexpr uint32_t kMaxPickLights = 8;    // per-vertex light-contribution cache kept in registers
+
+// MISWeight2Power, path_tracer_base.hxx:522-541 (sample counts 1)
+PG3_DEV float MisPower(float p1, float p2) {
+    const float q1 = p1 * p1, q2 = p2 * p2;
+    return q1 / (q1 + q2) / 1u;
+}
+
+struct LightCache {            // LightSamplingContext, path_tracer_base.hxx:22-32
+    float est[kMaxPickLights];
+    float sum;
+};
+PG3_DEV void FillLightCache(const SceneView &S, F3 sp, const Frame &fr, uint32_t sides, const RngKey &key,
+                            uint32_t vtx, LightCache &lc) {
+    lc.sum = 0.f;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxPickLights; ++i) {
+        float e = 0.f;
+        if (i < S.numLights) e = EstimateContribution(S, i, sp, fr, sides, key, vtx);
+        lc.est[i] = e;
+        lc.sum += e;
+    }
+}
+// LightPickingProbability, path_tracer_base.hxx:260-313
+PG3_DEV float PickProb(const SceneView &S, const LightCache &lc, uint32_t id) {
+    if (S.numLights == 0) return 0.f;
+    if (S.numLights == 1) return 1.f;
+    float e = 0.f;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxPickLights; ++i) if (i == id) e = lc.est[i];
+    return (lc.sum > 0.f) ? e / lc.sum : 1.f / S.numLights;
+}
+// PickSingleLight, path_tracer_base.hxx:191-257
+PG3_DEV int32_t PickLight(const SceneView &S, const LightCache &lc, float u, float &prob) {
+    const uint32_t n = S.numLights;
+    if (n == 0) return -1;
+    if (n == 1) { prob = 1.f; return 0; }
+    if (lc.sum > 0.f) {
+        const float rv = u * lc.sum;
+        float cdf = 0.f, prev = 0.f;
+        uint32_t li = 0;
+        bool found = false;
+#pragma unroll
+        for (uint32_t i = 0; i < kMaxPickLights; ++i) {
+            if (i < n && !found) {
+                prev = cdf;
+                cdf += lc.est[i];
+                li = i;
+                if (!(rv > cdf)) found = true;
+            }
+        }
+        if (!found) { li = n; prev = cdf; }   // rv beyond the pseudo-CDF (reference loop bound)
+        prob = (found ? (cdf - prev) : 0.f) / lc.sum;
+        return (int32_t)li;
+    }
+    prob = 1.f / n;
+    return min((int32_t)(u * n), (int32_t)(n - 1));
+}
+
+struct PathCounters {
+    uint32_t segments;     // extension rays traced from surface vertices (each with <= 1 shadow ray)
+};
+
+// One sample of PathTracer::EstimateIncomingRadiancePT with -sb 1 (see header comment).
+PG3_DEV F3 RadiancePT(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
+    F3 L = mk(0.f), beta = mk(1.f), eThr = mk(0.f);
+    float tmin = 0.f;
+    bool hasPrev = false, prevFinite = true, computeRefl = true;
+    float prevBsdfPdf = 0.f;
+    F3 prevN = mk(0.f, 0.f, 1.f);
+    uint32_t prevSides = 0;
+    LightCache lc;
+    lc.sum = 0.f;
+    for (uint32_t len = 1;; ++len) {
+        Hit h;
+        h.dist = 1e36f;
+        if (!Intersect(S, org, dir, tmin, h)) {
+            if (len >= S.minPL && S.bgLight >= 0) {                                     // :408-422
+                const F3 Le = BgRadiance(S, dir);
+                if (!hasPrev) {
+                    L += Le;
+                } else if (!IsZero(Le)) {
+                    if (prevFinite) {
+                        const float lp = BgPdf(S, dir, FrameFromZ(prevN), prevSides) * PickProb(S, lc, (uint32_t)S.bgLight);
+                        L += eThr * Le * MisPower(prevBsdfPdf, lp);
+                    } else {
+                        L += eThr * Le;
+                    }
+                }
+            }
+            break;
+        }
+        const F3 sp = org + dir * h.dist;
+        const Frame fr = FrameFromZ(h.n);
+        const F3 wol = ToLocal(fr, -dir);
+        const DevMaterial mat = S.mats[h.mat];
+        if (mat.lightId >= 0 && len >= S.minPL) {                                       // :198-210
+            const DevLight &Lt = S.lights[mat.lightId];
+            F3 Le = mk(0.f);
+            if (Lt.type == kLightArea && wol.z > 0.f) Le = Lt.radiance;
+            if (!hasPrev) {
+                L += Le;
+            } else if (!IsZero(Le)) {                                                   // :327-369
+                if (prevFinite) {
+                    const float lp = AreaEmissionPdf(Lt, sp, org) * PickProb(S, lc, (uint32_t)mat.lightId);
+                    L += eThr * Le * MisPower(prevBsdfPdf, lp);
+                } else {
+                    L += eThr * Le;
+                }
+            }
+        }
+        if (!computeRefl) break;                                                        // :212-238
+        if (ReflectanceZero(mat)) break;
+        if (S.maxPL > 0 && len >= S.maxPL) break;
+        if (len >= 1200u) break;
+        ++segments;
+
+        VertexRng vr;
+        RngVertex(key, len, vr);
+        const uint32_t sides = mat.sides;
+        if (S.numLights > 1) FillLightCache(S, sp, fr, sides, key, len, lc);
+        if ((len + 1) >= S.minPL) {                                                     // :252-265
+            float prob = 1.f;
+            const int32_t li = PickLight(S, lc, vr.u[kDimLightPick], prob);
+            LightSample ls;
+            if (li >= 0 && SampleIllumination(S, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
+                                              vr.u[kDimLight2], ls)) {
+                ls.prob = prob;
+                if (MaxC(ls.sample) > 0.f && !Occluded(S, sp, ls.wig, ls.dist)) {     // :350-425
+                    BsdfRec r;
+                    EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
+                    if (ls.pdfW != kInf) {
+                        const float lightPdf = ls.pdfW * ls.prob;
+                        const float bsdfPdf = r.pdfW * r.compProb;
+                        L += beta * ((ls.sample * r.att * MisPower(lightPdf, bsdfPdf)) / lightPdf);
+                    } else {
+                        L += beta * ((ls.sample * r.att) / (ls.prob * 1u));
+                    }
+                }
+            }
+        }
+        float rr = 1.0f;                                                                // :267-303
+        bool cut = false;
+        if (S.maxPL == 0) {
+            rr = Clampf(RRProb(mat, wol), 0.0f, 0.997f);
+            cut = vr.u[kDimRR] > rr;
+        }
+        BsdfRec r;
+        SampleBsdf(mat, wol, vr.u[kDimBsdf0], vr.u[kDimBsdf1], vr.u[kDimBsdf2], r);
+        if (IsBlocker(r)) break;
+        const float cosAbs = fabsf(r.wil.z);
+        prevFinite = (r.pdfW != kInf);
+        if (prevFinite) {
+            prevBsdfPdf = r.pdfW * r.compProb;
+            eThr = beta * r.att * cosAbs / prevBsdfPdf;
+            if (!cut) beta = beta * ((r.att * cosAbs) / (r.pdfW * 1u * rr * r.compProb));
+        } else {
+            eThr = beta * r.att / (1.f * r.compProb);
+            if (!cut) beta = beta * (r.att / (1u * rr * r.compProb));
+        }
+        computeRefl = !cut;
+        hasPrev = true;
+        prevN = fr.z;
+        prevSides = sides;
+        org = sp;
+        dir = ToWorld(fr, r.wil);
+        tmin = EpsRayCos(cosAbs);
+    }
+    return L;
+}
+
+// PathTracer::EstimateIncomingRadiancePTNaive, path_tracer.hxx:45-165
+PG3_DEV F3 RadiancePTNaive(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
+    F3 out = mk(0.f), thr = mk(1.0f);
+    float tmin = 0.f;
+    for (uint32_t len = 1;; ++len) {
+        Hit h;
+        h.dist = 1e36f;
+        if (!Intersect(S, org, dir, tmin, h)) {
+            if (len >= S.minPL && S.bgLight >= 0) out += BgRadiance(S, dir) * thr;
+            return out;
+        }
+        const F3 sp = org + dir * h.dist;
+        const Frame fr = FrameFromZ(h.n);
+        const F3 wol = ToLocal(fr, -dir);
+        const DevMaterial mat = S.mats[h.mat];
+        if (mat.lightId >= 0 && len >= S.minPL) {
+            const DevLight &Lt = S.lights[mat.lightId];
+            if (Lt.type == kLightArea && wol.z > 0.f) out += Lt.radiance * thr;
+        }
+        if (ReflectanceZero(mat)) return out;
+        if (S.maxPL > 0 && len >= S.maxPL) return out;
+        if (len >= 1200u) return out;
+        ++segments;
+        VertexRng vr;
+        RngVertex(key, len, vr);
+        float rr = 1.0f;
+        if (S.maxPL == 0) {
+            rr = Clampf(RRProb(mat, wol), 0.0f, 0.997f);
+            if (vr.u[kDimRR] > rr) return out;
+        }
+        BsdfRec r;
+        SampleBsdf(mat, wol, vr.u[kDimBsdf0], vr.u[kDimBsdf1], vr.u[kDimBsdf2], r);
+        if (IsBlocker(r)) return out;
+        const float cosAbs = fabsf(r.wil.z);
+        thr = thr * ((r.pdfW != kInf) ? (r.att * cosAbs) / (r.pdfW * rr * r.compProb) : r.att / (rr * r.compProb));
+        org = sp;
+        dir = ToWorld(fr, r.wil);
+        tmin = EpsRayCos(cosAbs);
+    }
+}
+
+// DirectIllumination::EstimateIncomingRadiance, direct_illumination.hxx:15-156
+PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
+    Hit h;
+    h.dist = 1e36f;
+    if (!Intersect(S, org, dir, 0.f, h)) return (S.bgLight >= 0) ? BgRadiance(S, dir) : mk(0.f);
+    ++segments;
+    const F3 sp = org + dir * h.dist;
+    const Frame fr = FrameFromZ(h.n);
+    const F3 wol = ToLocal(fr, -dir);
+    const DevMaterial mat = S.mats[h.mat];
+    const uint32_t sides = mat.sides;
+    const uint32_t vtx = 1;
+    VertexRng vr;
+    RngVertex(key, vtx, vr);
+    LightCache lc;
+    lc.sum = 0.f;
+    if (S.numLights > 1 && S.algorithm != 3) FillLightCache(S, sp, fr, sides, key, vtx, lc);
+    F3 Lo = mk(0.f);
+    const int alg = S.algorithm;
+    if (alg == 3) {                                    // dlsa: every light once
+        for (uint32_t i = 0; i < S.numLights; ++i) {
+            LightSample ls;
+            const float u0 = RngGet(key, vtx, i, kDimLight0), u1 = RngGet(key, vtx, i, kDimLight1),
+                        u2 = RngGet(key, vtx, i, kDimLight2);
+            if (SampleIllumination(S, i, sp, fr, sides, u0, u1, u2, ls) && MaxC(ls.sample) > 0.f &&
+                !Occluded(S, sp, ls.wig, ls.dist)) {                                    // :315-348
+                BsdfRec r;
+                EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
+                Lo += (ls.pdfW != kInf) ? ls.sample * r.att / (ls.pdfW * 1.f) : ls.sample * r.att / 1.f;
+            }
+        }
+    } else if (alg == 4 || alg == 5) {                 // dlss / dmis light part
+        float prob = 1.f;
+        const int32_t li = PickLight(S, lc, vr.u[kDimLightPick], prob);
+        LightSample ls;
+        if (li >= 0 && SampleIllumination(S, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
+                                          vr.u[kDimLight2], ls)) {
+            ls.prob = prob;
+            if (MaxC(ls.sample) > 0.f && !Occluded(S, sp, ls.wig, ls.dist)) {
+                BsdfRec r;
+                EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
+                if (alg == 4) {
+                    Lo += (ls.pdfW != kInf) ? ls.sample * r.att / (ls.pdfW * ls.prob) : ls.sample * r.att / ls.prob;
+                } else if (ls.pdfW != kInf) {
+                    const float lightPdf = ls.pdfW * ls.prob;
+                    Lo += (ls.sample * r.att * MisPower(lightPdf, r.pdfW * r.compProb)) / lightPdf;
+                } else {
+                    Lo += (ls.sample * r.att) / ls.prob;
+                }
+            }
+        }
+    }
+    if (alg == 2 || alg == 5) {                        // dbs / dmis BSDF part
+        BsdfRec r;
+        SampleBsdf(mat, wol, vr.u[kDimBsdf0], vr.u[kDimBsdf1], vr.u[kDimBsdf2], r);
+        if (!IsBlocker(r)) {
+            // GetDirectRadianceFromDirection, path_tracer_base.hxx:92-154
+            const F3 wig = ToWorld(fr, r.wil);
+            Hit h2;
+            h2.dist = 1e36f;
+            F3 Li = mk(0.f);
+            float lpdf = 0.f;
+            int32_t lid = -1;
+            if (Intersect(S, sp, wig, EpsRayCos(r.wil.z), h2)) {
+                const int32_t l2 = S.mats[h2.mat].lightId;
+                if (l2 >= 0) {
+                    const DevLight &Lt = S.lights[l2];
+                    const F3 lp = sp + wig * h2.dist;
+                    const F3 lwol = ToLocal(FrameFromZ(h2.n), -wig);
+                    if (Lt.type == kLightArea) {
+                        if (lwol.z > 0.f) Li = Lt.radiance;
+                        lpdf = AreaEmissionPdf(Lt, lp, sp);
+                    }
+                    lid = l2;
+                }
+            } else if (S.bgLight >= 0) {
+                Li = BgRadiance(S, wig);
+                lpdf = BgPdf(S, wig, fr, sides);
+                lid = S.bgLight;
+            }
+            const float cosAbs = fabsf(r.wil.z);
+            if (alg == 2) {
+                Lo = (r.pdfW != kInf) ? (r.att * cosAbs * Li) / (r.pdfW * r.compProb) : (r.att * Li) / r.compProb;
+            } else if (MaxC(Li) > 0.f) {                                                  // :427-490
+                if (r.pdfW != kInf) {
+                    const float bp = r.pdfW * r.compProb;
+                    const float pick = (lid >= 0) ? PickProb(S, lc, (uint32_t)lid) : 0.f;
+                    Lo += (r.att * cosAbs * Li * MisPower(bp, lpdf * pick)) / bp;
+                } else {
+                    Lo += (r.att * Li) / (1.f * r.compProb);
+                }
+            }
+        }
+    }
+    F3 Le = mk(0.f);
+    if (mat.lightId >= 0) {
+        const DevLight &Lt = S.lights[mat.lightId];
+        if (Lt.type == kLightArea && wol.z > 0.f) Le = Lt.radiance;
+    }
+    return Le + Lo;
+}
+
+PG3_DEV F3 RadianceSample(const SceneView &S, uint32_t pixel, uint32_t iteration, uint32_t &segments) {
+    RngKey key{S.k0, S.k1, pixel, iteration, S.sceneId << 16};
+    const float jx = RngGet(key, 0, 0, 0), jy = RngGet(key, 0, 0, 1);   // path_tracer_base.hxx:74-77
+    const uint32_t x = pixel % S.resX, y = pixel / S.resX;
+    F3 org, dir;
+    GenerateRay(S, float(x) + jx, float(y) + jy, org, dir);
+    if (S.algorithm == 7) return RadiancePT(S, key, org, dir, segments);
+    if (S.algorithm == 6) return RadiancePTNaive(S, key, org, dir, segments);
+    return RadianceDirect(S, key, org, dir, segments);
+}
+
+}  // namespace pg3

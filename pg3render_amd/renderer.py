@@ -1,0 +1,240 @@
+"""Host-side mirror of the reference renderer interface for the GPU path.
+
+`GpuPathTracer` has the shape of `AbstractRenderer` (src/renderer.hxx:232-278):
+`run_iteration()` is `RunIteration` (one sample per pixel, src/path_tracer_base.hxx:57-90),
+`get_framebuffer()` is `GetFramebuffer` (FP64 sum / iterations, src/renderer.hxx:255-261) and
+`was_used()` is `WasUsed`.  Scenes come from the C++ builders in libpg3g (include/pg3g_scenes.h).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import check
+
+
+class SceneBuild:
+    """Owns a host scene description built by libpg3g (workload or reference scene ID)."""
+
+    def __init__(self, handle):
+        self._lib = _ffi.load()
+        self.handle = handle
+
+    @classmethod
+    def workload(cls, workload, seed=1, variant=0, res=0):
+        lib = _ffi.load()
+        h = C.c_void_p()
+        check(lib.pg3g_build_workload_scene(workload, seed, variant, res, C.byref(h)), "pg3g_build_workload_scene")
+        return cls(h)
+
+    @classmethod
+    def reference(cls, scene_id, em_type=0, resolution=0):
+        lib = _ffi.load()
+        h = C.c_void_p()
+        check(lib.pg3g_build_reference_scene(scene_id, em_type, resolution, C.byref(h)), "pg3g_build_reference_scene")
+        return cls(h)
+
+    @property
+    def desc(self):
+        return self._lib.pg3g_scene_build_mutable(self.handle).contents
+
+    @property
+    def desc_ptr(self):
+        return self._lib.pg3g_scene_build_mutable(self.handle)
+
+    @property
+    def spp(self):
+        return int(self._lib.pg3g_scene_build_spp(self.handle))
+
+    @property
+    def resolution(self):
+        c = self.desc.camera
+        return int(c.res_x), int(c.res_y)
+
+    def em_texels(self):
+        """numpy view (h, w, 3) of the EM image (mutable), or None."""
+        d = self.desc
+        if d.em.width == 0:
+            return None
+        p = self._lib.pg3g_scene_build_em_texels(self.handle)
+        return np.ctypeslib.as_array(p, shape=(d.em.height, d.em.width, 3))
+
+    def materials(self):
+        d = self.desc
+        return [d.materials[i] for i in range(d.num_materials)]
+
+    def set_params(self, **kw):
+        p = self.desc.params
+        for k, v in kw.items():
+            if k == "algorithm" and isinstance(v, str):
+                v = _ffi.ALGORITHM_ACRONYMS[v]
+            setattr(p, k, v)
+
+    def close(self):
+        if self.handle:
+            self._lib.pg3g_scene_build_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """A libpg3g context on one HIP device."""
+
+    def __init__(self, device=0):
+        self._lib = _ffi.load()
+        self.ptr = C.c_void_p()
+        check(self._lib.pg3g_create(C.byref(self.ptr), device), "pg3g_create")
+
+    def close(self):
+        if self.ptr:
+            self._lib.pg3g_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene, slot=0):
+        check(self._lib.pg3g_upload_scene_slot(self.ptr, slot, scene.desc_ptr), "pg3g_upload_scene_slot")
+
+    def set_params(self, slot, params):
+        check(self._lib.pg3g_set_params(self.ptr, slot, C.byref(params)), "pg3g_set_params")
+
+    def set_kernel_mode(self, mode):
+        check(self._lib.pg3g_set_kernel_mode(self.ptr, mode), "pg3g_set_kernel_mode")
+
+    def render(self, first_iter, n_iter, slot=0):
+        check(self._lib.pg3g_render_slot(self.ptr, slot, first_iter, n_iter), "pg3g_render_slot")
+
+    def render_timed(self, first_iter, n_iter, slot=0):
+        """Renders and returns the GPU time (ms) between HIP events on the library's stream."""
+        ms = C.c_float()
+        check(self._lib.pg3g_render_timed(self.ptr, slot, first_iter, n_iter, C.byref(ms)), "pg3g_render_timed")
+        return float(ms.value)
+
+    def synchronize(self):
+        check(self._lib.pg3g_synchronize(self.ptr), "pg3g_synchronize")
+
+    def framebuffer(self, slot, res):
+        w, h = res
+        out = np.zeros((h, w, 3), dtype=np.float64)
+        it = C.c_uint64()
+        check(self._lib.pg3g_read_framebuffer_slot(self.ptr, slot, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                   C.byref(it)), "pg3g_read_framebuffer_slot")
+        return out, int(it.value)
+
+    def clear(self, slot=0):
+        check(self._lib.pg3g_clear_framebuffer(self.ptr, slot), "pg3g_clear_framebuffer")
+
+    def counters(self):
+        s, c = C.c_uint64(), C.c_uint64()
+        check(self._lib.pg3g_read_counters(self.ptr, C.byref(s), C.byref(c)), "pg3g_read_counters")
+        return int(s.value), int(c.value)
+
+    def reset_counters(self):
+        check(self._lib.pg3g_reset_counters(self.ptr), "pg3g_reset_counters")
+
+    # ---- function-level entry points --------------------------------------------------------
+    def bsdf_eval(self, mat, wil, wol):
+        n = len(wil)
+        wil = np.ascontiguousarray(wil, np.float32)
+        wol = np.ascontiguousarray(wol, np.float32)
+        f = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        comp = np.zeros(n, np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        check(self._lib.pg3g_bsdf_eval_batch(self.ptr, C.byref(mat), n, fp(wil), fp(wol), fp(f), fp(pdf), fp(comp)),
+              "pg3g_bsdf_eval_batch")
+        return f, pdf, comp
+
+    def bsdf_sample(self, mat, wol, u):
+        n = len(wol)
+        wol = np.ascontiguousarray(wol, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        wil = np.zeros((n, 3), np.float32)
+        f = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        comp = np.zeros(n, np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        check(self._lib.pg3g_bsdf_sample_batch(self.ptr, C.byref(mat), n, fp(wol), fp(u), fp(wil), fp(f), fp(pdf),
+                                               fp(comp)), "pg3g_bsdf_sample_batch")
+        return wil, f, pdf, comp
+
+    def em_sample(self, normals, sides, u):
+        n = len(normals)
+        normals = np.ascontiguousarray(normals, np.float32)
+        sides = np.ascontiguousarray(sides, np.int32)
+        u = np.ascontiguousarray(u, np.float32)
+        d = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        rc = np.zeros((n, 3), np.float32)
+        ok = np.zeros(n, np.int32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        check(self._lib.pg3g_em_sample_batch(self.ptr, n, fp(normals), ip(sides), fp(u), fp(d), fp(pdf), fp(rc), ip(ok)),
+              "pg3g_em_sample_batch")
+        return d, pdf, rc, ok
+
+    def em_pdf(self, dirs, normals, sides):
+        n = len(dirs)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        normals = np.ascontiguousarray(normals, np.float32)
+        sides = np.ascontiguousarray(sides, np.int32)
+        pdf = np.zeros(n, np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        check(self._lib.pg3g_em_pdf_batch(self.ptr, n, fp(dirs), fp(normals),
+                                          sides.ctypes.data_as(C.POINTER(C.c_int32)), fp(pdf)), "pg3g_em_pdf_batch")
+        return pdf
+
+    def em_eval(self, dirs):
+        n = len(dirs)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        rgb = np.zeros((n, 3), np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        check(self._lib.pg3g_em_eval_batch(self.ptr, n, fp(dirs), fp(rgb)), "pg3g_em_eval_batch")
+        return rgb
+
+    def steerable_leaves(self):
+        nl, nv, lv = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(self._lib.pg3g_steerable_info(self.ptr, C.byref(nl), C.byref(nv), C.byref(lv)), "pg3g_steerable_info")
+        out = np.zeros((nl.value, 3, 3), np.float32)
+        check(self._lib.pg3g_steerable_leaves(self.ptr, out.ctypes.data_as(C.POINTER(C.c_float))), "pg3g_steerable_leaves")
+        w = np.zeros(9, np.float32)
+        check(self._lib.pg3g_steerable_root_weight(self.ptr, w.ctypes.data_as(C.POINTER(C.c_float))),
+              "pg3g_steerable_root_weight")
+        return out, int(nv.value), int(lv.value), w
+
+
+class GpuPathTracer:
+    """AbstractRenderer-shaped GPU renderer for one scene (src/renderer.hxx:232-278)."""
+
+    def __init__(self, scene, context=None, slot=0):
+        self.scene = scene
+        self.ctx = context or Context()
+        self.slot = slot
+        self.ctx.upload(scene, slot)
+        self.iterations = 0
+
+    def run_iteration(self, iteration):
+        """One sample per pixel (PathTracerBase::RunIteration, src/path_tracer_base.hxx:57-90)."""
+        self.ctx.render(iteration, 1, self.slot)
+        self.iterations += 1
+
+    def run_iterations(self, first, count):
+        self.ctx.render(first, count, self.slot)
+        self.iterations += count
+
+    def get_framebuffer(self):
+        """FP64 image = sum / iterations (src/renderer.hxx:255-261), shape (h, w, 3)."""
+        fb, it = self.ctx.framebuffer(self.slot, self.scene.resolution)
+        return fb / it if it > 0 else fb
+
+    def was_used(self):
+        return self.iterations > 0

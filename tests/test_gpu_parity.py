@@ -1,0 +1,221 @@
+"""GPU path (libpg3g.so through the C ABI) against the CPU oracle on identical seeded inputs.
+
+Function level: BSDF eval/pdf/sample and EM eval/sample/pdf agree to <= 1e-5 relative (FP32,
+north_star tier 1).  Image level: with the same counter-based RNG both sides trace the same paths,
+so per-pixel sums agree except where FP32 rounding sends a path another way.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pg3render_amd import Context, SceneBuild, _ffi
+from pg3render_amd.scene_desc import CustomScene, light, material
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5      # tier-1 tolerance (relative, FP32)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def rel_err(a, b, floor):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), floor)
+
+
+def dirs(rng, n, both=False):
+    u = rng.random((n, 2))
+    z = u[:, 1] * 0.998 + 0.001
+    r = np.sqrt(1 - z * z)
+    d = np.stack([r * np.cos(2 * np.pi * u[:, 0]), r * np.sin(2 * np.pi * u[:, 0]), z], 1)
+    if both:
+        d[rng.random(n) < 0.5, 2] *= -1
+    return d.astype(np.float32)
+
+
+MATERIALS = [
+    ("lambert", material(_ffi.MAT_LAMBERT, diffuse=(0.6, 0.3, 0.9)), False),
+    ("phong", material(_ffi.MAT_PHONG, diffuse=(0.8, 0.8, 0.15), glossy=(0.7, 0.7, 0.7), phong_exponent=200,
+                       phong_diffuse_on=1, phong_glossy_on=1), False),
+    ("ggx_conductor_0.1", material(_ffi.MAT_GGX_CONDUCTOR, alpha=0.1, inner_ior=0.2, absorbance=3.8157), False),
+    ("ggx_conductor_0.5", material(_ffi.MAT_GGX_CONDUCTOR, alpha=0.5, inner_ior=0.468, outer_ior=1.000277,
+                                   absorbance=2.81), False),
+    ("ggx_dielectric_0.3", material(_ffi.MAT_GGX_DIELECTRIC, alpha=0.3, inner_ior=1.5), True),
+    ("ggx_dielectric_0.01", material(_ffi.MAT_GGX_DIELECTRIC, alpha=0.01, inner_ior=1.51, outer_ior=1.000277), True),
+    ("smooth_conductor", material(_ffi.MAT_SMOOTH_CONDUCTOR, inner_ior=0.121, absorbance=3.66), False),
+    ("smooth_dielectric", material(_ffi.MAT_SMOOTH_DIELECTRIC, inner_ior=1.51, outer_ior=1.000277), True),
+]
+
+
+@pytest.mark.parametrize("name,mat,both", MATERIALS, ids=[m[0] for m in MATERIALS])
+def test_bsdf_eval_parity(ctx, name, mat, both):
+    rng = np.random.default_rng(hash(name) % 2**32)
+    n = 20000
+    wil, wol = dirs(rng, n, both), dirs(rng, n, both)
+    fg, pg, cg = ctx.bsdf_eval(mat, wil, wol)
+    fo, po, co = O.bsdf_eval(mat, wil, wol)
+    scale_f = max(1e-3, float(np.abs(fo).max()) * 1e-6)
+    ef = rel_err(fg, fo, scale_f)
+    ep = rel_err(pg, po, max(1e-3, float(np.abs(po).max()) * 1e-6))
+    assert np.all(cg == co)
+    assert np.quantile(ef, 0.999) <= REL and np.quantile(ep, 0.999) <= REL, (ef.max(), ep.max())
+    assert ef.max() <= 1e-3 and ep.max() <= 1e-3
+
+
+@pytest.mark.parametrize("name,mat,both", MATERIALS, ids=[m[0] for m in MATERIALS])
+def test_bsdf_sample_parity(ctx, name, mat, both):
+    rng = np.random.default_rng(1 + hash(name) % 2**32)
+    n = 20000
+    wol = dirs(rng, n, both)
+    u = rng.random((n, 3)).astype(np.float32)
+    wg, fg, pg, cg = ctx.bsdf_sample(mat, wol, u)
+    wo_, fo, po, co = O.bsdf_sample(mat, wol, u)
+    ew = np.abs(wg - wo_).max(axis=1)
+    ef = rel_err(fg, fo, max(1e-3, float(np.abs(fo).max()) * 1e-6)).max(axis=1)
+    finite = np.isfinite(po)
+    ep = rel_err(pg[finite], po[finite], max(1e-3, float(np.abs(po[finite]).max()) * 1e-6))
+    assert np.all(np.isinf(pg[~finite]))
+    ec = rel_err(cg, co, 1e-6)
+    for e in (ew, ef, ep, ec):
+        assert np.quantile(e, 0.999) <= REL, e.max()
+        # near-degenerate samples (grazing/TIR) may round to a neighbouring branch
+        assert np.mean(e > 1e-3) < 1e-3
+
+
+def em_scene(sampler, w=256, h=128, seed=3, levels=(4, 4)):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    img = 0.2 + 0.8 * (1 - yy / h)[..., None] * np.array([0.5, 0.75, 1.0])
+    img = img + 200.0 * np.exp(-((xx - w * 0.3) ** 2 + (yy - h * 0.3) ** 2) / 8.0)[..., None]
+    img = img * (0.9 + 0.2 * rng.random((h, w, 3)))
+    return CustomScene(materials=[material(_ffi.MAT_LAMBERT, diffuse=(0.5, 0.5, 0.5))],
+                       lights=[light(_ffi.LIGHT_BACKGROUND)], em=img.astype(np.float32), em_sampler=sampler,
+                       steerable_levels=levels, camera={"res": (8, 8)})
+
+
+@pytest.mark.parametrize("sampler", [_ffi.EM_SIMPLE, _ffi.EM_STEERABLE, _ffi.EM_COSINE])
+def test_em_parity(ctx, sampler):
+    sc = em_scene(sampler)
+    ctx.upload(sc)
+    orc = O.OracleScene(sc)
+    rng = np.random.default_rng(17)
+    n = 20000
+    normals = dirs(rng, n)
+    normals = normals[:, [0, 2, 1]] * np.where(rng.random((n, 1)) < 0.5, -1, 1).astype(np.float32)
+    sides = rng.integers(1, 4, n).astype(np.int32)
+    u = rng.random((n, 3)).astype(np.float32)
+    # evaluation
+    d = dirs(rng, n, True)
+    eg, eo = ctx.em_eval(d), orc.em_eval(d)
+    assert np.quantile(rel_err(eg, eo, 1e-3), 0.999) <= REL
+    # sampling
+    dg, pg, rg, okg = ctx.em_sample(normals, sides, u)
+    do, po, ro, oko = orc.em_sample(normals, sides, u)
+    assert np.all(okg == oko)
+    good = oko == 1
+    ed = np.abs(dg - do).max(axis=1)[good]
+    ep = rel_err(pg, po, 1e-6)[good]
+    er = rel_err(rg, ro, 1e-3).max(axis=1)[good]
+    for e in (ed, ep, er):
+        assert np.quantile(e, 0.995) <= REL * 2, e.max()
+        assert np.mean(e > 1e-3) < 2e-3
+    # pdf of arbitrary directions
+    pg2 = ctx.em_pdf(d, normals, sides)
+    po2 = orc.em_pdf(d, normals, sides)
+    e = rel_err(pg2, po2, 1e-6)
+    assert np.quantile(e, 0.999) <= REL * 2 and np.mean(e > 1e-3) < 1e-3
+
+
+def test_steerable_tables_match_oracle_build(ctx):
+    sb = SceneBuild.workload(2, seed=5)
+    ctx.upload(sb)
+    leaves_g, nv_g, levels, wg = ctx.steerable_leaves()
+    leaves_o, nv_o, wo = O.OracleScene(sb).steerable()
+    assert leaves_g.shape == leaves_o.shape == (5120, 3, 3)
+    assert nv_g == nv_o
+    assert np.max(np.abs(leaves_g - leaves_o)) <= 1e-6
+    assert levels == 6   # ceil(log6 5120) + 1 (em_steerable_sampler.hxx:4518-4628)
+    assert np.allclose(wg, wo, rtol=1e-5, atol=1e-5 * np.abs(wo).max())
+
+
+def render_pair(ctx, sb, spp, first=0):
+    ctx.upload(sb)
+    ctx.render(first, spp)
+    g, it = ctx.framebuffer(0, sb.resolution)
+    o, _ = O.OracleScene(sb).render(first, spp, threads=8)
+    assert it == spp
+    return g, o
+
+
+def assert_image_parity(g, o, frac=0.02):
+    assert np.isfinite(g).all()
+    lum_g, lum_o = g.sum(-1), o.sum(-1)
+    diff = np.abs(lum_g - lum_o) / np.maximum(np.abs(lum_o), 1e-3 * max(1e-6, lum_o.mean()))
+    # identical RNG streams: most pixels agree to FP32 rounding
+    assert np.mean(diff > 1e-3) < frac, np.mean(diff > 1e-3)
+    assert abs(lum_g.sum() - lum_o.sum()) / lum_o.sum() < 5e-3
+
+
+@pytest.mark.parametrize("kernel", [_ffi.KERNEL_MEGAKERNEL])
+def test_workload1_image_parity(ctx, kernel):
+    ctx.set_kernel_mode(kernel)
+    sb = SceneBuild.workload(1, seed=21, res=48)
+    g, o = render_pair(ctx, sb, 8)
+    assert_image_parity(g, o)
+
+
+def test_workload2_image_parity(ctx):
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(2, seed=22, res=40)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+
+
+@pytest.mark.parametrize("alg", ["ptn", "dbs", "dlss", "dlsa", "dmis"])
+def test_algorithms_image_parity(ctx, alg):
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(1, seed=23, res=40)
+    sb.set_params(algorithm=alg)
+    g, o = render_pair(ctx, sb, 8)
+    assert_image_parity(g, o)
+
+
+@pytest.mark.parametrize("scene_id", [2, 3, 11, 18, 32, 34, 37])
+def test_reference_scene_parity(ctx, scene_id):
+    """The reference's own Cornell-box scene IDs with -sb 1 -iic 0 -maxpl 8."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.reference(scene_id, em_type=0, resolution=40)
+    sb.set_params(splitting_budget=1.0, indirect_clipping=0.0, max_path_length=8, seed=7)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+
+
+def test_iterations_accumulate_like_run_iteration(ctx):
+    """Rendering iterations [0,4) then [4,8) equals rendering [0,8) (RunIteration is additive)."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(1, seed=24, res=32)
+    ctx.upload(sb)
+    ctx.render(0, 4)
+    ctx.render(4, 4)
+    a, it = ctx.framebuffer(0, sb.resolution)
+    ctx.clear(0)
+    ctx.render(0, 8)
+    b, _ = ctx.framebuffer(0, sb.resolution)
+    assert it == 8
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-12)
+
+
+def test_unsupported_parameters_are_rejected(ctx):
+    sb = SceneBuild.workload(1, seed=1, res=16)
+    sb.set_params(splitting_budget=4.0)
+    with pytest.raises(_ffi.Pg3gError) as e:
+        ctx.upload(sb)
+    assert e.value.code == _ffi.PG3G_ERR_UNSUPPORTED
