@@ -647,7 +647,7 @@ static Scene *BuildScene(const pg3g_scene_desc *d) {
         S->em.data.resize((size_t)S->em.w * S->em.h);
         for (size_t i = 0; i < S->em.data.size(); ++i) S->em.data[i] = V3(d->em.texels[i * 3], d->em.texels[i * 3 + 1], d->em.texels[i * 3 + 2]);
         S->emSampler = d->em.sampler;
-        if (S->emSampler == PG3G_EM_SIMPLE) S->simple.Build(S->em);
+        S->simple.Build(S->em);   // also serves EstimateIrradiance's MC fallback (em.hxx:33-49)
         if (S->emSampler == PG3G_EM_STEERABLE) {
             const uint32_t mn = d->em.st_min_subdiv_level >= 0 ? (uint32_t)d->em.st_min_subdiv_level : 4u;
             const uint32_t mx = d->em.st_max_subdiv_level >= 0 ? (uint32_t)d->em.st_max_subdiv_level : 9u;

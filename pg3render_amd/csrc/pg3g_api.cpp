@@ -340,8 +340,9 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
             tex[i] = make_float4(d->em.texels[i * 3], d->em.texels[i * 3 + 1], d->em.texels[i * 3 + 2], 0.f);
         PG3_HIP(Upload(S.texels, tex.data(), tex.size(), st));
         E.texels = S.texels.as<float4>();
-        if (E.sampler == PG3G_EM_SIMPLE || E.sampler == PG3G_EM_STEERABLE) {
-            // the simple sampler's CDF is also used by EstimateIrradiance's MC fallback
+        {
+            // The 2-D CDF is built for every image EM: besides the simple sampler it serves
+            // EnvironmentMap::EstimateIrradiance's MC fallback (em.hxx:123-170).
             PG3_HIP(S.margCdf.Alloc(sizeof(float) * (E.h + 1)));
             PG3_HIP(S.condCdf.Alloc(sizeof(float) * (size_t)E.h * (E.w + 1)));
             PG3_HIP(S.rowInt.Alloc(sizeof(float) * E.h));

@@ -62,8 +62,9 @@ PG3_DEV int32_t PickLight(const SceneView &S, const LightCache &lc, float u, flo
                 if (!(rv > cdf)) found = true;
             }
         }
-        if (!found) { li = n; prev = cdf; }   // rv beyond the pseudo-CDF (reference loop bound)
-        prob = (found ? (cdf - prev) : 0.f) / lc.sum;
+        // rv <= sum, so the scan always stops inside the table; keep the index in range regardless
+        if (!found) li = n - 1;
+        prob = (cdf - prev) / lc.sum;
         return (int32_t)li;
     }
     prob = 1.f / n;
