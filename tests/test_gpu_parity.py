@@ -81,7 +81,8 @@ def test_bsdf_sample_parity(ctx, name, mat, both):
     ew = np.abs(wg - wo_).max(axis=1)
     ef = rel_err(fg, fo, max(1e-3, float(np.abs(fo).max()) * 1e-6)).max(axis=1)
     finite = np.isfinite(po)
-    ep = rel_err(pg[finite], po[finite], max(1e-3, float(np.abs(po[finite]).max()) * 1e-6))
+    pmax = float(np.abs(po[finite]).max()) if finite.any() else 1.0
+    ep = rel_err(pg[finite], po[finite], max(1e-3, pmax * 1e-6)) if finite.any() else np.zeros(1)
     assert np.all(np.isinf(pg[~finite]))
     ec = rel_err(cg, co, 1e-6)
     for e in (ew, ef, ep, ec):
