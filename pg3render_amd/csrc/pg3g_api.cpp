@@ -72,8 +72,9 @@ struct pg3g_context {
     hipStream_t stream = nullptr;
     std::vector<std::unique_ptr<SceneSlot>> slots;
     DevBuf counters;   // [segments, camera samples]
-    int kernelMode = PG3G_KERNEL_MEGAKERNEL;
+    int kernelMode = PG3G_KERNEL_WAVEFRONT;
     DevBuf scratch[8];
+    WavefrontContext *wavefront = nullptr;
 };
 
 namespace {
@@ -426,6 +427,8 @@ int pg3g_destroy(pg3g_context *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->slots.clear();
+    DestroyWavefrontContext(ctx->wavefront);
+    ctx->wavefront = nullptr;
     ctx->counters.Free();
     for (auto &b : ctx->scratch) b.Free();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -465,8 +468,10 @@ int pg3g_render_slot(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
     if ((uint64_t)first_iter + n_iter > 0xFFFFFFFFull) return Fail(PG3G_ERR_INVALID_ARG, "iteration range overflows");
     PG3_HIP(hipSetDevice(ctx->device));
     if (ctx->kernelMode == PG3G_KERNEL_WAVEFRONT) {
+        if (!ctx->wavefront) ctx->wavefront = CreateWavefrontContext();
+        if (!ctx->wavefront) return Fail(PG3G_ERR_OUT_OF_MEMORY, "host allocation failed");
         PG3_HIP(LaunchWavefront(S->view, first_iter, n_iter, S->fb.as<double>(),
-                                ctx->counters.as<unsigned long long>(), nullptr, ctx->stream));
+                                ctx->counters.as<unsigned long long>(), ctx->wavefront, ctx->stream));
     } else {
         PG3_HIP(LaunchMegakernel(S->view, first_iter, n_iter, S->fb.as<double>(), ctx->counters.as<unsigned long long>(),
                                  ctx->stream));

@@ -14,8 +14,11 @@ namespace pg3 {
 
 hipError_t LaunchMegakernel(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
                             unsigned long long *counters, hipStream_t stream);
+struct WavefrontContext;   // queue storage of the wavefront path tracer (one per pg3g_context)
+WavefrontContext *CreateWavefrontContext();
+void DestroyWavefrontContext(WavefrontContext *w);
 hipError_t LaunchWavefront(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
-                           unsigned long long *counters, void *queues, hipStream_t stream);
+                           unsigned long long *counters, WavefrontContext *queues, hipStream_t stream);
 hipError_t LaunchBsdfEval(const DevMaterial &m, uint32_t n, const float *wil, const float *wol, float *f, float *pdf,
                           float *comp, hipStream_t s);
 hipError_t LaunchBsdfSample(const DevMaterial &m, uint32_t n, const float *wol, const float *u, float *wil, float *f,

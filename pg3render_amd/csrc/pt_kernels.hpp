@@ -71,116 +71,165 @@ PG3_DEV int32_t PickLight(const SceneView &S, const LightCache &lc, float u, flo
     return min((int32_t)(u * n), (int32_t)(n - 1));
 }
 
-struct PathCounters {
-    uint32_t segments;     // extension rays traced from surface vertices (each with <= 1 shadow ray)
+// State of one path between two vertices of PathTracer::EstimateIncomingRadiancePT (-sb 1).
+struct PathState {
+    F3 org, dir;
+    float tmin;
+    F3 beta;             // throughput up to the current vertex (product of indirect factors)
+    F3 eThr;             // weight of emission found by the last BSDF sample (MIS applied on hit)
+    F3 L;                // accumulated radiance estimate of the sample
+    float prevBsdfPdf;   // pdfW * compProb of the last BSDF sample
+    F3 prevN;            // shading normal of the previous vertex (EM pdf w.r.t. its frame)
+    uint32_t len;        // path length of the next vertex (1 = camera ray hit)
+    uint32_t flags;      // kHasPrev | kPrevFinite | kComputeRefl | previous sides << 4
+    LightCache lc;       // light-contribution cache of the previous vertex (> 1 light)
+};
+enum : uint32_t { kHasPrev = 1u, kPrevFinite = 2u, kComputeRefl = 4u };
+
+PG3_DEV void InitPath(PathState &ps, F3 org, F3 dir) {
+    ps.org = org;
+    ps.dir = dir;
+    ps.tmin = 0.f;
+    ps.beta = mk(1.f);
+    ps.eThr = mk(0.f);
+    ps.L = mk(0.f);
+    ps.prevBsdfPdf = 0.f;
+    ps.prevN = mk(0.f, 0.f, 1.f);
+    ps.len = 1;
+    ps.flags = kComputeRefl | kPrevFinite;
+    ps.lc.sum = 0.f;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxPickLights; ++i) ps.lc.est[i] = 0.f;
+}
+
+// Next-event request produced at a vertex: the contribution is added when the segment from `p`
+// along `dir` up to `tmax` is unoccluded (Scene::Occluded, scene.hxx:61-74).
+struct ShadowReq {
+    F3 p, dir, contrib;
+    float tmax;
+    bool valid;
 };
 
-// One sample of PathTracer::EstimateIncomingRadiancePT with -sb 1 (see header comment).
+// One vertex of EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with one light sample and
+// one BSDF sample: adds the MIS-weighted emission found by the previous BSDF sample, emits the
+// light-sample request and samples the BSDF.  Returns true if the path continues.
+PG3_DEV bool ShadeVertex(const SceneView &S, const RngKey &key, PathState &ps, bool hit, const Hit &h,
+                         ShadowReq &sr, uint32_t &segments) {
+    sr.valid = false;
+    const uint32_t len = ps.len;
+    const bool hasPrev = ps.flags & kHasPrev, prevFinite = ps.flags & kPrevFinite;
+    if (!hit) {
+        if (len >= S.minPL && S.bgLight >= 0) {                                       // :408-422
+            const F3 Le = BgRadiance(S, ps.dir);
+            if (!hasPrev) {
+                ps.L += Le;
+            } else if (!IsZero(Le)) {
+                if (prevFinite) {
+                    const float lp = BgPdf(S, ps.dir, FrameFromZ(ps.prevN), ps.flags >> 4) *
+                                     PickProb(S, ps.lc, (uint32_t)S.bgLight);
+                    ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
+                } else {
+                    ps.L += ps.eThr * Le;
+                }
+            }
+        }
+        return false;
+    }
+    const F3 sp = ps.org + ps.dir * h.dist;
+    const Frame fr = FrameFromZ(h.n);
+    const F3 wol = ToLocal(fr, -ps.dir);
+    const DevMaterial mat = S.mats[h.mat];
+    if (mat.lightId >= 0 && len >= S.minPL) {                                         // :198-210
+        const DevLight &Lt = S.lights[mat.lightId];
+        F3 Le = mk(0.f);
+        if (Lt.type == kLightArea && wol.z > 0.f) Le = Lt.radiance;
+        if (!hasPrev) {
+            ps.L += Le;
+        } else if (!IsZero(Le)) {                                                     // :327-369
+            if (prevFinite) {
+                const float lp = AreaEmissionPdf(Lt, sp, ps.org) * PickProb(S, ps.lc, (uint32_t)mat.lightId);
+                ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
+            } else {
+                ps.L += ps.eThr * Le;
+            }
+        }
+    }
+    if (!(ps.flags & kComputeRefl)) return false;                                     // :212-238
+    if (ReflectanceZero(mat)) return false;
+    if (S.maxPL > 0 && len >= S.maxPL) return false;
+    if (len >= 1200u) return false;
+    ++segments;
+
+    VertexRng vr;
+    RngVertex(key, len, vr);
+    const uint32_t sides = mat.sides;
+    if (S.numLights > 1) FillLightCache(S, sp, fr, sides, key, len, ps.lc);
+    if ((len + 1) >= S.minPL) {                                                       // :252-265
+        float prob = 1.f;
+        const int32_t li = PickLight(S, ps.lc, vr.u[kDimLightPick], prob);
+        LightSample ls;
+        if (li >= 0 && SampleIllumination(S, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
+                                          vr.u[kDimLight2], ls)) {
+            ls.prob = prob;
+            if (MaxC(ls.sample) > 0.f) {                                              // :350-425
+                BsdfRec r;
+                EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
+                if (ls.pdfW != kInf) {
+                    const float lightPdf = ls.pdfW * ls.prob;
+                    const float bsdfPdf = r.pdfW * r.compProb;
+                    sr.contrib = ps.beta * ((ls.sample * r.att * MisPower(lightPdf, bsdfPdf)) / lightPdf);
+                } else {
+                    sr.contrib = ps.beta * ((ls.sample * r.att) / (ls.prob * 1u));
+                }
+                sr.p = sp;
+                sr.dir = ls.wig;
+                sr.tmax = ls.dist;
+                sr.valid = true;
+            }
+        }
+    }
+    float rr = 1.0f;                                                                  // :267-303
+    bool cut = false;
+    if (S.maxPL == 0) {
+        rr = Clampf(RRProb(mat, wol), 0.0f, 0.997f);
+        cut = vr.u[kDimRR] > rr;
+    }
+    BsdfRec r;
+    SampleBsdf(mat, wol, vr.u[kDimBsdf0], vr.u[kDimBsdf1], vr.u[kDimBsdf2], r);
+    if (IsBlocker(r)) return false;
+    const float cosAbs = fabsf(r.wil.z);
+    const bool finite = (r.pdfW != kInf);
+    if (finite) {
+        ps.prevBsdfPdf = r.pdfW * r.compProb;
+        ps.eThr = ps.beta * r.att * cosAbs / ps.prevBsdfPdf;
+        if (!cut) ps.beta = ps.beta * ((r.att * cosAbs) / (r.pdfW * 1u * rr * r.compProb));
+    } else {
+        ps.eThr = ps.beta * r.att / (1.f * r.compProb);
+        if (!cut) ps.beta = ps.beta * (r.att / (1u * rr * r.compProb));
+    }
+    ps.flags = kHasPrev | (finite ? kPrevFinite : 0u) | (cut ? 0u : kComputeRefl) | (sides << 4);
+    ps.prevN = fr.z;
+    ps.org = sp;
+    ps.dir = ToWorld(fr, r.wil);
+    ps.tmin = EpsRayCos(cosAbs);
+    ps.len = len + 1;
+    return true;
+}
+
+// One sample of PathTracer::EstimateIncomingRadiancePT with -sb 1, traced in one thread.
 PG3_DEV F3 RadiancePT(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
-    F3 L = mk(0.f), beta = mk(1.f), eThr = mk(0.f);
-    float tmin = 0.f;
-    bool hasPrev = false, prevFinite = true, computeRefl = true;
-    float prevBsdfPdf = 0.f;
-    F3 prevN = mk(0.f, 0.f, 1.f);
-    uint32_t prevSides = 0;
-    LightCache lc;
-    lc.sum = 0.f;
-    for (uint32_t len = 1;; ++len) {
+    PathState ps;
+    InitPath(ps, org, dir);
+    for (;;) {
         Hit h;
         h.dist = 1e36f;
-        if (!Intersect(S, org, dir, tmin, h)) {
-            if (len >= S.minPL && S.bgLight >= 0) {                                     // :408-422
-                const F3 Le = BgRadiance(S, dir);
-                if (!hasPrev) {
-                    L += Le;
-                } else if (!IsZero(Le)) {
-                    if (prevFinite) {
-                        const float lp = BgPdf(S, dir, FrameFromZ(prevN), prevSides) * PickProb(S, lc, (uint32_t)S.bgLight);
-                        L += eThr * Le * MisPower(prevBsdfPdf, lp);
-                    } else {
-                        L += eThr * Le;
-                    }
-                }
-            }
-            break;
-        }
-        const F3 sp = org + dir * h.dist;
-        const Frame fr = FrameFromZ(h.n);
-        const F3 wol = ToLocal(fr, -dir);
-        const DevMaterial mat = S.mats[h.mat];
-        if (mat.lightId >= 0 && len >= S.minPL) {                                       // :198-210
-            const DevLight &Lt = S.lights[mat.lightId];
-            F3 Le = mk(0.f);
-            if (Lt.type == kLightArea && wol.z > 0.f) Le = Lt.radiance;
-            if (!hasPrev) {
-                L += Le;
-            } else if (!IsZero(Le)) {                                                   // :327-369
-                if (prevFinite) {
-                    const float lp = AreaEmissionPdf(Lt, sp, org) * PickProb(S, lc, (uint32_t)mat.lightId);
-                    L += eThr * Le * MisPower(prevBsdfPdf, lp);
-                } else {
-                    L += eThr * Le;
-                }
-            }
-        }
-        if (!computeRefl) break;                                                        // :212-238
-        if (ReflectanceZero(mat)) break;
-        if (S.maxPL > 0 && len >= S.maxPL) break;
-        if (len >= 1200u) break;
-        ++segments;
-
-        VertexRng vr;
-        RngVertex(key, len, vr);
-        const uint32_t sides = mat.sides;
-        if (S.numLights > 1) FillLightCache(S, sp, fr, sides, key, len, lc);
-        if ((len + 1) >= S.minPL) {                                                     // :252-265
-            float prob = 1.f;
-            const int32_t li = PickLight(S, lc, vr.u[kDimLightPick], prob);
-            LightSample ls;
-            if (li >= 0 && SampleIllumination(S, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
-                                              vr.u[kDimLight2], ls)) {
-                ls.prob = prob;
-                if (MaxC(ls.sample) > 0.f && !Occluded(S, sp, ls.wig, ls.dist)) {     // :350-425
-                    BsdfRec r;
-                    EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
-                    if (ls.pdfW != kInf) {
-                        const float lightPdf = ls.pdfW * ls.prob;
-                        const float bsdfPdf = r.pdfW * r.compProb;
-                        L += beta * ((ls.sample * r.att * MisPower(lightPdf, bsdfPdf)) / lightPdf);
-                    } else {
-                        L += beta * ((ls.sample * r.att) / (ls.prob * 1u));
-                    }
-                }
-            }
-        }
-        float rr = 1.0f;                                                                // :267-303
-        bool cut = false;
-        if (S.maxPL == 0) {
-            rr = Clampf(RRProb(mat, wol), 0.0f, 0.997f);
-            cut = vr.u[kDimRR] > rr;
-        }
-        BsdfRec r;
-        SampleBsdf(mat, wol, vr.u[kDimBsdf0], vr.u[kDimBsdf1], vr.u[kDimBsdf2], r);
-        if (IsBlocker(r)) break;
-        const float cosAbs = fabsf(r.wil.z);
-        prevFinite = (r.pdfW != kInf);
-        if (prevFinite) {
-            prevBsdfPdf = r.pdfW * r.compProb;
-            eThr = beta * r.att * cosAbs / prevBsdfPdf;
-            if (!cut) beta = beta * ((r.att * cosAbs) / (r.pdfW * 1u * rr * r.compProb));
-        } else {
-            eThr = beta * r.att / (1.f * r.compProb);
-            if (!cut) beta = beta * (r.att / (1u * rr * r.compProb));
-        }
-        computeRefl = !cut;
-        hasPrev = true;
-        prevN = fr.z;
-        prevSides = sides;
-        org = sp;
-        dir = ToWorld(fr, r.wil);
-        tmin = EpsRayCos(cosAbs);
+        const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
+        ShadowReq sr;
+        const bool cont = ShadeVertex(S, key, ps, hit, h, sr, segments);
+        if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
+        if (!cont) break;
     }
-    return L;
+    return ps.L;
 }
 
 // PathTracer::EstimateIncomingRadiancePTNaive, path_tracer.hxx:45-165

@@ -72,23 +72,46 @@ def test_bsdf_eval_parity(ctx, name, mat, both):
 
 @pytest.mark.parametrize("name,mat,both", MATERIALS, ids=[m[0] for m in MATERIALS])
 def test_bsdf_sample_parity(ctx, name, mat, both):
+    """SampleBsdf = warp (uniforms -> direction, component) + value (attenuation, pdf).  The warp is
+    compared directly; the value is compared with the oracle's EvalBsdf at the direction the GPU
+    sampled, which is what SampleBsdf returns in the reference for every finite BSDF (the GGX
+    materials use the sampled microfacet normal instead of the recomputed halfway vector, equal up
+    to rounding).  Comparing values at two different directions would measure the conditioning of
+    the BSDF (e.g. cos^200 for the Phong lobe), not the implementation."""
     rng = np.random.default_rng(1 + hash(name) % 2**32)
     n = 20000
     wol = dirs(rng, n, both)
     u = rng.random((n, 3)).astype(np.float32)
     wg, fg, pg, cg = ctx.bsdf_sample(mat, wol, u)
     wo_, fo, po, co = O.bsdf_sample(mat, wol, u)
-    ew = np.abs(wg - wo_).max(axis=1)
-    ef = rel_err(fg, fo, max(1e-3, float(np.abs(fo).max()) * 1e-6)).max(axis=1)
     finite = np.isfinite(po)
-    pmax = float(np.abs(po[finite]).max()) if finite.any() else 1.0
-    ep = rel_err(pg[finite], po[finite], max(1e-3, pmax * 1e-6)) if finite.any() else np.zeros(1)
-    assert np.all(np.isinf(pg[~finite]))
-    ec = rel_err(cg, co, 1e-6)
-    for e in (ew, ef, ep, ec):
-        assert np.quantile(e, 0.999) <= REL, e.max()
-        # near-degenerate samples (grazing/TIR) may round to a neighbouring branch
-        assert np.mean(e > 1e-3) < 1e-3
+    assert np.array_equal(finite, np.isfinite(pg))
+    # warp: unit directions, component/branch choice
+    ew = np.abs(wg - wo_).max(axis=1)
+    same_branch = np.sign(wg[:, 2]) == np.sign(wo_[:, 2])
+    assert np.mean(same_branch) > 0.999
+    assert np.mean(ew <= REL) > 0.999, (np.quantile(ew, 0.999), ew.max())
+    assert np.mean(ew > 1e-3) < 1e-3
+    # value at the GPU-sampled direction
+    fe, pe, ce = O.bsdf_eval(mat, wg, wol)
+    live = finite & (fg.max(axis=1) > 0) & (fe.max(axis=1) > 0)
+    if live.any():
+        ef = rel_err(fg[live], fe[live], 1e-30).max(axis=1)
+        ep = rel_err(pg[live], pe[live], 1e-30)
+        tol = REL if mat.type in (_ffi.MAT_LAMBERT, _ffi.MAT_PHONG) else 20 * REL
+        print(f"{name}: warp q999 {np.quantile(ew, 0.999):.2e}, value q999 f {np.quantile(ef, 0.999):.2e} "
+              f"pdf {np.quantile(ep, 0.999):.2e}")
+        assert np.quantile(ef, 0.999) <= tol and np.quantile(ep, 0.999) <= tol
+        # where the two warps landed on the same FP32 direction, the sampled values agree to 1e-5
+        same = live & (ew <= 1.2e-7)
+        if same.sum() > 100:
+            assert np.quantile(rel_err(fg[same], fo[same], 1e-30).max(axis=1), 0.999) <= REL
+            assert np.quantile(rel_err(pg[same], po[same], 1e-30), 0.999) <= REL
+    # Dirac components: values compared directly (no direction dependence beyond the branch)
+    if (~finite).any():
+        d = ~finite & same_branch
+        assert np.quantile(rel_err(fg[d], fo[d], 1e-6).max(axis=1), 0.999) <= REL
+        assert np.quantile(rel_err(cg[d], co[d], 1e-6), 0.999) <= REL
 
 
 def em_scene(sampler, w=256, h=128, seed=3, levels=(4, 4)):
@@ -165,7 +188,10 @@ def assert_image_parity(g, o, frac=0.02):
     assert abs(lum_g.sum() - lum_o.sum()) / lum_o.sum() < 5e-3
 
 
-@pytest.mark.parametrize("kernel", [_ffi.KERNEL_MEGAKERNEL])
+KERNELS = [pytest.param(_ffi.KERNEL_MEGAKERNEL, id="megakernel"), pytest.param(_ffi.KERNEL_WAVEFRONT, id="wavefront")]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 def test_workload1_image_parity(ctx, kernel):
     ctx.set_kernel_mode(kernel)
     sb = SceneBuild.workload(1, seed=21, res=48)
@@ -173,11 +199,35 @@ def test_workload1_image_parity(ctx, kernel):
     assert_image_parity(g, o)
 
 
-def test_workload2_image_parity(ctx):
-    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_workload2_image_parity(ctx, kernel):
+    ctx.set_kernel_mode(kernel)
     sb = SceneBuild.workload(2, seed=22, res=40)
     g, o = render_pair(ctx, sb, 4)
     assert_image_parity(g, o, frac=0.03)
+
+
+def test_wavefront_equals_megakernel(ctx):
+    """Both kernel organisations run the same per-vertex code on the same RNG streams."""
+    sb = SceneBuild.workload(2, seed=25, res=64)
+    out = []
+    for k in (_ffi.KERNEL_MEGAKERNEL, _ffi.KERNEL_WAVEFRONT):
+        ctx.set_kernel_mode(k)
+        ctx.upload(sb)
+        ctx.render(0, 8)
+        out.append(ctx.framebuffer(0, sb.resolution)[0])
+    a, b = out
+    rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-6)
+    assert np.mean(rel > 1e-6) < 1e-3
+    assert abs(a.sum() - b.sum()) / a.sum() < 1e-6
+
+
+def test_wavefront_batch_larger_than_queue_capacity(ctx):
+    """A render of more samples than one wavefront batch holds (2^21): two batches, partial tail."""
+    ctx.set_kernel_mode(_ffi.KERNEL_WAVEFRONT)
+    sb = SceneBuild.workload(1, seed=26, res=200)
+    g, o = render_pair(ctx, sb, 60)
+    assert_image_parity(g, o)
 
 
 @pytest.mark.parametrize("alg", ["ptn", "dbs", "dlss", "dlsa", "dmis"])
