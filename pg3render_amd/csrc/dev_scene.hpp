@@ -77,7 +77,11 @@ struct Hit {
     F3 n;
 };
 
+// The intersection routines are built with FMA contraction (the rest of the device code is not):
+// they are the bulk of the arithmetic, and their results enter the parity checks only through
+// whole images, where one-ulp hit distances are invisible.
 PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, float tmin, Hit &h) {       // :103-134
+#pragma clang fp contract(fast)
     const F3 ao = mk(t.p0[0], t.p0[1], t.p0[2]) - org;
     const F3 bo = mk(t.p1[0], t.p1[1], t.p1[2]) - org;
     const F3 co = mk(t.p2[0], t.p2[1], t.p2[2]) - org;
@@ -94,6 +98,7 @@ PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, float tmin, Hit &h) {      
 // :176-245.  B*B - 4*A*C is evaluated in FP32 (all float operands) and widened; the root, q and
 // both roots are FP64 as in the reference.
 PG3_DEV bool SphHit(const DevSphere &s, F3 org, F3 dir, float tmin, Hit &h) {
+#pragma clang fp contract(fast)
     const F3 o = org - mk(s.c[0], s.c[1], s.c[2]);
     const float A = Dot(dir, dir);
     const float B = 2 * Dot(dir, o);

@@ -251,7 +251,7 @@ struct WfStorage {
         mem = nullptr;
         const size_t f4 = sizeof(float4) * (size_t)cap;
         const size_t bytes = 6 * f4 + sizeof(float) * kMaxPickLights * (size_t)cap + f4 + sizeof(int32_t) * (size_t)cap +
-                             2 * sizeof(uint32_t) * (size_t)cap + 3 * f4 + 256;
+                             2 * sizeof(uint32_t) * (size_t)cap + 3 * f4 + 64 + 17 * 256;  // + alignment
         hipError_t e = hipMalloc(&mem, bytes);
         if (e != hipSuccess) return e;
         capacity = cap;
