@@ -29,7 +29,7 @@ struct alignas(16) DevLight {
     F3 radiance;         // area: Le = power*invArea/pi; point: intensity; background: constant radiance
 };
 
-enum : int32_t { kEmNone = 0, kEmCosine = 1, kEmSimple = 2, kEmSteerable = 3 };
+enum : int32_t { kEmAbsent = -1, kEmNone = 0, kEmCosine = 1, kEmSimple = 2, kEmSteerable = 3 };
 constexpr int kMaxTreeLevels = 16;
 
 // Environment map and its sampling tables.
@@ -77,9 +77,9 @@ struct Hit {
     F3 n;
 };
 
-// The intersection routines are built with FMA contraction (the rest of the device code is not):
-// they are the bulk of the arithmetic, and their results enter the parity checks only through
-// whole images, where one-ulp hit distances are invisible.
+// The intersection routines are built with FMA contraction and a fast reciprocal division (the
+// rest of the device code is not): they are the bulk of the arithmetic, and their results enter the
+// parity checks only through whole images, where one-ulp hit distances are invisible.
 PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, float tmin, Hit &h) {       // :103-134
 #pragma clang fp contract(fast)
     const F3 ao = mk(t.p0[0], t.p0[1], t.p0[2]) - org;
@@ -90,7 +90,7 @@ PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, float tmin, Hit &h) {      
     const float v2d = Dot(Cross(ao, co), dir);
     if (((v0d < 0.f) && (v1d < 0.f) && (v2d < 0.f)) || ((v0d >= 0.f) && (v1d >= 0.f) && (v2d >= 0.f))) {
         const F3 n = mk(t.n[0], t.n[1], t.n[2]);
-        const float d = Dot(n, ao) / Dot(n, dir);
+        const float d = __fdividef(Dot(n, ao), Dot(n, dir));
         if ((d > tmin) & (d < h.dist)) { h.dist = d; h.mat = t.mat; h.n = n; return true; }
     }
     return false;
@@ -119,18 +119,55 @@ PG3_DEV bool SphHit(const DevSphere &s, F3 org, F3 dir, float tmin, Hit &h) {
     h.n = Normalize(o + mk(resT) * dir);
     return true;
 }
+// Primitive records are read through the constant address space: every lane of a wavefront
+// tests the same primitive in the same order, so the loads are wave-uniform and become scalar
+// loads into SGPRs (the tables are never written while a kernel runs).
+template <class T> using ConstPtr = const __attribute__((address_space(4))) T *;
+template <class T> PG3_DEV ConstPtr<T> AsConst(const T *p) { return (ConstPtr<T>)(p); }
+typedef float F4v __attribute__((ext_vector_type(4)));
+PG3_DEV DevTri LoadTri(const DevTri *base, uint32_t i) {
+    const ConstPtr<F4v> q = (ConstPtr<F4v>)AsConst(base + i);
+    const F4v a = q[0], b = q[1], c = q[2], d = q[3];
+    DevTri t;
+    t.p0[0] = a.x; t.p0[1] = a.y; t.p0[2] = a.z; t.p1[0] = a.w;
+    t.p1[1] = b.x; t.p1[2] = b.y; t.p2[0] = b.z; t.p2[1] = b.w;
+    t.p2[2] = c.x; t.n[0] = c.y; t.n[1] = c.z; t.n[2] = c.w;
+    t.mat = __float_as_int(d.x);
+    return t;
+}
+PG3_DEV DevSphere LoadSphere(const DevSphere *base, uint32_t i) {
+    const ConstPtr<F4v> q = (ConstPtr<F4v>)AsConst(base + i);
+    const F4v a = q[0], b = q[1];
+    DevSphere s;
+    s.c[0] = a.x; s.c[1] = a.y; s.c[2] = a.z; s.r = a.w;
+    s.mat = __float_as_int(b.x);
+    return s;
+}
+
 PG3_DEV bool Intersect(const SceneView &S, F3 org, F3 dir, float tmin, Hit &h) {  // scene.hxx:42-59
     bool any = false;
-    for (uint32_t i = 0; i < S.numTris; ++i) any |= TriHit(S.tris[i], org, dir, tmin, h);
-    for (uint32_t i = 0; i < S.numSphs; ++i) any |= SphHit(S.sphs[i], org, dir, tmin, h);
+    for (uint32_t i = 0; i < S.numTris; ++i) {
+        const DevTri t = LoadTri(S.tris, i);
+        any |= TriHit(t, org, dir, tmin, h);
+    }
+    for (uint32_t i = 0; i < S.numSphs; ++i) {
+        const DevSphere s = LoadSphere(S.sphs, i);
+        any |= SphHit(s, org, dir, tmin, h);
+    }
     return any;
 }
 PG3_DEV bool Occluded(const SceneView &S, F3 p, F3 dir, float tmax) {              // scene.hxx:61-74
     const F3 org = p + dir * kEpsRay;
     Hit h;
     h.dist = tmax - 2 * kEpsRay;
-    for (uint32_t i = 0; i < S.numTris; ++i) if (TriHit(S.tris[i], org, dir, 0.f, h)) return true;
-    for (uint32_t i = 0; i < S.numSphs; ++i) if (SphHit(S.sphs[i], org, dir, 0.f, h)) return true;
+    for (uint32_t i = 0; i < S.numTris; ++i) {
+        const DevTri t = LoadTri(S.tris, i);
+        if (TriHit(t, org, dir, 0.f, h)) return true;
+    }
+    for (uint32_t i = 0; i < S.numSphs; ++i) {
+        const DevSphere s = LoadSphere(S.sphs, i);
+        if (SphHit(s, org, dir, 0.f, h)) return true;
+    }
     return false;
 }
 
@@ -415,16 +452,17 @@ PG3_DEV float SteerIrradiance(const DevEm &E, F3 n, uint32_t sides) {           
 }
 
 // EnvironmentMap::PdfW of the active sampler (em.hxx:113-120)
-PG3_DEV float EmPdf(const DevEm &E, F3 dir, const Frame &fr, uint32_t sides) {
-    if (E.sampler == kEmSimple) return SimplePdf(E, dir);
-    if (E.sampler == kEmSteerable) return SteerPdf(E, dir, fr.z, sides);
+// `kind` is the sampler (E.sampler at run time, or a constant in specialised kernels).
+PG3_DEV float EmPdf(const DevEm &E, int kind, F3 dir, const Frame &fr, uint32_t sides) {
+    if (kind == kEmSimple) return SimplePdf(E, dir);
+    if (kind == kEmSteerable) return SteerPdf(E, dir, fr.z, sides);
     return CosSpherePdfW(sides & kSideFront, sides & kSideBack, ToLocal(fr, dir));
 }
 // EnvironmentMapSampler::Sample of the active sampler (em_sampler.hxx:49-64)
-PG3_DEV bool EmSample(const DevEm &E, const Frame &fr, uint32_t sides, float u0, float u1, float u2, F3 &dir,
+PG3_DEV bool EmSample(const DevEm &E, int kind, const Frame &fr, uint32_t sides, float u0, float u1, float u2, F3 &dir,
                       float &pdf, F3 &radCos) {
-    if (E.sampler == kEmSimple) { SimpleSample(E, fr, sides, u0, u1, dir, pdf, radCos); return true; }
-    if (E.sampler == kEmSteerable) return SteerSample(E, fr.z, sides, u2, u0, u1, dir, pdf, radCos);
+    if (kind == kEmSimple) { SimpleSample(E, fr, sides, u0, u1, dir, pdf, radCos); return true; }
+    if (kind == kEmSteerable) return SteerSample(E, fr.z, sides, u2, u0, u1, dir, pdf, radCos);
     const F3 w = SampleCosSphereParamPdfW(u0, u1, u2, sides & kSideFront, sides & kSideBack, pdf);
     dir = ToWorld(fr, w);
     radCos = EmEval(E, dir) * fabsf(w.z);
@@ -465,13 +503,24 @@ PG3_DEV void PointCompute(const DevLight &L, F3 sp, F3 n, uint32_t sides, LightS
     o.pdfW = kInf;
     o.prob = 1.0f;
 }
+// What a kernel knows about the scene's lights: the EM sampler (kEmAbsent when the background
+// has no image) and the light count.  Generic kernels read both from the scene; specialised
+// kernels pass constants, which removes the other branches at compile time.
+struct LightSpec {
+    int em;
+    uint32_t numLights;
+};
+PG3_HD LightSpec SceneLightSpec(const SceneView &S) {
+    return LightSpec{S.em.w > 0 ? S.em.sampler : (int)kEmAbsent, S.numLights};
+}
+
 // BackgroundLight::GetEmmision(dir), lights.hxx:379-407 (radiance only)
-PG3_DEV F3 BgRadiance(const SceneView &S, F3 dir) {
-    return (S.em.w > 0) ? EmEval(S.em, dir) : S.lights[S.bgLight].radiance;
+PG3_DEV F3 BgRadiance(const SceneView &S, const LightSpec &ls, F3 dir) {
+    return (ls.em != kEmAbsent) ? EmEval(S.em, dir) : S.lights[S.bgLight].radiance;
 }
 // ... and its PDF w.r.t. a shading frame/material
-PG3_DEV float BgPdf(const SceneView &S, F3 dir, const Frame &fr, uint32_t sides) {
-    if (S.em.w > 0) return EmPdf(S.em, dir, fr, sides);
+PG3_DEV float BgPdf(const SceneView &S, const LightSpec &ls, F3 dir, const Frame &fr, uint32_t sides) {
+    if (ls.em != kEmAbsent) return EmPdf(S.em, ls.em, dir, fr, sides);
     return CosSpherePdfW(sides & kSideFront, sides & kSideBack, ToLocal(fr, dir));
 }
 // AreaLight::GetEmmision PDF part, lights.hxx:88-99
@@ -503,7 +552,7 @@ PG3_DEV float EmEstimateMC(const SceneView &S, const Frame &fr, uint32_t sides, 
     return sum / 10;
 }
 // AbstractLight::EstimateContribution
-PG3_DEV float EstimateContribution(const SceneView &S, uint32_t li, F3 sp, const Frame &fr, uint32_t sides,
+PG3_DEV float EstimateContribution(const SceneView &S, const LightSpec &spec, uint32_t li, F3 sp, const Frame &fr, uint32_t sides,
                                    const RngKey &key, uint32_t vtx) {
     const DevLight &L = S.lights[li];
     if (L.type == kLightArea) {                                                        // :133-167
@@ -522,14 +571,14 @@ PG3_DEV float EstimateContribution(const SceneView &S, uint32_t li, F3 sp, const
         PointCompute(L, sp, fr.z, sides, s);
         return Luminance(s.sample);
     }
-    if (S.em.w > 0) {                                                                  // :451-467
-        if (S.em.sampler == kEmSteerable) return SteerIrradiance(S.em, fr.z, sides);
+    if (spec.em != kEmAbsent) {                                                        // :451-467
+        if (spec.em == kEmSteerable) return SteerIrradiance(S.em, fr.z, sides);
         return EmEstimateMC(S, fr, sides, key, vtx);
     }
     return Luminance(L.radiance) * kPiF;
 }
 // AbstractLight::SampleIllumination
-PG3_DEV bool SampleIllumination(const SceneView &S, uint32_t li, F3 sp, const Frame &fr, uint32_t sides,
+PG3_DEV bool SampleIllumination(const SceneView &S, const LightSpec &spec, uint32_t li, F3 sp, const Frame &fr, uint32_t sides,
                                 float u0, float u1, float u2, LightSample &o) {
     const DevLight &L = S.lights[li];
     if (L.type == kLightArea) {                                                        // :114-128
@@ -540,7 +589,7 @@ PG3_DEV bool SampleIllumination(const SceneView &S, uint32_t li, F3 sp, const Fr
     if (L.type == kLightPoint) { PointCompute(L, sp, fr.z, sides, o); return true; }
     o.prob = 1.0f;
     o.dist = 3.402823466e+38f;   // em_sampler.hxx:57
-    if (S.em.w > 0) return EmSample(S.em, fr, sides, u0, u1, u2, o.wig, o.pdfW, o.sample);
+    if (spec.em != kEmAbsent) return EmSample(S.em, spec.em, fr, sides, u0, u1, u2, o.wig, o.pdfW, o.sample);
     const F3 w = SampleCosSphereParamPdfW(u0, u1, u2, sides & kSideFront, sides & kSideBack, o.pdfW);  // em_cosine_sampler.hxx:12-33
     o.wig = ToWorld(fr, w);
     o.sample = L.radiance * fabsf(w.z);

@@ -60,6 +60,77 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
     }
 }
 
+// Path tracer (-sb 1) specialised on the EM sampler and the light count, with path
+// regeneration: a lane whose path ends starts its pixel's next iteration at once instead of
+// idling until the longest path of its wavefront ends.  The samples, their RNG streams and the
+// per-pixel sums are those of RenderMegakernel; only the order of the vertex loop changes.
+//   kEmKind: kEmAbsent = no EM image (constant or no background), else the sampler of the EM.
+//   kOneLight: the scene has exactly one light (light picking is trivial, no contribution cache).
+// Register budget of the path-tracing kernel: 4 waves per SIMD (128 VGPRs) measured fastest on
+// the box + steerable-EM workload (1, 3, 4 and 5 waves tried); the excess spills to scratch.
+#ifndef PG3_PT_WAVES_PER_EU
+#define PG3_PT_WAVES_PER_EU 4
+#endif
+template <int kEmKind, bool kOneLight>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PG3_PT_WAVES_PER_EU)))
+RenderPTKernel(SceneView S, uint32_t firstIter, uint32_t nIter,
+                                                       uint32_t itersPerThread, uint32_t numChunks,
+                                                       double *__restrict__ fb,
+                                                       unsigned long long *__restrict__ counters) {
+    const LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
+    const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
+    const uint32_t numTiles = tilesX * tilesY;
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = (uint32_t)(gid & 63u);
+    const uint64_t waveId = gid >> 6;
+    const uint32_t tile = (uint32_t)(waveId % numTiles);
+    const uint32_t chunk = (uint32_t)(waveId / numTiles);
+    const uint32_t x = (tile % tilesX) * 8u + (lane & 7u);
+    const uint32_t y = (tile / tilesX) * 8u + (lane >> 3);
+    uint32_t segments = 0, samples = 0;
+    if (chunk < numChunks && x < S.resX && y < S.resY) {
+        const uint32_t pixel = y * S.resX + x;
+        uint32_t it = firstIter + chunk * itersPerThread;
+        const uint32_t itEnd = min(firstIter + nIter, it + itersPerThread);
+        if (it < itEnd) {
+            double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+            RngKey key;
+            PathState ps;
+            {
+                F3 org, dir;
+                CameraSample(S, pixel, it, key, org, dir);
+                InitPath(ps, org, dir);
+            }
+            for (;;) {
+                Hit h;
+                h.dist = 1e36f;
+                const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
+                ShadowReq sr;
+                const bool cont = ShadeVertex(S, spec, key, ps, hit, h, sr, segments);
+                if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
+                if (!cont) {
+                    acc0 += ps.L.x;
+                    acc1 += ps.L.y;
+                    acc2 += ps.L.z;
+                    ++samples;
+                    if (++it >= itEnd) break;
+                    F3 org, dir;
+                    CameraSample(S, pixel, it, key, org, dir);
+                    InitPath(ps, org, dir);
+                }
+            }
+            atomicAdd(fb + (size_t)pixel * 3 + 0, acc0);
+            atomicAdd(fb + (size_t)pixel * 3 + 1, acc1);
+            atomicAdd(fb + (size_t)pixel * 3 + 2, acc2);
+        }
+    }
+    const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
+    if (lane == 0 && counters) {
+        atomicAdd(counters + 0, (unsigned long long)segSum);
+        atomicAdd(counters + 1, (unsigned long long)smpSum);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Function-level kernels (tier-1 parity of BSDF and EM functions)
 __global__ void BsdfEvalKernel(DevMaterial m, uint32_t n, const float *__restrict__ wil, const float *__restrict__ wol,
@@ -90,7 +161,7 @@ __global__ void EmSampleKernel(SceneView S, uint32_t n, const float *__restrict_
     const Frame fr = FrameFromZ(mk(nrm[i * 3], nrm[i * 3 + 1], nrm[i * 3 + 2]));
     F3 d = mk(0.f), radCos = mk(0.f);
     float p = 0.f;
-    const bool good = EmSample(S.em, fr, (uint32_t)sides[i], u[i * 3], u[i * 3 + 1], u[i * 3 + 2], d, p, radCos);
+    const bool good = EmSample(S.em, S.em.sampler, fr, (uint32_t)sides[i], u[i * 3], u[i * 3 + 1], u[i * 3 + 2], d, p, radCos);
     ok[i] = good ? 1 : 0;
     dir[i * 3] = d.x; dir[i * 3 + 1] = d.y; dir[i * 3 + 2] = d.z;
     pdf[i] = p;
@@ -101,7 +172,7 @@ __global__ void EmPdfKernel(SceneView S, uint32_t n, const float *__restrict__ d
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Frame fr = FrameFromZ(mk(nrm[i * 3], nrm[i * 3 + 1], nrm[i * 3 + 2]));
-    pdf[i] = EmPdf(S.em, mk(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]), fr, (uint32_t)sides[i]);
+    pdf[i] = EmPdf(S.em, S.em.sampler, mk(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]), fr, (uint32_t)sides[i]);
 }
 __global__ void EmEvalKernel(SceneView S, uint32_t n, const float *__restrict__ dirs, float *__restrict__ rgb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -222,6 +293,23 @@ __global__ void __launch_bounds__(256) CdfMarginalKernel(uint32_t h, const float
 
 // ---------------------------------------------------------------------------------------------
 // Launchers (host)
+using RenderKernelFn = void (*)(SceneView, uint32_t, uint32_t, uint32_t, uint32_t, double *, unsigned long long *);
+
+// Kernel for a scene: the specialised path tracer for `pt`, the generic kernel otherwise.
+static RenderKernelFn PickRenderKernel(const SceneView &S) {
+    if (S.algorithm != 7) return RenderMegakernel;
+    const bool one = (S.numLights == 1);
+    const int em = SceneLightSpec(S).em;
+#define PG3_PT_CASE(K)                                                                        \
+    if (em == (K)) return one ? RenderPTKernel<(K), true> : RenderPTKernel<(K), false>;
+    PG3_PT_CASE(kEmAbsent)
+    PG3_PT_CASE(kEmCosine)
+    PG3_PT_CASE(kEmSimple)
+    PG3_PT_CASE(kEmSteerable)
+#undef PG3_PT_CASE
+    return RenderMegakernel;
+}
+
 hipError_t LaunchMegakernel(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
                             unsigned long long *counters, hipStream_t stream) {
     if (nIter == 0) return hipSuccess;
@@ -235,8 +323,8 @@ hipError_t LaunchMegakernel(const SceneView &S, uint32_t firstIter, uint32_t nIt
     const uint32_t tiles = ((S.resX + 7u) / 8u) * ((S.resY + 7u) / 8u);
     const uint64_t threads = (uint64_t)tiles * 64u * numChunks;
     const uint32_t blocks = (uint32_t)((threads + 255) / 256);
-    hipLaunchKernelGGL(RenderMegakernel, dim3(blocks), dim3(256), 0, stream, S, firstIter, nIter, ipt, numChunks, fb,
-                       counters);
+    hipLaunchKernelGGL(PickRenderKernel(S), dim3(blocks), dim3(256), 0, stream, S, firstIter, nIter, ipt, numChunks,
+                       fb, counters);
     return hipGetLastError();
 }
 

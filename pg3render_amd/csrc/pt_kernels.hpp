@@ -23,29 +23,29 @@ struct LightCache {            // LightSamplingContext, path_tracer_base.hxx:22-
     float est[kMaxPickLights];
     float sum;
 };
-PG3_DEV void FillLightCache(const SceneView &S, F3 sp, const Frame &fr, uint32_t sides, const RngKey &key,
-                            uint32_t vtx, LightCache &lc) {
+PG3_DEV void FillLightCache(const SceneView &S, const LightSpec &spec, F3 sp, const Frame &fr, uint32_t sides,
+                            const RngKey &key, uint32_t vtx, LightCache &lc) {
     lc.sum = 0.f;
 #pragma unroll
     for (uint32_t i = 0; i < kMaxPickLights; ++i) {
         float e = 0.f;
-        if (i < S.numLights) e = EstimateContribution(S, i, sp, fr, sides, key, vtx);
+        if (i < spec.numLights) e = EstimateContribution(S, spec, i, sp, fr, sides, key, vtx);
         lc.est[i] = e;
         lc.sum += e;
     }
 }
 // LightPickingProbability, path_tracer_base.hxx:260-313
-PG3_DEV float PickProb(const SceneView &S, const LightCache &lc, uint32_t id) {
-    if (S.numLights == 0) return 0.f;
-    if (S.numLights == 1) return 1.f;
+PG3_DEV float PickProb(const LightSpec &spec, const LightCache &lc, uint32_t id) {
+    if (spec.numLights == 0) return 0.f;
+    if (spec.numLights == 1) return 1.f;
     float e = 0.f;
 #pragma unroll
     for (uint32_t i = 0; i < kMaxPickLights; ++i) if (i == id) e = lc.est[i];
-    return (lc.sum > 0.f) ? e / lc.sum : 1.f / S.numLights;
+    return (lc.sum > 0.f) ? e / lc.sum : 1.f / spec.numLights;
 }
 // PickSingleLight, path_tracer_base.hxx:191-257
-PG3_DEV int32_t PickLight(const SceneView &S, const LightCache &lc, float u, float &prob) {
-    const uint32_t n = S.numLights;
+PG3_DEV int32_t PickLight(const LightSpec &spec, const LightCache &lc, float u, float &prob) {
+    const uint32_t n = spec.numLights;
     if (n == 0) return -1;
     if (n == 1) { prob = 1.f; return 0; }
     if (lc.sum > 0.f) {
@@ -113,20 +113,20 @@ struct ShadowReq {
 // One vertex of EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with one light sample and
 // one BSDF sample: adds the MIS-weighted emission found by the previous BSDF sample, emits the
 // light-sample request and samples the BSDF.  Returns true if the path continues.
-PG3_DEV bool ShadeVertex(const SceneView &S, const RngKey &key, PathState &ps, bool hit, const Hit &h,
+PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey &key, PathState &ps, bool hit, const Hit &h,
                          ShadowReq &sr, uint32_t &segments) {
     sr.valid = false;
     const uint32_t len = ps.len;
     const bool hasPrev = ps.flags & kHasPrev, prevFinite = ps.flags & kPrevFinite;
     if (!hit) {
         if (len >= S.minPL && S.bgLight >= 0) {                                       // :408-422
-            const F3 Le = BgRadiance(S, ps.dir);
+            const F3 Le = BgRadiance(S, spec, ps.dir);
             if (!hasPrev) {
                 ps.L += Le;
             } else if (!IsZero(Le)) {
                 if (prevFinite) {
-                    const float lp = BgPdf(S, ps.dir, FrameFromZ(ps.prevN), ps.flags >> 4) *
-                                     PickProb(S, ps.lc, (uint32_t)S.bgLight);
+                    const float lp = BgPdf(S, spec, ps.dir, FrameFromZ(ps.prevN), ps.flags >> 4) *
+                                     PickProb(spec, ps.lc, (uint32_t)S.bgLight);
                     ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
                 } else {
                     ps.L += ps.eThr * Le;
@@ -147,7 +147,7 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const RngKey &key, PathState &ps, b
             ps.L += Le;
         } else if (!IsZero(Le)) {                                                     // :327-369
             if (prevFinite) {
-                const float lp = AreaEmissionPdf(Lt, sp, ps.org) * PickProb(S, ps.lc, (uint32_t)mat.lightId);
+                const float lp = AreaEmissionPdf(Lt, sp, ps.org) * PickProb(spec, ps.lc, (uint32_t)mat.lightId);
                 ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
             } else {
                 ps.L += ps.eThr * Le;
@@ -163,12 +163,12 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const RngKey &key, PathState &ps, b
     VertexRng vr;
     RngVertex(key, len, vr);
     const uint32_t sides = mat.sides;
-    if (S.numLights > 1) FillLightCache(S, sp, fr, sides, key, len, ps.lc);
+    if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, sides, key, len, ps.lc);
     if ((len + 1) >= S.minPL) {                                                       // :252-265
         float prob = 1.f;
-        const int32_t li = PickLight(S, ps.lc, vr.u[kDimLightPick], prob);
+        const int32_t li = PickLight(spec, ps.lc, vr.u[kDimLightPick], prob);
         LightSample ls;
-        if (li >= 0 && SampleIllumination(S, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
+        if (li >= 0 && SampleIllumination(S, spec, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
                                           vr.u[kDimLight2], ls)) {
             ls.prob = prob;
             if (MaxC(ls.sample) > 0.f) {                                              // :350-425
@@ -218,6 +218,7 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const RngKey &key, PathState &ps, b
 
 // One sample of PathTracer::EstimateIncomingRadiancePT with -sb 1, traced in one thread.
 PG3_DEV F3 RadiancePT(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
+    const LightSpec spec = SceneLightSpec(S);
     PathState ps;
     InitPath(ps, org, dir);
     for (;;) {
@@ -225,7 +226,7 @@ PG3_DEV F3 RadiancePT(const SceneView &S, const RngKey &key, F3 org, F3 dir, uin
         h.dist = 1e36f;
         const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
         ShadowReq sr;
-        const bool cont = ShadeVertex(S, key, ps, hit, h, sr, segments);
+        const bool cont = ShadeVertex(S, spec, key, ps, hit, h, sr, segments);
         if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
         if (!cont) break;
     }
@@ -234,13 +235,14 @@ PG3_DEV F3 RadiancePT(const SceneView &S, const RngKey &key, F3 org, F3 dir, uin
 
 // PathTracer::EstimateIncomingRadiancePTNaive, path_tracer.hxx:45-165
 PG3_DEV F3 RadiancePTNaive(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
+    const LightSpec spec = SceneLightSpec(S);
     F3 out = mk(0.f), thr = mk(1.0f);
     float tmin = 0.f;
     for (uint32_t len = 1;; ++len) {
         Hit h;
         h.dist = 1e36f;
         if (!Intersect(S, org, dir, tmin, h)) {
-            if (len >= S.minPL && S.bgLight >= 0) out += BgRadiance(S, dir) * thr;
+            if (len >= S.minPL && S.bgLight >= 0) out += BgRadiance(S, spec, dir) * thr;
             return out;
         }
         const F3 sp = org + dir * h.dist;
@@ -275,9 +277,10 @@ PG3_DEV F3 RadiancePTNaive(const SceneView &S, const RngKey &key, F3 org, F3 dir
 
 // DirectIllumination::EstimateIncomingRadiance, direct_illumination.hxx:15-156
 PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
+    const LightSpec spec = SceneLightSpec(S);
     Hit h;
     h.dist = 1e36f;
-    if (!Intersect(S, org, dir, 0.f, h)) return (S.bgLight >= 0) ? BgRadiance(S, dir) : mk(0.f);
+    if (!Intersect(S, org, dir, 0.f, h)) return (S.bgLight >= 0) ? BgRadiance(S, spec, dir) : mk(0.f);
     ++segments;
     const F3 sp = org + dir * h.dist;
     const Frame fr = FrameFromZ(h.n);
@@ -289,7 +292,7 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
     RngVertex(key, vtx, vr);
     LightCache lc;
     lc.sum = 0.f;
-    if (S.numLights > 1 && S.algorithm != 3) FillLightCache(S, sp, fr, sides, key, vtx, lc);
+    if (S.numLights > 1 && S.algorithm != 3) FillLightCache(S, spec, sp, fr, sides, key, vtx, lc);
     F3 Lo = mk(0.f);
     const int alg = S.algorithm;
     if (alg == 3) {                                    // dlsa: every light once
@@ -297,7 +300,7 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
             LightSample ls;
             const float u0 = RngGet(key, vtx, i, kDimLight0), u1 = RngGet(key, vtx, i, kDimLight1),
                         u2 = RngGet(key, vtx, i, kDimLight2);
-            if (SampleIllumination(S, i, sp, fr, sides, u0, u1, u2, ls) && MaxC(ls.sample) > 0.f &&
+            if (SampleIllumination(S, spec, i, sp, fr, sides, u0, u1, u2, ls) && MaxC(ls.sample) > 0.f &&
                 !Occluded(S, sp, ls.wig, ls.dist)) {                                    // :315-348
                 BsdfRec r;
                 EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
@@ -306,9 +309,9 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
         }
     } else if (alg == 4 || alg == 5) {                 // dlss / dmis light part
         float prob = 1.f;
-        const int32_t li = PickLight(S, lc, vr.u[kDimLightPick], prob);
+        const int32_t li = PickLight(spec, lc, vr.u[kDimLightPick], prob);
         LightSample ls;
-        if (li >= 0 && SampleIllumination(S, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
+        if (li >= 0 && SampleIllumination(S, spec, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
                                           vr.u[kDimLight2], ls)) {
             ls.prob = prob;
             if (MaxC(ls.sample) > 0.f && !Occluded(S, sp, ls.wig, ls.dist)) {
@@ -349,8 +352,8 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
                     lid = l2;
                 }
             } else if (S.bgLight >= 0) {
-                Li = BgRadiance(S, wig);
-                lpdf = BgPdf(S, wig, fr, sides);
+                Li = BgRadiance(S, spec, wig);
+                lpdf = BgPdf(S, spec, wig, fr, sides);
                 lid = S.bgLight;
             }
             const float cosAbs = fabsf(r.wil.z);
@@ -359,7 +362,7 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
             } else if (MaxC(Li) > 0.f) {                                                  // :427-490
                 if (r.pdfW != kInf) {
                     const float bp = r.pdfW * r.compProb;
-                    const float pick = (lid >= 0) ? PickProb(S, lc, (uint32_t)lid) : 0.f;
+                    const float pick = (lid >= 0) ? PickProb(spec, lc, (uint32_t)lid) : 0.f;
                     Lo += (r.att * cosAbs * Li * MisPower(bp, lpdf * pick)) / bp;
                 } else {
                     Lo += (r.att * Li) / (1.f * r.compProb);
@@ -375,12 +378,19 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
     return Le + Lo;
 }
 
-PG3_DEV F3 RadianceSample(const SceneView &S, uint32_t pixel, uint32_t iteration, uint32_t &segments) {
-    RngKey key{S.k0, S.k1, pixel, iteration, S.sceneId << 16};
+// Camera sample `iteration` of `pixel`: jitter, primary ray (PathTracerBase::RunIteration,
+// path_tracer_base.hxx:66-80) and the RNG key of its path.
+PG3_DEV void CameraSample(const SceneView &S, uint32_t pixel, uint32_t iteration, RngKey &key, F3 &org, F3 &dir) {
+    key = RngKey{S.k0, S.k1, pixel, iteration, S.sceneId << 16};
     const float jx = RngGet(key, 0, 0, 0), jy = RngGet(key, 0, 0, 1);   // path_tracer_base.hxx:74-77
     const uint32_t x = pixel % S.resX, y = pixel / S.resX;
-    F3 org, dir;
     GenerateRay(S, float(x) + jx, float(y) + jy, org, dir);
+}
+
+PG3_DEV F3 RadianceSample(const SceneView &S, uint32_t pixel, uint32_t iteration, uint32_t &segments) {
+    RngKey key;
+    F3 org, dir;
+    CameraSample(S, pixel, iteration, key, org, dir);
     if (S.algorithm == 7) return RadiancePT(S, key, org, dir, segments);
     if (S.algorithm == 6) return RadiancePTNaive(S, key, org, dir, segments);
     return RadianceDirect(S, key, org, dir, segments);

@@ -144,7 +144,12 @@ __global__ void __launch_bounds__(256) WfIntersect(SceneView S, WfQueues Q, uint
     }
 }
 
-__global__ void __launch_bounds__(256) WfShade(SceneView S, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce) {
+// Shading stage, specialised like the fused kernel (kEmKind: EM sampler or kEmAbsent; kOneLight)
+// and kept at 4 waves per SIMD.
+template <int kEmKind, bool kOneLight>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+WfShade(SceneView S, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce) {
+    const LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
     const uint32_t count = Q.counts[qi];
     const uint32_t *q = Q.rayQ[qi];
     uint32_t segments = 0;
@@ -159,7 +164,7 @@ __global__ void __launch_bounds__(256) WfShade(SceneView S, WfQueues Q, uint32_t
             i = q[j];
             PathState ps;
             uint32_t pixel, it;
-            LoadPath(Q, i, ps, pixel, it, S.numLights);
+            LoadPath(Q, i, ps, pixel, it, spec.numLights);
             ps.len = bounce;
             const float4 ha = Q.hA[i];
             Hit h;
@@ -167,9 +172,9 @@ __global__ void __launch_bounds__(256) WfShade(SceneView S, WfQueues Q, uint32_t
             h.n = mk(ha.x, ha.y, ha.z);
             h.mat = Q.hMat[i];
             RngKey key{S.k0, S.k1, pixel, it, S.sceneId << 16};
-            cont = ShadeVertex(S, key, ps, h.mat >= 0, h, sr, segments);
+            cont = ShadeVertex(S, spec, key, ps, h.mat >= 0, h, sr, segments);
             shadow = sr.valid;
-            StorePath(Q, i, ps, pixel, it, S.numLights);
+            StorePath(Q, i, ps, pixel, it, spec.numLights);
         }
         // queue appends are wave-uniform calls (inactive tail lanes pass false)
         const uint32_t rs = WaveAppend(cont, &Q.counts[qo]);
@@ -273,6 +278,20 @@ struct WfStorage {
 
 }  // namespace
 
+using ShadeFn = void (*)(SceneView, WfQueues, uint32_t, uint32_t, uint32_t);
+static ShadeFn PickShadeKernel(const SceneView &S) {
+    const bool one = (S.numLights == 1);
+    const int em = SceneLightSpec(S).em;
+#define PG3_WF_CASE(K) \
+    if (em == (K)) return one ? WfShade<(K), true> : WfShade<(K), false>;
+    PG3_WF_CASE(kEmAbsent)
+    PG3_WF_CASE(kEmCosine)
+    PG3_WF_CASE(kEmSimple)
+    PG3_WF_CASE(kEmSteerable)
+#undef PG3_WF_CASE
+    return nullptr;
+}
+
 struct WavefrontContext {
     WfStorage st;
     uint32_t capacity = 1u << 21;   // samples per batch (~2M paths, ~300 MB of queues)
@@ -300,6 +319,8 @@ hipError_t LaunchWavefront(const SceneView &S, uint32_t firstIter, uint32_t nIte
     hipError_t e = W->st.Ensure(cap);
     if (e != hipSuccess) return e;
     const WfQueues &Q = W->st.q;
+    const ShadeFn shade = PickShadeKernel(S);
+    if (!shade) return LaunchMegakernel(S, firstIter, nIter, fb, counters, stream);
     const dim3 grid(W->gridBlocks), block(256);
     for (uint64_t base = 0; base < total; base += cap) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(cap, total - base);
@@ -310,7 +331,7 @@ hipError_t LaunchWavefront(const SceneView &S, uint32_t firstIter, uint32_t nIte
             hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, qi);
             hipLaunchKernelGGL(WfSetCounts, dim3(1), dim3(64), 0, stream, Q.counts, 0u, 0u, 0u, 0u,
                                (qo == 0 ? 1u : 2u) | 4u);
-            hipLaunchKernelGGL(WfShade, grid, block, 0, stream, S, Q, qi, qo, b);
+            hipLaunchKernelGGL(shade, grid, block, 0, stream, S, Q, qi, qo, b);
             hipLaunchKernelGGL(WfShadow, grid, block, 0, stream, S, Q);
         }
         hipLaunchKernelGGL(WfAccumulate, grid, block, 0, stream, Q, count, fb, counters);
