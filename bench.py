@@ -31,6 +31,8 @@ def parse():
     ap.add_argument("--kernel", choices=["megakernel", "wavefront"], default="megakernel")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-vs-plain measurement")
+    ap.add_argument("--batch-tune", type=int, default=5, help="autotuner candidates for the batch (0 = all)")
     return ap.parse_args()
 
 
@@ -50,6 +52,52 @@ def cpu_baseline(sb, spp, target_s):
     samples = rows * w * spp
     return {"value": samples / t / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": f"rows {r0}-{r0 + rows - 1} of {h} ({rows * w} pixels) x {spp} spp, {t:.1f} s wall"}
+
+
+def batch_measure(ctx, tune_candidates):
+    """64-scene batch workload: per-scene megakernel launches vs one persistent batched pass.
+
+    Both render the same samples (each scene's spp iterations 0..spp-1, same RNG streams), so
+    they reach the same accuracy; the ratio of their rates is the batching speedup."""
+    from pg3render_amd import Batch, SceneBuild, _ffi
+    scenes = [SceneBuild.workload(3, seed=2024, variant=v) for v in range(64)]
+    slots = list(range(1, 65))
+    for s, sb in zip(slots, scenes):
+        ctx.upload(sb, s)
+    spp = [sb.spp for sb in scenes]
+    total = sum(sb.resolution[0] * sb.resolution[1] * sb.spp for sb in scenes)
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+
+    def plain_pass():
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for s, n in zip(slots, spp):
+            ctx.render(0, n, s)
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    plain_pass()                                   # warm-up (code objects, caches)
+    plain_s = min(plain_pass() for _ in range(2))
+    batch = Batch(ctx, slots, spp)
+    t0 = time.perf_counter()
+    batch.autotune(tune_candidates)
+    tune_s = time.perf_counter() - t0
+    batch.render(1)                                # warm-up of the captured graph
+    ctx.synchronize()
+    walls, gpu_ms = [], []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        gpu_ms.append(batch.render_timed(1))
+        walls.append(time.perf_counter() - t0)
+    batched_s = min(walls)
+    cfg = batch.config()
+    batch.close()
+    return {"workload": "64 seeded box scenes (2 area-light triangles + 256x128 sky/sun EM, steerable), "
+                        "side 64..512, spp in {4,16,64,256}, maxpl 8", "samples_per_pass": total,
+            "plain_per_scene_msamples_per_s": round(total / plain_s / 1e6, 3),
+            "batched_msamples_per_s": round(total / batched_s / 1e6, 3),
+            "speedup": round(plain_s / batched_s, 3), "batched_gpu_ms_per_pass": round(min(gpu_ms), 3),
+            "plain_ms_per_pass": round(plain_s * 1e3, 3), "autotuned_config": cfg, "autotune_s": round(tune_s, 3)}
 
 
 def main():
@@ -106,6 +154,8 @@ def main():
                                   "spheres, pt -sb 1 -iic 0 -maxpl 10", "resolution": [w, h], "spp": spp,
                       "kernel": args.kernel},
            "rays_per_s": round(rays_per_s, 1), "roofline": roofline}
+    if not args.no_batch:
+        out["batched"] = batch_measure(ctx, args.batch_tune)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sb, spp, args.cpu_seconds)
     print(json.dumps(out))

@@ -218,6 +218,37 @@ int pg3g_steerable_leaves(pg3g_context *ctx, float *leaf_dirs);
 /* Root weight (9 SH floats) = sum of all leaf weights grouped as the 6-ary tree does. */
 int pg3g_steerable_root_weight(pg3g_context *ctx, float *weight9);
 
+/* ---- batched multi-scene rendering ------------------------------------------------------ */
+/* No counterpart in the reference, which renders one scene per process (src/pg3render.cxx:
+ * 54-174).  A batch renders many uploaded slots in ONE persistent launch per pass: every scene
+ * is cut into work items (an 8x8 pixel tile x a run of iterations) that resident wavefronts take
+ * from a device counter.  Pass p renders iterations [p*spp[i], (p+1)*spp[i]) of scene i, traced
+ * exactly like pg3g_render_slot (same samples), into the slot's framebuffer; a pass is a
+ * captured hipGraph (reset counter, persistent kernel, advance pass) replayed per call.
+ * Scenes must use the path tracer (`pt`).  A batch keeps copies of its slots' scene tables:
+ * uploading a scene or setting parameters on the context invalidates every batch of it
+ * (pg3g_batch_render then fails with PG3G_ERR_NO_SCENE). */
+typedef struct pg3g_batch pg3g_batch;
+typedef struct pg3g_batch_config {
+    uint32_t iters_per_item;   /* iterations of one tile per work item (>= 1) */
+    uint32_t waves_per_cu;     /* persistent wavefronts per compute unit (multiple of 4, >= 4) */
+    uint32_t use_graph;        /* 1: replay the captured hipGraph; 0: launch the three kernels */
+} pg3g_batch_config;
+
+int pg3g_batch_create(pg3g_context *ctx, uint32_t num_scenes, const uint32_t *slots,
+                      const uint32_t *spp, pg3g_batch **batch);
+int pg3g_batch_destroy(pg3g_batch *batch);
+int pg3g_batch_set_config(pg3g_batch *batch, const pg3g_batch_config *config);
+int pg3g_batch_get_config(const pg3g_batch *batch, pg3g_batch_config *config);
+/* Renders `passes` passes (asynchronous). */
+int pg3g_batch_render(pg3g_batch *batch, uint32_t passes);
+/* Same, bracketed by HIP events on the context stream; waits and returns the GPU time. */
+int pg3g_batch_render_timed(pg3g_batch *batch, uint32_t passes, float *elapsed_ms);
+/* Autotuner: times one pass for each candidate configuration (iterations per item x
+ * persistent wavefronts per CU, at most `max_candidates`, 0 = all), keeps the fastest, then
+ * clears the batch's framebuffers and restarts at pass 0.  Returns the best pass time. */
+int pg3g_batch_autotune(pg3g_batch *batch, uint32_t max_candidates, float *best_pass_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,6 @@ only binds it.  There is no CPU fallback: if the library is not built, importing
 raises.
 """
 from . import _ffi
-from .renderer import Context, GpuPathTracer, SceneBuild
+from .renderer import Batch, Context, GpuPathTracer, SceneBuild
 
-__all__ = ["Context", "GpuPathTracer", "SceneBuild", "_ffi"]
+__all__ = ["Batch", "Context", "GpuPathTracer", "SceneBuild", "_ffi"]

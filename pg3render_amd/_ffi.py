@@ -70,6 +70,10 @@ class SceneDesc(C.Structure):
                 ("camera", CameraDesc), ("params", RenderParams)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("iters_per_item", C.c_uint32), ("waves_per_cu", C.c_uint32), ("use_graph", C.c_uint32)]
+
+
 _P = C.c_void_p
 _FP = C.POINTER(C.c_float)
 _IP = C.POINTER(C.c_int32)
@@ -103,6 +107,13 @@ _SIGNATURES = {
     "pg3g_steerable_info": (C.c_int, [_P, _U32P, _U32P, _U32P]),
     "pg3g_steerable_leaves": (C.c_int, [_P, _FP]),
     "pg3g_steerable_root_weight": (C.c_int, [_P, _FP]),
+    "pg3g_batch_create": (C.c_int, [_P, C.c_uint32, _U32P, _U32P, C.POINTER(_P)]),
+    "pg3g_batch_destroy": (C.c_int, [_P]),
+    "pg3g_batch_set_config": (C.c_int, [_P, C.POINTER(BatchConfig)]),
+    "pg3g_batch_get_config": (C.c_int, [_P, C.POINTER(BatchConfig)]),
+    "pg3g_batch_render": (C.c_int, [_P, C.c_uint32]),
+    "pg3g_batch_render_timed": (C.c_int, [_P, C.c_uint32, _FP]),
+    "pg3g_batch_autotune": (C.c_int, [_P, C.c_uint32, _FP]),
     "pg3g_build_workload_scene": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "pg3g_build_reference_scene": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "pg3g_scene_build_desc": (C.POINTER(SceneDesc), [_P]),

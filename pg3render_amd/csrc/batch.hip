@@ -1,0 +1,93 @@
+// Batched multi-scene path tracing: one persistent launch renders one pass (each scene's spp
+// iterations) of many scenes of uneven resolution and sample count.  The reference renders one
+// scene per process (pg3render.cxx:54-174, one AbstractRenderer per thread); here every scene is
+// a set of work items (one 8x8 pixel tile x a run of iterations) that resident wavefronts take
+// from a device counter until none is left.  Each item is traced exactly as the per-scene kernel
+// traces it (TracePixel: same samples, same RNG streams), so a batched pass and per-scene renders
+// of the same iterations produce the same per-pixel sums.
+#include <hip/hip_runtime.h>
+
+#include "pg3g_internal.hpp"
+#include "pt_kernels.hpp"
+
+namespace pg3 {
+
+constexpr int kEmDynamic = -2;   // light spec read from each scene at run time
+
+// Wave-uniform scene of a global item index: the last scene whose first item is <= item.
+__device__ __forceinline__ uint32_t SceneOfItem(const BatchScene *__restrict__ info, uint32_t numScenes,
+                                                uint32_t item) {
+    uint32_t lo = 0, hi = numScenes;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (info[mid].itemBegin <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int kEmKind, bool kOneLight>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+BatchKernel(const SceneView *__restrict__ views, const BatchScene *__restrict__ info, uint32_t numScenes,
+            uint32_t numItems, uint32_t *__restrict__ work, unsigned long long *__restrict__ counters) {
+    const uint32_t lane = __lane_id();
+    const uint32_t pass = work[1];
+    __shared__ double accLds[4][64 * 3];
+    uint32_t segments = 0, samples = 0;
+    for (;;) {
+        uint32_t item = 0;
+        if (lane == 0) item = atomicAdd(&work[0], 1u);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= numItems) break;   // every wave leaves once the counter passes the last item
+        const uint32_t s = SceneOfItem(info, numScenes, item);
+        const BatchScene bi = info[s];
+        const SceneView &S = views[s];
+        const LightSpec spec = (kEmKind == kEmDynamic) ? SceneLightSpec(S)
+                                                       : LightSpec{kEmKind, kOneLight ? 1u : S.numLights};
+        const uint32_t local = item - bi.itemBegin;
+        const uint32_t tile = local % bi.numTiles, chunk = local / bi.numTiles;
+        const uint32_t base = pass * bi.spp;
+        const uint32_t itBegin = base + chunk * bi.itersPerItem;
+        const uint32_t itEnd = base + min(bi.spp, (chunk + 1u) * bi.itersPerItem);
+        TraceTile(S, spec, (tile % bi.tilesX) * 8u, (tile / bi.tilesX) * 8u, itBegin, itEnd, accLds[threadIdx.x >> 6],
+                  bi.fb, segments, samples);
+    }
+    const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
+    if (lane == 0 && counters) {
+        atomicAdd(counters + 0, (unsigned long long)segSum);
+        atomicAdd(counters + 1, (unsigned long long)smpSum);
+    }
+}
+
+// Pass bookkeeping nodes of the captured graph (plain kernels: no memset nodes in graphs).
+__global__ void BatchBeginPass(uint32_t *work) {
+    if (threadIdx.x == 0) work[0] = 0u;
+}
+__global__ void BatchEndPass(uint32_t *work) {
+    if (threadIdx.x == 0) work[1] += 1u;
+}
+
+using BatchFn = void (*)(const SceneView *, const BatchScene *, uint32_t, uint32_t, uint32_t *, unsigned long long *);
+
+static BatchFn PickBatchKernel(int emKind, int oneLight) {
+    if (oneLight < 0) return BatchKernel<kEmDynamic, false>;
+#define PG3_BATCH_CASE(K)                                                                     \
+    if (emKind == (K)) return oneLight ? BatchKernel<(K), true> : BatchKernel<(K), false>;
+    PG3_BATCH_CASE(kEmAbsent)
+    PG3_BATCH_CASE(kEmCosine)
+    PG3_BATCH_CASE(kEmSimple)
+    PG3_BATCH_CASE(kEmSteerable)
+#undef PG3_BATCH_CASE
+    return BatchKernel<kEmDynamic, false>;
+}
+
+hipError_t LaunchBatchPass(const BatchLaunch &L, hipStream_t stream) {
+    if (L.numItems == 0 || L.numScenes == 0) return hipSuccess;
+    hipLaunchKernelGGL(BatchBeginPass, dim3(1), dim3(64), 0, stream, L.work);
+    hipLaunchKernelGGL(PickBatchKernel(L.emKind, L.oneLight), dim3(L.gridBlocks), dim3(256), 0, stream, L.views,
+                       L.info, L.numScenes, L.numItems, L.work, L.counters);
+    hipLaunchKernelGGL(BatchEndPass, dim3(1), dim3(64), 0, stream, L.work);
+    return hipGetLastError();
+}
+
+}  // namespace pg3

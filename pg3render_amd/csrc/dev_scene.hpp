@@ -124,6 +124,20 @@ PG3_DEV bool SphHit(const DevSphere &s, F3 org, F3 dir, float tmin, Hit &h) {
 // loads into SGPRs (the tables are never written while a kernel runs).
 template <class T> using ConstPtr = const __attribute__((address_space(4))) T *;
 template <class T> PG3_DEV ConstPtr<T> AsConst(const T *p) { return (ConstPtr<T>)(p); }
+// Scene tables reached through pointers stored in memory (the batched kernel reads its scenes
+// from an array) are generic pointers to the compiler; casting them to the global address space
+// gives global loads instead of flat loads.
+template <class T> using GlobalPtr = const __attribute__((address_space(1))) T *;
+template <class T> PG3_DEV GlobalPtr<T> AsGlobal(const T *p) { return (GlobalPtr<T>)(p); }
+template <class T> PG3_DEV T LoadGlobal(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "4-byte records only");
+    const GlobalPtr<uint32_t> src = (GlobalPtr<uint32_t>)(p);
+    T out;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (size_t i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
+    return out;
+}
 typedef float F4v __attribute__((ext_vector_type(4)));
 PG3_DEV DevTri LoadTri(const DevTri *base, uint32_t i) {
     const ConstPtr<F4v> q = (ConstPtr<F4v>)AsConst(base + i);
@@ -190,7 +204,7 @@ PG3_DEV void GenerateRay(const SceneView &S, float x, float y, F3 &org, F3 &dir)
 // ---------------------------------------------------------------------------------------------
 // Environment map image (em_image.hxx)
 PG3_DEV F3 Texel(const DevEm &E, uint32_t x, uint32_t y) {
-    const float4 t = E.texels[(size_t)E.w * y + x];
+    const F4v t = ((GlobalPtr<F4v>)AsGlobal(E.texels))[(size_t)E.w * y + x];
     return mk(t.x, t.y, t.z);
 }
 PG3_DEV void EmNorm(const DevEm &E, int32_t x, int32_t y, uint32_t &ox, uint32_t &oy) {   // :245-278
@@ -230,7 +244,8 @@ PG3_DEV F3 EmEval(const DevEm &E, F3 dir) {
 // Simple spherical sampler (em_simple_spherical_sampler.hxx) over the 2-D CDF
 // (distribution.hxx:391-455, 510-541): upper_bound over the full CDF selects the same segment as
 // the reference's 16-float hierarchical blocks.
-PG3_DEV uint32_t UpperBound(const float *cdf, uint32_t n1, float x) {   // first index with cdf > x
+PG3_DEV uint32_t UpperBound(const float *cdfp, uint32_t n1, float x) {   // first index with cdf > x
+    const GlobalPtr<float> cdf = AsGlobal(cdfp);
     uint32_t lo = 0, len = n1;
     while (len > 0) {
         const uint32_t half = len >> 1;
@@ -239,9 +254,10 @@ PG3_DEV uint32_t UpperBound(const float *cdf, uint32_t n1, float x) {   // first
     }
     return lo;
 }
-PG3_DEV void Sample1D(const float *cdf, uint32_t count, float u, float &x, uint32_t &seg, float &pdf) {
+PG3_DEV void Sample1D(const float *cdfp, uint32_t count, float u, float &x, uint32_t &seg, float &pdf) {
+    const GlobalPtr<float> cdf = AsGlobal(cdfp);
     const float ut = u * 0.999999f;
-    int32_t s = (int32_t)UpperBound(cdf, count + 1, ut) - 1;
+    int32_t s = (int32_t)UpperBound(cdfp, count + 1, ut) - 1;
     s = max(0, min(s, (int32_t)count - 1));
     seg = (uint32_t)s;
     const float c0 = cdf[seg], c1 = cdf[seg + 1];
@@ -269,9 +285,10 @@ PG3_DEV float SimplePdf(const DevEm &E, F3 dir) {                               
     Dir2LatLong(dir, u, v);
     const uint32_t iu = min((uint32_t)fmaxf(0.f, u * E.w), E.w - 1);
     const uint32_t iv = min((uint32_t)fmaxf(0.f, v * E.h), E.h - 1);
-    const float *cc = E.condCdf + (size_t)iv * (E.w + 1);
+    const GlobalPtr<float> cc = AsGlobal(E.condCdf + (size_t)iv * (E.w + 1));
+    const GlobalPtr<float> mc = AsGlobal(E.margCdf);
     const float pc = (cc[iu + 1] - cc[iu]) * E.w;
-    const float pm = (E.margCdf[iv + 1] - E.margCdf[iv]) * E.h;
+    const float pm = (mc[iv + 1] - mc[iv]) * E.h;
     const uint32_t seg = min((uint32_t)(v * E.h), E.h - 1u);
     return (pc * pm) * E.plan2ang / SinMidTheta(E.h, seg);
 }
@@ -295,7 +312,7 @@ PG3_DEV SH9 ClampedCos(F3 n) {                                 // GenerateForCla
     s.c[8] = c1 * (Sqr(n.x) - Sqr(n.y));
     return s;
 }
-PG3_DEV float Dot9(const float *w, const SH9 &c) {            // :135-146 (sequential sum)
+template <class P> PG3_DEV float Dot9(P w, const SH9 &c) {            // :135-146 (sequential sum)
     float r = 0.0f;
 #pragma unroll
     for (int i = 0; i < 9; ++i) r += w[i] * c.c[i];
@@ -323,7 +340,7 @@ PG3_DEV uint32_t PickLeaf(const DevEm &E, const SH9 &c, float &u) {
     for (int32_t lvl = (int32_t)E.numLevels - 1; lvl > 0; --lvl) {
         const uint32_t first = node * 6u;
         const uint32_t cc = min(6u, E.levelCount[lvl - 1] - first);
-        const float *w = E.nodeW + (size_t)(E.levelBase[lvl - 1] + first) * 12u;
+        const GlobalPtr<float> w = AsGlobal(E.nodeW + (size_t)(E.levelBase[lvl - 1] + first) * 12u);
         float cs[7];
         cs[0] = 0.f;
 #pragma unroll
@@ -387,7 +404,7 @@ PG3_DEV bool SteerSampleHemisphere(const DevEm &E, F3 n, const SH9 &c, float u0,
                                    float &pdf, F3 &radCos) {
     float ux = u0;
     const uint32_t leaf = PickLeaf(E, c, ux);
-    const float *lv = E.leafVerts + (size_t)leaf * 36u;
+    const GlobalPtr<float> lv = AsGlobal(E.leafVerts + (size_t)leaf * 36u);
     const float v0 = Dot9(lv + 3, c), v1 = Dot9(lv + 15, c), v2 = Dot9(lv + 27, c);
     float b0, b1;
     SampleTriBilinear(ux, u1, v0, v1, v2, b0, b1);
@@ -516,7 +533,7 @@ PG3_HD LightSpec SceneLightSpec(const SceneView &S) {
 
 // BackgroundLight::GetEmmision(dir), lights.hxx:379-407 (radiance only)
 PG3_DEV F3 BgRadiance(const SceneView &S, const LightSpec &ls, F3 dir) {
-    return (ls.em != kEmAbsent) ? EmEval(S.em, dir) : S.lights[S.bgLight].radiance;
+    return (ls.em != kEmAbsent) ? EmEval(S.em, dir) : LoadGlobal(S.lights + S.bgLight).radiance;
 }
 // ... and its PDF w.r.t. a shading frame/material
 PG3_DEV float BgPdf(const SceneView &S, const LightSpec &ls, F3 dir, const Frame &fr, uint32_t sides) {
@@ -554,7 +571,7 @@ PG3_DEV float EmEstimateMC(const SceneView &S, const Frame &fr, uint32_t sides, 
 // AbstractLight::EstimateContribution
 PG3_DEV float EstimateContribution(const SceneView &S, const LightSpec &spec, uint32_t li, F3 sp, const Frame &fr, uint32_t sides,
                                    const RngKey &key, uint32_t vtx) {
-    const DevLight &L = S.lights[li];
+    const DevLight L = LoadGlobal(S.lights + li);
     if (L.type == kLightArea) {                                                        // :133-167
         const F3 P1 = L.p0 + L.e1, P2 = L.p0 + L.e2;
         const F3 P3 = L.p0 + 0.33f * L.e1 + 0.33f * L.e2;
@@ -580,7 +597,7 @@ PG3_DEV float EstimateContribution(const SceneView &S, const LightSpec &spec, ui
 // AbstractLight::SampleIllumination
 PG3_DEV bool SampleIllumination(const SceneView &S, const LightSpec &spec, uint32_t li, F3 sp, const Frame &fr, uint32_t sides,
                                 float u0, float u1, float u2, LightSample &o) {
-    const DevLight &L = S.lights[li];
+    const DevLight L = LoadGlobal(S.lights + li);
     if (L.type == kLightArea) {                                                        // :114-128
         const F3 pt = SampleUniformTriangle(L.p0, L.p0 + L.e1, L.p0 + L.e2, u0, u1);
         AreaComputeSample(L, sp, pt, fr.z, sides, o);

@@ -3,6 +3,7 @@
 // leave a message for pg3g_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -60,6 +61,7 @@ struct SceneSlot {
     SceneView view{};
     pg3g_render_params params{};
     DevBuf tris, sphs, mats, lights, texels, margCdf, condCdf, rowInt, nodeW, leafVerts, fb;
+    DevBuf viewDev;      // device copy of `view`, refreshed on upload and parameter changes
     uint64_t iterations = 0;
     SteerableTables steer;
     bool hasSteer = false;
@@ -75,6 +77,29 @@ struct pg3g_context {
     int kernelMode = PG3G_KERNEL_WAVEFRONT;
     DevBuf scratch[8];
     WavefrontContext *wavefront = nullptr;
+    uint64_t sceneEpoch = 0;   // bumped by every upload / parameter change (invalidates batches)
+    int numCUs = 0;
+};
+
+struct pg3g_batch {
+    pg3g_context *ctx = nullptr;
+    uint64_t epoch = 0;
+    std::vector<uint32_t> slots, spp;
+    std::vector<BatchScene> info;
+    DevBuf views, infoDev, work;
+    pg3g_batch_config cfg{8, 16, 1};
+    uint32_t numItems = 0;
+    uint32_t passes = 0;       // passes rendered since creation / the last autotune
+    int emKind = 0, oneLight = -1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~pg3g_batch() { DropGraph(); }
+    void DropGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
 };
 
 namespace {
@@ -382,8 +407,118 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
     PG3_HIP(S.fb.Alloc(fbBytes));
     PG3_HIP(hipMemsetAsync(S.fb.p, 0, fbBytes, st));
     S.iterations = 0;
+    PG3_HIP(S.viewDev.Alloc(sizeof(SceneView)));
+    PG3_HIP(hipMemcpyAsync(S.viewDev.p, &S.view, sizeof(SceneView), hipMemcpyHostToDevice, st));
     PG3_HIP(hipStreamSynchronize(st));
     S.valid = true;
+    ++ctx->sceneEpoch;
+    return PG3G_OK;
+}
+
+// ---- batches ---------------------------------------------------------------------------------
+BatchLaunch MakeBatchLaunch(pg3g_batch *b) {
+    BatchLaunch L;
+    L.views = b->views.as<SceneView>();
+    L.info = b->infoDev.as<BatchScene>();
+    L.numScenes = (uint32_t)b->slots.size();
+    L.numItems = b->numItems;
+    L.work = b->work.as<uint32_t>();
+    L.counters = b->ctx->counters.as<unsigned long long>();
+    L.gridBlocks = (uint32_t)std::max(1, b->ctx->numCUs) * (b->cfg.waves_per_cu / 4u);
+    L.emKind = b->emKind;
+    L.oneLight = b->oneLight;
+    return L;
+}
+
+// Work items per scene for the current configuration (host table uploaded to the device).
+int BuildBatchItems(pg3g_batch *b) {
+    pg3g_context *ctx = b->ctx;
+    uint64_t items = 0;
+    for (size_t i = 0; i < b->slots.size(); ++i) {
+        const SceneView &v = ctx->slots[b->slots[i]]->view;
+        BatchScene &bs = b->info[i];
+        bs.fb = ctx->slots[b->slots[i]]->fb.as<double>();
+        bs.tilesX = (v.resX + 7u) / 8u;
+        bs.numTiles = bs.tilesX * ((v.resY + 7u) / 8u);
+        bs.spp = b->spp[i];
+        bs.itersPerItem = std::min(b->cfg.iters_per_item, bs.spp);
+        bs.itemBegin = (uint32_t)items;
+        items += (uint64_t)bs.numTiles * ((bs.spp + bs.itersPerItem - 1u) / bs.itersPerItem);
+        bs.pad = 0;
+        if (items > 0xFFFFFFF0ull) return Fail(PG3G_ERR_INVALID_ARG, "batch has too many work items");
+    }
+    b->numItems = (uint32_t)items;
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(b->infoDev.Alloc(sizeof(BatchScene) * b->info.size()));
+    PG3_HIP(hipMemcpyAsync(b->infoDev.p, b->info.data(), sizeof(BatchScene) * b->info.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    b->DropGraph();
+    return PG3G_OK;
+}
+
+int BatchCheck(pg3g_batch *b) {
+    if (!b) return Fail(PG3G_ERR_INVALID_ARG, "batch is null");
+    if (b->epoch != b->ctx->sceneEpoch)
+        return Fail(PG3G_ERR_NO_SCENE, "the context's scenes changed after the batch was created");
+    return PG3G_OK;
+}
+
+int BatchRenderPasses(pg3g_batch *b, uint32_t passes) {
+    pg3g_context *ctx = b->ctx;
+    PG3_HIP(hipSetDevice(ctx->device));
+    const BatchLaunch L = MakeBatchLaunch(b);
+    if (b->cfg.use_graph && !b->exec) {
+        PG3_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = LaunchBatchPass(L, ctx->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        PG3_HIP(le);
+        PG3_HIP(ce);
+        b->graph = g;
+        PG3_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    for (uint32_t p = 0; p < passes; ++p) {
+        if (b->cfg.use_graph) PG3_HIP(hipGraphLaunch(b->exec, ctx->stream));
+        else PG3_HIP(LaunchBatchPass(L, ctx->stream));
+    }
+    for (size_t i = 0; i < b->slots.size(); ++i) ctx->slots[b->slots[i]]->iterations += (uint64_t)b->spp[i] * passes;
+    b->passes += passes;
+    return PG3G_OK;
+}
+
+int BatchTimed(pg3g_batch *b, uint32_t passes, float *ms) {
+    pg3g_context *ctx = b->ctx;
+    PG3_HIP(hipSetDevice(ctx->device));
+    hipEvent_t e0, e1;
+    PG3_HIP(hipEventCreate(&e0));
+    PG3_HIP(hipEventCreate(&e1));
+    hipError_t e = hipEventRecord(e0, ctx->stream);
+    const int rc = (e == hipSuccess) ? BatchRenderPasses(b, passes) : PG3G_OK;
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != PG3G_OK) return rc;
+    PG3_HIP(e);
+    if (ms) *ms = t;
+    return PG3G_OK;
+}
+
+// Clears the batch's framebuffers and restarts at pass 0.
+int BatchRestart(pg3g_batch *b) {
+    pg3g_context *ctx = b->ctx;
+    PG3_HIP(hipSetDevice(ctx->device));
+    for (size_t i = 0; i < b->slots.size(); ++i) {
+        SceneSlot &S = *ctx->slots[b->slots[i]];
+        PG3_HIP(hipMemsetAsync(S.fb.p, 0, S.fb.bytes, ctx->stream));
+        S.iterations = 0;
+    }
+    PG3_HIP(hipMemsetAsync(b->work.p, 0, sizeof(uint32_t) * 2, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    b->passes = 0;
     return PG3G_OK;
 }
 
@@ -415,6 +550,7 @@ int pg3g_create(pg3g_context **ctx, int device) {
     if (!c) return Fail(PG3G_ERR_OUT_OF_MEMORY, "host allocation failed");
     c->device = device;
     PG3_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    PG3_HIP(hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device));
     PG3_HIP(c->counters.Alloc(sizeof(unsigned long long) * 2));
     PG3_HIP(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned long long) * 2, c->stream));
     PG3_HIP(hipStreamSynchronize(c->stream));
@@ -450,6 +586,10 @@ int pg3g_set_params(pg3g_context *ctx, uint32_t slot, const pg3g_render_params *
     if (int rc = ValidateParams(*p)) return rc;
     S->params = *p;
     ApplyParams(S->view, *p);
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(hipMemcpyAsync(S->viewDev.p, &S->view, sizeof(SceneView), hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    ++ctx->sceneEpoch;
     return PG3G_OK;
 }
 
@@ -470,10 +610,10 @@ int pg3g_render_slot(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
     if (ctx->kernelMode == PG3G_KERNEL_WAVEFRONT) {
         if (!ctx->wavefront) ctx->wavefront = CreateWavefrontContext();
         if (!ctx->wavefront) return Fail(PG3G_ERR_OUT_OF_MEMORY, "host allocation failed");
-        PG3_HIP(LaunchWavefront(S->view, first_iter, n_iter, S->fb.as<double>(),
+        PG3_HIP(LaunchWavefront(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(),
                                 ctx->counters.as<unsigned long long>(), ctx->wavefront, ctx->stream));
     } else {
-        PG3_HIP(LaunchMegakernel(S->view, first_iter, n_iter, S->fb.as<double>(), ctx->counters.as<unsigned long long>(),
+        PG3_HIP(LaunchMegakernel(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(), ctx->counters.as<unsigned long long>(),
                                  ctx->stream));
     }
     S->iterations += n_iter;
@@ -692,6 +832,103 @@ int pg3g_steerable_root_weight(pg3g_context *ctx, float *weight9) {
     if (!S || !S->hasSteer) return Fail(PG3G_ERR_NO_SCENE, "slot 0 has no steerable sampler");
     if (!weight9) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
     for (int j = 0; j < 9; ++j) weight9[j] = S->steer.rootW[j];
+    return PG3G_OK;
+}
+
+int pg3g_batch_create(pg3g_context *ctx, uint32_t num_scenes, const uint32_t *slots, const uint32_t *spp,
+                      pg3g_batch **batch) {
+    if (!ctx || !slots || !spp || !batch) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    *batch = nullptr;
+    if (num_scenes == 0 || num_scenes > PG3G_MAX_SCENES) return Fail(PG3G_ERR_INVALID_ARG, "bad scene count");
+    std::unique_ptr<pg3g_batch> b(new (std::nothrow) pg3g_batch());
+    if (!b) return Fail(PG3G_ERR_OUT_OF_MEMORY, "host allocation failed");
+    b->ctx = ctx;
+    b->epoch = ctx->sceneEpoch;
+    std::vector<SceneView> views;
+    int em = -100, one = -100;
+    for (uint32_t i = 0; i < num_scenes; ++i) {
+        SceneSlot *S = GetSlot(ctx, slots[i]);
+        if (!S) return Fail(PG3G_ERR_NO_SCENE, "batch names an empty slot");
+        if (spp[i] == 0) return Fail(PG3G_ERR_INVALID_ARG, "spp must be >= 1");
+        if (S->view.algorithm != PG3G_ALG_PATH_TRACING)
+            return Fail(PG3G_ERR_UNSUPPORTED, "batched rendering supports the path tracer (pt) only");
+        for (uint32_t j = 0; j < i; ++j)
+            if (slots[j] == slots[i]) return Fail(PG3G_ERR_INVALID_ARG, "a slot appears twice in the batch");
+        const LightSpec ls = SceneLightSpec(S->view);
+        const int o = (ls.numLights == 1) ? 1 : 0;
+        em = (em == -100 || em == ls.em) ? ls.em : -200;
+        one = (one == -100 || one == o) ? o : -200;
+        views.push_back(S->view);
+        b->slots.push_back(slots[i]);
+        b->spp.push_back(spp[i]);
+    }
+    b->emKind = (em == -200 || one == -200) ? 0 : em;
+    b->oneLight = (em == -200 || one == -200) ? -1 : one;
+    b->info.resize(num_scenes);
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(b->views.Alloc(sizeof(SceneView) * num_scenes));
+    PG3_HIP(hipMemcpyAsync(b->views.p, views.data(), sizeof(SceneView) * num_scenes, hipMemcpyHostToDevice,
+                           ctx->stream));
+    PG3_HIP(b->work.Alloc(sizeof(uint32_t) * 2));
+    PG3_HIP(hipMemsetAsync(b->work.p, 0, sizeof(uint32_t) * 2, ctx->stream));
+    if (int rc = BuildBatchItems(b.get())) return rc;
+    *batch = b.release();
+    return PG3G_OK;
+}
+
+int pg3g_batch_destroy(pg3g_batch *batch) {
+    if (!batch) return PG3G_OK;
+    (void)hipSetDevice(batch->ctx->device);
+    (void)hipStreamSynchronize(batch->ctx->stream);
+    delete batch;
+    return PG3G_OK;
+}
+
+int pg3g_batch_set_config(pg3g_batch *batch, const pg3g_batch_config *config) {
+    if (!batch || !config) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    if (config->iters_per_item == 0 || config->waves_per_cu < 4 || config->waves_per_cu % 4 != 0 ||
+        config->waves_per_cu > 64)
+        return Fail(PG3G_ERR_INVALID_ARG, "bad batch configuration");
+    batch->cfg = *config;
+    batch->cfg.use_graph = config->use_graph ? 1u : 0u;
+    return BuildBatchItems(batch);
+}
+
+int pg3g_batch_get_config(const pg3g_batch *batch, pg3g_batch_config *config) {
+    if (!batch || !config) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    *config = batch->cfg;
+    return PG3G_OK;
+}
+
+int pg3g_batch_render(pg3g_batch *batch, uint32_t passes) {
+    if (int rc = BatchCheck(batch)) return rc;
+    return BatchRenderPasses(batch, passes);
+}
+
+int pg3g_batch_render_timed(pg3g_batch *batch, uint32_t passes, float *elapsed_ms) {
+    if (int rc = BatchCheck(batch)) return rc;
+    return BatchTimed(batch, passes, elapsed_ms);
+}
+
+int pg3g_batch_autotune(pg3g_batch *batch, uint32_t max_candidates, float *best_pass_ms) {
+    if (int rc = BatchCheck(batch)) return rc;
+    // Work-item length (iterations of one tile) x persistent wavefronts per CU; the first
+    // candidates are the ones most often best, so a small max_candidates still covers them.
+    static const uint32_t kCand[][2] = {{8, 16}, {4, 16}, {16, 16}, {2, 16}, {32, 16}, {8, 8}, {8, 32}, {4, 32}};
+    const uint32_t n = (max_candidates == 0) ? 8u : std::min<uint32_t>(max_candidates, 8u);
+    const uint32_t useGraph = batch->cfg.use_graph;
+    float best = 3.4e38f;
+    pg3g_batch_config bestCfg = batch->cfg;
+    for (uint32_t c = 0; c < n; ++c) {
+        pg3g_batch_config cfg{kCand[c][0], kCand[c][1], useGraph};
+        if (int rc = pg3g_batch_set_config(batch, &cfg)) return rc;
+        float ms = 0.f;
+        if (int rc = BatchTimed(batch, 1, &ms)) return rc;
+        if (ms < best) { best = ms; bestCfg = cfg; }
+    }
+    if (int rc = pg3g_batch_set_config(batch, &bestCfg)) return rc;
+    if (int rc = BatchRestart(batch)) return rc;
+    if (best_pass_ms) *best_pass_ms = best;
     return PG3G_OK;
 }
 

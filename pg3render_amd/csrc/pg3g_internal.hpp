@@ -12,13 +12,37 @@
 
 namespace pg3 {
 
-hipError_t LaunchMegakernel(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
+// S: host copy of the scene view (launch geometry, generic kernel argument); Sdev: the same view
+// in device memory (read by the specialised path-tracer kernels).
+hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
                             unsigned long long *counters, hipStream_t stream);
 struct WavefrontContext;   // queue storage of the wavefront path tracer (one per pg3g_context)
 WavefrontContext *CreateWavefrontContext();
 void DestroyWavefrontContext(WavefrontContext *w);
-hipError_t LaunchWavefront(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
+hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
                            unsigned long long *counters, WavefrontContext *queues, hipStream_t stream);
+// Batched multi-scene pass (batch.hip).  Scene s owns global work items
+// [itemBegin, itemBegin + numTiles * ceil(spp / itersPerItem)); item = chunk * numTiles + tile.
+struct BatchScene {
+    double *fb;                 // the scene's FP64 framebuffer
+    uint32_t itemBegin;
+    uint32_t tilesX, numTiles;  // 8x8 pixel tiles
+    uint32_t spp;               // iterations per pass
+    uint32_t itersPerItem;      // iterations of one tile per work item
+    uint32_t pad;
+};
+struct BatchLaunch {
+    const SceneView *views;     // device, numScenes
+    const BatchScene *info;     // device, numScenes
+    uint32_t numScenes, numItems;
+    uint32_t *work;             // device: [0] next item, [1] pass index
+    unsigned long long *counters;
+    uint32_t gridBlocks;        // persistent workgroups of 256 lanes
+    int emKind;                 // EM sampler shared by every scene, or ignored when oneLight < 0
+    int oneLight;               // 1 / 0 when shared by every scene, -1 = read per scene
+};
+hipError_t LaunchBatchPass(const BatchLaunch &L, hipStream_t stream);
+
 hipError_t LaunchBsdfEval(const DevMaterial &m, uint32_t n, const float *wil, const float *wol, float *f, float *pdf,
                           float *comp, hipStream_t s);
 hipError_t LaunchBsdfSample(const DevMaterial &m, uint32_t n, const float *wol, const float *u, float *wil, float *f,

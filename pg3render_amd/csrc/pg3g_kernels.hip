@@ -7,14 +7,6 @@
 namespace pg3 {
 
 // ---------------------------------------------------------------------------------------------
-// Wave-level helpers (64-wide wavefronts)
-__device__ __forceinline__ uint32_t WaveSum(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// ---------------------------------------------------------------------------------------------
 // Fused path-tracing kernel: one thread traces `itersPerThread` consecutive iterations of one pixel
 // (PathTracerBase::RunIteration, path_tracer_base.hxx:57-90, over a range of iterations) and adds
 // its FP64 sum to the framebuffer (frame_buffer.hxx:30-46).  Consecutive lanes are consecutive
@@ -73,57 +65,26 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
 #endif
 template <int kEmKind, bool kOneLight>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PG3_PT_WAVES_PER_EU)))
-RenderPTKernel(SceneView S, uint32_t firstIter, uint32_t nIter,
-                                                       uint32_t itersPerThread, uint32_t numChunks,
-                                                       double *__restrict__ fb,
-                                                       unsigned long long *__restrict__ counters) {
+RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nIter, uint32_t itersPerThread,
+               uint32_t numChunks, double *__restrict__ fb, unsigned long long *__restrict__ counters) {
+    // The scene is read from a device copy rather than from kernel arguments: held in SGPRs,
+    // its ~450 bytes of fields spill into VGPR lanes of this register-heavy kernel.
+    const SceneView &S = *Sp;
     const LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
+    __shared__ double accLds[4][64 * 3];
     const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
     const uint32_t numTiles = tilesX * tilesY;
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = (uint32_t)(gid & 63u);
-    const uint64_t waveId = gid >> 6;
-    const uint32_t tile = (uint32_t)(waveId % numTiles);
-    const uint32_t chunk = (uint32_t)(waveId / numTiles);
-    const uint32_t x = (tile % tilesX) * 8u + (lane & 7u);
-    const uint32_t y = (tile / tilesX) * 8u + (lane >> 3);
+    const uint32_t waveId = (uint32_t)(gid >> 6);
+    const uint32_t tile = waveId % numTiles;
+    const uint32_t chunk = waveId / numTiles;
     uint32_t segments = 0, samples = 0;
-    if (chunk < numChunks && x < S.resX && y < S.resY) {
-        const uint32_t pixel = y * S.resX + x;
-        uint32_t it = firstIter + chunk * itersPerThread;
-        const uint32_t itEnd = min(firstIter + nIter, it + itersPerThread);
-        if (it < itEnd) {
-            double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-            RngKey key;
-            PathState ps;
-            {
-                F3 org, dir;
-                CameraSample(S, pixel, it, key, org, dir);
-                InitPath(ps, org, dir);
-            }
-            for (;;) {
-                Hit h;
-                h.dist = 1e36f;
-                const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
-                ShadowReq sr;
-                const bool cont = ShadeVertex(S, spec, key, ps, hit, h, sr, segments);
-                if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
-                if (!cont) {
-                    acc0 += ps.L.x;
-                    acc1 += ps.L.y;
-                    acc2 += ps.L.z;
-                    ++samples;
-                    if (++it >= itEnd) break;
-                    F3 org, dir;
-                    CameraSample(S, pixel, it, key, org, dir);
-                    InitPath(ps, org, dir);
-                }
-            }
-            atomicAdd(fb + (size_t)pixel * 3 + 0, acc0);
-            atomicAdd(fb + (size_t)pixel * 3 + 1, acc1);
-            atomicAdd(fb + (size_t)pixel * 3 + 2, acc2);
-        }
-    }
+    const uint32_t it = firstIter + chunk * itersPerThread;
+    const uint32_t itEnd = min(firstIter + nIter, it + itersPerThread);
+    if (chunk < numChunks && it < itEnd)
+        TraceTile(S, spec, (tile % tilesX) * 8u, (tile / tilesX) * 8u, it, itEnd, accLds[threadIdx.x >> 6], fb, segments,
+                  samples);
     const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
     if (lane == 0 && counters) {
         atomicAdd(counters + 0, (unsigned long long)segSum);
@@ -293,11 +254,10 @@ __global__ void __launch_bounds__(256) CdfMarginalKernel(uint32_t h, const float
 
 // ---------------------------------------------------------------------------------------------
 // Launchers (host)
-using RenderKernelFn = void (*)(SceneView, uint32_t, uint32_t, uint32_t, uint32_t, double *, unsigned long long *);
+using PTKernelFn = void (*)(const SceneView *, uint32_t, uint32_t, uint32_t, uint32_t, double *, unsigned long long *);
 
-// Kernel for a scene: the specialised path tracer for `pt`, the generic kernel otherwise.
-static RenderKernelFn PickRenderKernel(const SceneView &S) {
-    if (S.algorithm != 7) return RenderMegakernel;
+// Specialised path-tracer kernel for a scene (`pt` only).
+static PTKernelFn PickPTKernel(const SceneView &S) {
     const bool one = (S.numLights == 1);
     const int em = SceneLightSpec(S).em;
 #define PG3_PT_CASE(K)                                                                        \
@@ -307,10 +267,10 @@ static RenderKernelFn PickRenderKernel(const SceneView &S) {
     PG3_PT_CASE(kEmSimple)
     PG3_PT_CASE(kEmSteerable)
 #undef PG3_PT_CASE
-    return RenderMegakernel;
+    return nullptr;
 }
 
-hipError_t LaunchMegakernel(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
+hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
                             unsigned long long *counters, hipStream_t stream) {
     if (nIter == 0) return hipSuccess;
     const uint32_t npix = S.resX * S.resY;
@@ -323,8 +283,10 @@ hipError_t LaunchMegakernel(const SceneView &S, uint32_t firstIter, uint32_t nIt
     const uint32_t tiles = ((S.resX + 7u) / 8u) * ((S.resY + 7u) / 8u);
     const uint64_t threads = (uint64_t)tiles * 64u * numChunks;
     const uint32_t blocks = (uint32_t)((threads + 255) / 256);
-    hipLaunchKernelGGL(PickRenderKernel(S), dim3(blocks), dim3(256), 0, stream, S, firstIter, nIter, ipt, numChunks,
-                       fb, counters);
+    const PTKernelFn pt = (S.algorithm == 7) ? PickPTKernel(S) : nullptr;
+    if (pt) hipLaunchKernelGGL(pt, dim3(blocks), dim3(256), 0, stream, Sdev, firstIter, nIter, ipt, numChunks, fb, counters);
+    else hipLaunchKernelGGL(RenderMegakernel, dim3(blocks), dim3(256), 0, stream, S, firstIter, nIter, ipt, numChunks, fb,
+                            counters);
     return hipGetLastError();
 }
 

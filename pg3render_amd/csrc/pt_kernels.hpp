@@ -13,6 +13,13 @@ namespace pg3 {
 
 constexpr uint32_t kMaxPickLights = 8;    // per-vertex light-contribution cache kept in registers
 
+// Sum over the 64 lanes of a wavefront.
+__device__ __forceinline__ uint32_t WaveSum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // MISWeight2Power, path_tracer_base.hxx:522-541 (sample counts 1)
 PG3_DEV float MisPower(float p1, float p2) {
     const float q1 = p1 * p1, q2 = p2 * p2;
@@ -27,10 +34,14 @@ PG3_DEV void FillLightCache(const SceneView &S, const LightSpec &spec, F3 sp, co
                             const RngKey &key, uint32_t vtx, LightCache &lc) {
     lc.sum = 0.f;
 #pragma unroll
-    for (uint32_t i = 0; i < kMaxPickLights; ++i) {
-        float e = 0.f;
-        if (i < spec.numLights) e = EstimateContribution(S, spec, i, sp, fr, sides, key, vtx);
-        lc.est[i] = e;
+    for (uint32_t k = 0; k < kMaxPickLights; ++k) lc.est[k] = 0.f;
+    // one copy of EstimateContribution in the code (a rolled loop); the table stays in registers
+#pragma unroll 1
+    for (uint32_t i = 0; i < spec.numLights; ++i) {
+        const float e = EstimateContribution(S, spec, i, sp, fr, sides, key, vtx);
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxPickLights; ++k)
+            if (k == i) lc.est[k] = e;
         lc.sum += e;
     }
 }
@@ -82,7 +93,7 @@ struct PathState {
     F3 prevN;            // shading normal of the previous vertex (EM pdf w.r.t. its frame)
     uint32_t len;        // path length of the next vertex (1 = camera ray hit)
     uint32_t flags;      // kHasPrev | kPrevFinite | kComputeRefl | previous sides << 4
-    LightCache lc;       // light-contribution cache of the previous vertex (> 1 light)
+    float lcSum;         // sum of the previous vertex's light-contribution cache (> 1 light)
 };
 enum : uint32_t { kHasPrev = 1u, kPrevFinite = 2u, kComputeRefl = 4u };
 
@@ -97,9 +108,20 @@ PG3_DEV void InitPath(PathState &ps, F3 org, F3 dir) {
     ps.prevN = mk(0.f, 0.f, 1.f);
     ps.len = 1;
     ps.flags = kComputeRefl | kPrevFinite;
-    ps.lc.sum = 0.f;
-#pragma unroll
-    for (uint32_t i = 0; i < kMaxPickLights; ++i) ps.lc.est[i] = 0.f;
+    ps.lcSum = 0.f;
+}
+
+// LightPickingProbability of light `id` at the previous vertex of the path (path_tracer_base.hxx:
+// 260-313).  Only the cache's sum travels with the path; the one entry needed is recomputed from
+// the previous vertex (position ps.org, normal ps.prevN, sides, RNG vertex ps.len - 1), which
+// gives the cached value bit for bit (same function, same arguments).
+PG3_DEV float PrevPickProb(const SceneView &S, const LightSpec &spec, const RngKey &key, const PathState &ps,
+                           uint32_t id) {
+    if (spec.numLights == 0) return 0.f;
+    if (spec.numLights == 1) return 1.f;
+    if (!(ps.lcSum > 0.f)) return 1.f / spec.numLights;
+    const float e = EstimateContribution(S, spec, id, ps.org, FrameFromZ(ps.prevN), ps.flags >> 4, key, ps.len - 1u);
+    return e / ps.lcSum;
 }
 
 // Next-event request produced at a vertex: the contribution is added when the segment from `p`
@@ -126,7 +148,7 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey
             } else if (!IsZero(Le)) {
                 if (prevFinite) {
                     const float lp = BgPdf(S, spec, ps.dir, FrameFromZ(ps.prevN), ps.flags >> 4) *
-                                     PickProb(spec, ps.lc, (uint32_t)S.bgLight);
+                                     PrevPickProb(S, spec, key, ps, (uint32_t)S.bgLight);
                     ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
                 } else {
                     ps.L += ps.eThr * Le;
@@ -138,16 +160,16 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey
     const F3 sp = ps.org + ps.dir * h.dist;
     const Frame fr = FrameFromZ(h.n);
     const F3 wol = ToLocal(fr, -ps.dir);
-    const DevMaterial mat = S.mats[h.mat];
+    const DevMaterial mat = LoadGlobal(S.mats + h.mat);
     if (mat.lightId >= 0 && len >= S.minPL) {                                         // :198-210
-        const DevLight &Lt = S.lights[mat.lightId];
+        const DevLight Lt = LoadGlobal(S.lights + mat.lightId);
         F3 Le = mk(0.f);
         if (Lt.type == kLightArea && wol.z > 0.f) Le = Lt.radiance;
         if (!hasPrev) {
             ps.L += Le;
         } else if (!IsZero(Le)) {                                                     // :327-369
             if (prevFinite) {
-                const float lp = AreaEmissionPdf(Lt, sp, ps.org) * PickProb(spec, ps.lc, (uint32_t)mat.lightId);
+                const float lp = AreaEmissionPdf(Lt, sp, ps.org) * PrevPickProb(S, spec, key, ps, (uint32_t)mat.lightId);
                 ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
             } else {
                 ps.L += ps.eThr * Le;
@@ -163,10 +185,12 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey
     VertexRng vr;
     RngVertex(key, len, vr);
     const uint32_t sides = mat.sides;
-    if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, sides, key, len, ps.lc);
+    LightCache lc;
+    lc.sum = 0.f;
+    if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, sides, key, len, lc);
     if ((len + 1) >= S.minPL) {                                                       // :252-265
         float prob = 1.f;
-        const int32_t li = PickLight(spec, ps.lc, vr.u[kDimLightPick], prob);
+        const int32_t li = PickLight(spec, lc, vr.u[kDimLightPick], prob);
         LightSample ls;
         if (li >= 0 && SampleIllumination(S, spec, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
                                           vr.u[kDimLight2], ls)) {
@@ -209,6 +233,7 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey
     }
     ps.flags = kHasPrev | (finite ? kPrevFinite : 0u) | (cut ? 0u : kComputeRefl) | (sides << 4);
     ps.prevN = fr.z;
+    ps.lcSum = lc.sum;
     ps.org = sp;
     ps.dir = ToWorld(fr, r.wil);
     ps.tmin = EpsRayCos(cosAbs);
@@ -248,9 +273,9 @@ PG3_DEV F3 RadiancePTNaive(const SceneView &S, const RngKey &key, F3 org, F3 dir
         const F3 sp = org + dir * h.dist;
         const Frame fr = FrameFromZ(h.n);
         const F3 wol = ToLocal(fr, -dir);
-        const DevMaterial mat = S.mats[h.mat];
+        const DevMaterial mat = LoadGlobal(S.mats + h.mat);
         if (mat.lightId >= 0 && len >= S.minPL) {
-            const DevLight &Lt = S.lights[mat.lightId];
+            const DevLight Lt = LoadGlobal(S.lights + mat.lightId);
             if (Lt.type == kLightArea && wol.z > 0.f) out += Lt.radiance * thr;
         }
         if (ReflectanceZero(mat)) return out;
@@ -285,7 +310,7 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
     const F3 sp = org + dir * h.dist;
     const Frame fr = FrameFromZ(h.n);
     const F3 wol = ToLocal(fr, -dir);
-    const DevMaterial mat = S.mats[h.mat];
+    const DevMaterial mat = LoadGlobal(S.mats + h.mat);
     const uint32_t sides = mat.sides;
     const uint32_t vtx = 1;
     VertexRng vr;
@@ -340,9 +365,9 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
             float lpdf = 0.f;
             int32_t lid = -1;
             if (Intersect(S, sp, wig, EpsRayCos(r.wil.z), h2)) {
-                const int32_t l2 = S.mats[h2.mat].lightId;
+                const int32_t l2 = LoadGlobal(S.mats + h2.mat).lightId;
                 if (l2 >= 0) {
-                    const DevLight &Lt = S.lights[l2];
+                    const DevLight Lt = LoadGlobal(S.lights + l2);
                     const F3 lp = sp + wig * h2.dist;
                     const F3 lwol = ToLocal(FrameFromZ(h2.n), -wig);
                     if (Lt.type == kLightArea) {
@@ -372,7 +397,7 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
     }
     F3 Le = mk(0.f);
     if (mat.lightId >= 0) {
-        const DevLight &Lt = S.lights[mat.lightId];
+        const DevLight Lt = LoadGlobal(S.lights + mat.lightId);
         if (Lt.type == kLightArea && wol.z > 0.f) Le = Lt.radiance;
     }
     return Le + Lo;
@@ -385,6 +410,77 @@ PG3_DEV void CameraSample(const SceneView &S, uint32_t pixel, uint32_t iteration
     const float jx = RngGet(key, 0, 0, 0), jy = RngGet(key, 0, 0, 1);   // path_tracer_base.hxx:74-77
     const uint32_t x = pixel % S.resX, y = pixel / S.resX;
     GenerateRay(S, float(x) + jx, float(y) + jy, org, dir);
+}
+
+// One work item of PathTracer -sb 1: the 8x8 pixel tile with corner (x0, y0) x iterations
+// [itBegin, itEnd), traced by one wavefront.  The item's samples q = 0 .. 64*(itEnd-itBegin)-1
+// (tile pixel q % 64, iteration itBegin + q / 64) are handed out in order, one to each lane whose
+// path has ended (path regeneration across pixels and iterations), so every lane stays busy
+// until the item's last samples are in flight.  Per-pixel FP64 sums are gathered in the
+// wavefront's LDS slice `acc` (64 pixels x 3) and added to the framebuffer once per pixel.
+// Which lane traces which sample changes nothing: a sample's path depends only on its
+// (pixel, iteration) RNG stream.
+PG3_DEV void TraceTile(const SceneView &S, const LightSpec &spec, uint32_t x0, uint32_t y0, uint32_t itBegin,
+                       uint32_t itEnd, double *acc, double *fb, uint32_t &segments, uint32_t &samples) {
+    const uint32_t lane = __lane_id();
+    acc[lane * 3 + 0] = 0.0;
+    acc[lane * 3 + 1] = 0.0;
+    acc[lane * 3 + 2] = 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t total = 64u * (itEnd - itBegin);
+    uint32_t next = 0;       // first sample not handed out yet (wave-uniform)
+    bool busy = false;
+    uint32_t pix = 0;        // tile pixel of this lane's sample
+    RngKey key{};
+    PathState ps;
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        if (next < total && idle != 0) {
+            const uint32_t q = next + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+            if (!busy && q < total) {
+                pix = q & 63u;
+                const uint32_t x = x0 + (pix & 7u), y = y0 + (pix >> 3);
+                if (x < S.resX && y < S.resY) {
+                    F3 org, dir;
+                    CameraSample(S, y * S.resX + x, itBegin + (q >> 6), key, org, dir);
+                    InitPath(ps, org, dir);
+                    busy = true;
+                }
+            }
+            next += (uint32_t)__popcll(idle);
+        }
+        if (__ballot(busy) == 0) {
+            if (next >= total) break;
+            continue;
+        }
+        if (busy) {
+            Hit h;
+            h.dist = 1e36f;
+            const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
+            ShadowReq sr;
+            const bool cont = ShadeVertex(S, spec, key, ps, hit, h, sr, segments);
+            if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
+            if (!cont) {
+                atomicAdd(acc + pix * 3 + 0, (double)ps.L.x);
+                atomicAdd(acc + pix * 3 + 1, (double)ps.L.y);
+                atomicAdd(acc + pix * 3 + 2, (double)ps.L.z);
+                ++samples;
+                busy = false;
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t x = x0 + (lane & 7u), y = y0 + (lane >> 3);
+    if (x < S.resX && y < S.resY) {
+        double *p = fb + (size_t)(y * S.resX + x) * 3;
+        atomicAdd(p + 0, acc[lane * 3 + 0]);
+        atomicAdd(p + 1, acc[lane * 3 + 1]);
+        atomicAdd(p + 2, acc[lane * 3 + 2]);
+    }
 }
 
 PG3_DEV F3 RadianceSample(const SceneView &S, uint32_t pixel, uint32_t iteration, uint32_t &segments) {

@@ -28,7 +28,6 @@ struct WfQueues {
     float4 *pD;    // eThr.xyz, iteration (the path length is the bounce index)
     float4 *pE;    // L.xyz, pixel
     float4 *pF;    // prevN.xyz, light-cache sum
-    float *est;    // light-contribution cache, kMaxPickLights x capacity (scenes with > 1 light)
     // hit records
     float4 *hA;    // n.xyz, dist (dist < 0: miss)
     int32_t *hMat;
@@ -58,21 +57,16 @@ __device__ __forceinline__ uint32_t WaveAppend(bool pred, uint32_t *counter) {
 }
 
 __device__ __forceinline__ void StorePath(const WfQueues &Q, uint32_t i, const PathState &ps, uint32_t pixel,
-                                          uint32_t iteration, uint32_t numLights) {
+                                          uint32_t iteration) {
     Q.pA[i] = make_float4(ps.org.x, ps.org.y, ps.org.z, ps.tmin);
     Q.pB[i] = make_float4(ps.dir.x, ps.dir.y, ps.dir.z, ps.prevBsdfPdf);
     Q.pC[i] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, AsF(ps.flags));
     Q.pD[i] = make_float4(ps.eThr.x, ps.eThr.y, ps.eThr.z, AsF(iteration));
     Q.pE[i] = make_float4(ps.L.x, ps.L.y, ps.L.z, AsF(pixel));
-    Q.pF[i] = make_float4(ps.prevN.x, ps.prevN.y, ps.prevN.z, ps.lc.sum);
-    if (numLights > 1) {
-#pragma unroll
-        for (uint32_t k = 0; k < kMaxPickLights; ++k)
-            if (k < numLights) Q.est[(size_t)k * Q.capacity + i] = ps.lc.est[k];
-    }
+    Q.pF[i] = make_float4(ps.prevN.x, ps.prevN.y, ps.prevN.z, ps.lcSum);
 }
 __device__ __forceinline__ void LoadPath(const WfQueues &Q, uint32_t i, PathState &ps, uint32_t &pixel,
-                                         uint32_t &iteration, uint32_t numLights) {
+                                         uint32_t &iteration) {
     const float4 a = Q.pA[i], b = Q.pB[i], c = Q.pC[i], d = Q.pD[i], e = Q.pE[i], f = Q.pF[i];
     ps.org = mk(a.x, a.y, a.z);
     ps.tmin = a.w;
@@ -85,10 +79,7 @@ __device__ __forceinline__ void LoadPath(const WfQueues &Q, uint32_t i, PathStat
     ps.L = mk(e.x, e.y, e.z);
     pixel = AsU(e.w);
     ps.prevN = mk(f.x, f.y, f.z);
-    ps.lc.sum = f.w;
-#pragma unroll
-    for (uint32_t k = 0; k < kMaxPickLights; ++k)
-        ps.lc.est[k] = (numLights > 1 && k < numLights) ? Q.est[(size_t)k * Q.capacity + i] : 0.f;
+    ps.lcSum = f.w;
 }
 
 // Sample s of the batch -> (pixel, iteration): iterations are the slow index, and pixels of one
@@ -122,7 +113,7 @@ __global__ void __launch_bounds__(256) WfGenerate(SceneView S, WfQueues Q, uint6
         } else {
             InitPath(ps, mk(0.f), mk(0.f, 0.f, 1.f));
         }
-        StorePath(Q, i, ps, valid ? pixel : 0xFFFFFFFFu, it, S.numLights);
+        StorePath(Q, i, ps, valid ? pixel : 0xFFFFFFFFu, it);
         const uint32_t slot = WaveAppend(valid, &Q.counts[0]);
         if (valid) Q.rayQ[0][slot] = i;
         n += valid;
@@ -164,7 +155,7 @@ WfShade(SceneView S, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce) {
             i = q[j];
             PathState ps;
             uint32_t pixel, it;
-            LoadPath(Q, i, ps, pixel, it, spec.numLights);
+            LoadPath(Q, i, ps, pixel, it);
             ps.len = bounce;
             const float4 ha = Q.hA[i];
             Hit h;
@@ -174,7 +165,7 @@ WfShade(SceneView S, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce) {
             RngKey key{S.k0, S.k1, pixel, it, S.sceneId << 16};
             cont = ShadeVertex(S, spec, key, ps, h.mat >= 0, h, sr, segments);
             shadow = sr.valid;
-            StorePath(Q, i, ps, pixel, it, spec.numLights);
+            StorePath(Q, i, ps, pixel, it);
         }
         // queue appends are wave-uniform calls (inactive tail lanes pass false)
         const uint32_t rs = WaveAppend(cont, &Q.counts[qo]);
@@ -255,7 +246,7 @@ struct WfStorage {
         if (mem) (void)hipFree(mem);
         mem = nullptr;
         const size_t f4 = sizeof(float4) * (size_t)cap;
-        const size_t bytes = 6 * f4 + sizeof(float) * kMaxPickLights * (size_t)cap + f4 + sizeof(int32_t) * (size_t)cap +
+        const size_t bytes = 6 * f4 + f4 + sizeof(int32_t) * (size_t)cap +
                              2 * sizeof(uint32_t) * (size_t)cap + 3 * f4 + 64 + 17 * 256;  // + alignment
         hipError_t e = hipMalloc(&mem, bytes);
         if (e != hipSuccess) return e;
@@ -265,7 +256,6 @@ struct WfStorage {
         q.capacity = cap;
         q.pA = (float4 *)take(f4); q.pB = (float4 *)take(f4); q.pC = (float4 *)take(f4);
         q.pD = (float4 *)take(f4); q.pE = (float4 *)take(f4); q.pF = (float4 *)take(f4);
-        q.est = (float *)take(sizeof(float) * kMaxPickLights * (size_t)cap);
         q.hA = (float4 *)take(f4);
         q.hMat = (int32_t *)take(sizeof(int32_t) * (size_t)cap);
         q.rayQ[0] = (uint32_t *)take(sizeof(uint32_t) * (size_t)cap);
@@ -301,10 +291,10 @@ struct WavefrontContext {
 WavefrontContext *CreateWavefrontContext() { return new (std::nothrow) WavefrontContext(); }
 void DestroyWavefrontContext(WavefrontContext *w) { delete w; }
 
-hipError_t LaunchWavefront(const SceneView &S, uint32_t firstIter, uint32_t nIter, double *fb,
+hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
                            unsigned long long *counters, WavefrontContext *W, hipStream_t stream) {
     if (!W) return hipErrorInvalidValue;
-    if (S.algorithm != 7 || S.maxPL == 0) return LaunchMegakernel(S, firstIter, nIter, fb, counters, stream);
+    if (S.algorithm != 7 || S.maxPL == 0) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, stream);
     if (nIter == 0) return hipSuccess;
     if (W->gridBlocks == 0) {
         int dev = 0, cus = 256;
@@ -320,7 +310,7 @@ hipError_t LaunchWavefront(const SceneView &S, uint32_t firstIter, uint32_t nIte
     if (e != hipSuccess) return e;
     const WfQueues &Q = W->st.q;
     const ShadeFn shade = PickShadeKernel(S);
-    if (!shade) return LaunchMegakernel(S, firstIter, nIter, fb, counters, stream);
+    if (!shade) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, stream);
     const dim3 grid(W->gridBlocks), block(256);
     for (uint64_t base = 0; base < total; base += cap) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(cap, total - base);

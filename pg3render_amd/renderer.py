@@ -238,3 +238,55 @@ class GpuPathTracer:
 
     def was_used(self):
         return self.iterations > 0
+
+
+class Batch:
+    """Many uploaded slots rendered by one persistent launch per pass (pg3g_batch_*).
+
+    Pass p renders iterations [p*spp[i], (p+1)*spp[i]) of scene i into its slot's framebuffer,
+    sample for sample what `Context.render(p*spp[i], spp[i], slot)` renders.
+    """
+
+    def __init__(self, ctx, slots, spp):
+        self._lib = _ffi.load()
+        self.ctx = ctx
+        self.slots = [int(s) for s in slots]
+        self.spp = [int(n) for n in spp]
+        n = len(self.slots)
+        sl = (C.c_uint32 * n)(*self.slots)
+        sp = (C.c_uint32 * n)(*self.spp)
+        self.ptr = C.c_void_p()
+        check(self._lib.pg3g_batch_create(ctx.ptr, n, sl, sp, C.byref(self.ptr)), "pg3g_batch_create")
+
+    def close(self):
+        if self.ptr:
+            self._lib.pg3g_batch_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self):
+        c = _ffi.BatchConfig()
+        check(self._lib.pg3g_batch_get_config(self.ptr, C.byref(c)), "pg3g_batch_get_config")
+        return {"iters_per_item": c.iters_per_item, "waves_per_cu": c.waves_per_cu, "use_graph": c.use_graph}
+
+    def set_config(self, iters_per_item=8, waves_per_cu=16, use_graph=True):
+        c = _ffi.BatchConfig(iters_per_item, waves_per_cu, 1 if use_graph else 0)
+        check(self._lib.pg3g_batch_set_config(self.ptr, C.byref(c)), "pg3g_batch_set_config")
+
+    def render(self, passes=1):
+        check(self._lib.pg3g_batch_render(self.ptr, passes), "pg3g_batch_render")
+
+    def render_timed(self, passes=1):
+        ms = C.c_float()
+        check(self._lib.pg3g_batch_render_timed(self.ptr, passes, C.byref(ms)), "pg3g_batch_render_timed")
+        return float(ms.value)
+
+    def autotune(self, max_candidates=0):
+        ms = C.c_float()
+        check(self._lib.pg3g_batch_autotune(self.ptr, max_candidates, C.byref(ms)), "pg3g_batch_autotune")
+        return float(ms.value)

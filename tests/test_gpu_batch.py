@@ -1,0 +1,143 @@
+"""Batched multi-scene renderer (pg3g_batch_*) against per-scene renders of the same iterations.
+
+A batched pass traces every (pixel, iteration) sample exactly as pg3g_render_slot does, so the
+per-pixel FP64 sums agree up to the order of the FP64 additions (work items group iterations
+differently from the per-scene kernel's iteration chunks) and, rarely, one-ulp differences of
+the separately compiled kernels that send a path another way.
+"""
+import numpy as np
+import pytest
+
+from pg3render_amd import Batch, Context, SceneBuild, _ffi
+from pg3render_amd._ffi import Pg3gError
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def c3_scenes(n, res_lo=24, res_hi=72):
+    """Small versions of the 64-scene batch workload (same materials, EMs, spp choices)."""
+    rng = np.random.default_rng(5)
+    out = []
+    for v in range(n):
+        sb = SceneBuild.workload(3, seed=2024, variant=v, res=int(rng.integers(res_lo, res_hi)))
+        out.append(sb)
+    return out
+
+
+def plain_images(ctx, scenes, base_slot, passes):
+    imgs = []
+    for i, sb in enumerate(scenes):
+        ctx.upload(sb, base_slot + i)
+        ctx.render(0, sb.spp * passes, base_slot + i)
+        imgs.append(ctx.framebuffer(base_slot + i, sb.resolution))
+    return imgs
+
+
+def assert_same(a, b, what):
+    (ia, na), (ib, nb) = a, b
+    assert na == nb, what
+    rel = np.abs(ia - ib) / np.maximum(np.abs(ia), 1e-6)
+    assert np.mean(rel > 1e-6) < 2e-3, (what, float(np.mean(rel > 1e-6)))
+    assert abs(ia.sum() - ib.sum()) / max(abs(ia.sum()), 1e-12) < 1e-5, what
+
+
+def test_batch_equals_per_scene_renders(ctx):
+    scenes = c3_scenes(6)
+    n = len(scenes)
+    for i, sb in enumerate(scenes):
+        ctx.upload(sb, i)
+    batch = Batch(ctx, range(n), [sb.spp for sb in scenes])
+    batch.set_config(iters_per_item=4, waves_per_cu=16, use_graph=True)
+    batch.render(2)
+    got = [ctx.framebuffer(i, sb.resolution) for i, sb in enumerate(scenes)]
+    batch.close()
+    want = plain_images(ctx, scenes, 100, passes=2)
+    for i in range(n):
+        assert_same(got[i], want[i], f"scene {i}")
+
+
+def test_batch_graph_and_direct_launch_agree(ctx):
+    scenes = c3_scenes(3)
+    for i, sb in enumerate(scenes):
+        ctx.upload(sb, i)
+    out = []
+    for graph in (True, False):
+        for i in range(len(scenes)):
+            ctx.clear(i)
+        b = Batch(ctx, range(len(scenes)), [sb.spp for sb in scenes])
+        b.set_config(iters_per_item=8, waves_per_cu=8, use_graph=graph)
+        b.render(1)
+        out.append([ctx.framebuffer(i, sb.resolution) for i, sb in enumerate(scenes)])
+        b.close()
+    for i in range(len(scenes)):
+        assert_same(out[0][i], out[1][i], f"scene {i}")
+
+
+def test_autotune_restarts_and_keeps_results(ctx):
+    scenes = c3_scenes(4)
+    for i, sb in enumerate(scenes):
+        ctx.upload(sb, i)
+    b = Batch(ctx, range(len(scenes)), [sb.spp for sb in scenes])
+    best = b.autotune(max_candidates=3)
+    assert best > 0
+    cfg = b.config()
+    assert cfg["iters_per_item"] in (2, 4, 8, 16, 32) and cfg["waves_per_cu"] in (8, 16, 32)
+    for i, sb in enumerate(scenes):
+        img, it = ctx.framebuffer(i, sb.resolution)
+        assert it == 0 and not img.any()
+    b.render(1)
+    got = [ctx.framebuffer(i, sb.resolution) for i, sb in enumerate(scenes)]
+    b.close()
+    want = plain_images(ctx, scenes, 100, passes=1)
+    for i in range(len(scenes)):
+        assert_same(got[i], want[i], f"scene {i}")
+
+
+def test_mixed_light_setups_use_the_generic_kernel(ctx):
+    """Scenes with different light setups (area lights only / EM + area lights / EM only)."""
+    scenes = [SceneBuild.workload(1, seed=3, res=40), SceneBuild.workload(3, seed=2024, variant=7, res=36),
+              SceneBuild.workload(2, seed=4, res=32)]
+    spp = [3, 5, 2]
+    for i, sb in enumerate(scenes):
+        ctx.upload(sb, i)
+    b = Batch(ctx, range(3), spp)
+    b.set_config(iters_per_item=2, waves_per_cu=16)
+    b.render(1)
+    got = [ctx.framebuffer(i, sb.resolution) for i, sb in enumerate(scenes)]
+    b.close()
+    for i, sb in enumerate(scenes):
+        ctx.upload(sb, 100 + i)
+        ctx.render(0, spp[i], 100 + i)
+        assert_same(got[i], ctx.framebuffer(100 + i, sb.resolution), f"scene {i}")
+
+
+def test_batch_is_invalidated_by_uploads_and_rejects_bad_input(ctx):
+    sb = SceneBuild.workload(3, seed=2024, variant=1, res=24)
+    ctx.upload(sb, 0)
+    with pytest.raises(Pg3gError):
+        Batch(ctx, [0, 0], [4, 4])            # duplicate slot
+    with pytest.raises(Pg3gError):
+        Batch(ctx, [0], [0])                  # zero spp
+    with pytest.raises(Pg3gError):
+        Batch(ctx, [250], [4])                # empty slot
+    b = Batch(ctx, [0], [4])
+    with pytest.raises(Pg3gError):
+        b.set_config(iters_per_item=0)
+    ctx.upload(sb, 1)
+    with pytest.raises(Pg3gError) as e:
+        b.render(1)
+    assert e.value.code == _ffi.PG3G_ERR_NO_SCENE
+    b.close()
+    ptn = SceneBuild.workload(1, seed=3, res=16)
+    ptn.set_params(algorithm="ptn")
+    ctx.upload(ptn, 2)
+    with pytest.raises(Pg3gError) as e:
+        Batch(ctx, [2], [1])
+    assert e.value.code == _ffi.PG3G_ERR_UNSUPPORTED
