@@ -249,6 +249,31 @@ int pg3g_batch_render_timed(pg3g_batch *batch, uint32_t passes, float *elapsed_m
  * clears the batch's framebuffers and restarts at pass 0.  Returns the best pass time. */
 int pg3g_batch_autotune(pg3g_batch *batch, uint32_t max_candidates, float *best_pass_ms);
 
+/* ---- differentiable rendering ------------------------------------------------------------ */
+/* No counterpart in the reference.  Path-replay backpropagation of an image loss for the path
+ * tracer (`pt`, -maxpl > 0) with frozen sampling: directions, pdfs and MIS weights come from the
+ * sampling parameters, integrand values (BSDF values, EM radiance) from the evaluation
+ * parameters; the two are equal after a refresh.  Differentiated: Lambert albedo (3 scalars),
+ * GGX conductor tint + alpha (4 scalars, in that order), and the EM texels (FP64 gradients). */
+/* Selects the differentiated materials (their order fixes the scalar layout). */
+int pg3g_diff_setup(pg3g_context *ctx, uint32_t slot, uint32_t num_materials,
+                    const uint32_t *material_ids, uint32_t *num_scalars);
+/* Current evaluation parameters: scalars (num_scalars) and EM texels (w*h*3), either may be NULL. */
+int pg3g_diff_get_params(pg3g_context *ctx, uint32_t slot, float *scalars, float *em_texels);
+/* Sets evaluation parameters (NULL keeps them); refresh_sampling != 0 also makes them the sampling
+ * parameters (materials and EM sampling tables rebuilt). */
+int pg3g_diff_set_params(pg3g_context *ctx, uint32_t slot, const float *scalars, const float *em_texels,
+                         int refresh_sampling);
+/* Differentiable forward render of iterations [first_iter, first_iter + n_iter) into the slot's
+ * framebuffer (like pg3g_render_slot).  elapsed_ms != NULL: waits and returns the GPU time. */
+int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
+                     float *elapsed_ms);
+/* Adjoint of the same iterations: dloss_dimage (res_x*res_y*3) is dLoss/dImage with image =
+ * sum / n_iter.  Writes grad_scalars (num_scalars) and grad_em (w*h*3, FP64 accumulated on the
+ * GPU); either may be NULL.  Waits; elapsed_ms (may be NULL) receives the GPU time. */
+int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
+                 const float *dloss_dimage, double *grad_scalars, double *grad_em, float *elapsed_ms);
+
 #ifdef __cplusplus
 }
 #endif

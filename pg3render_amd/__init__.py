@@ -5,6 +5,7 @@ only binds it.  There is no CPU fallback: if the library is not built, importing
 raises.
 """
 from . import _ffi
+from .diff import DiffScene, render_function
 from .renderer import Batch, Context, GpuPathTracer, SceneBuild
 
-__all__ = ["Batch", "Context", "GpuPathTracer", "SceneBuild", "_ffi"]
+__all__ = ["Batch", "Context", "DiffScene", "GpuPathTracer", "SceneBuild", "_ffi", "render_function"]

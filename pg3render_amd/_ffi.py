@@ -114,6 +114,11 @@ _SIGNATURES = {
     "pg3g_batch_render": (C.c_int, [_P, C.c_uint32]),
     "pg3g_batch_render_timed": (C.c_int, [_P, C.c_uint32, _FP]),
     "pg3g_batch_autotune": (C.c_int, [_P, C.c_uint32, _FP]),
+    "pg3g_diff_setup": (C.c_int, [_P, C.c_uint32, C.c_uint32, _U32P, _U32P]),
+    "pg3g_diff_get_params": (C.c_int, [_P, C.c_uint32, _FP, _FP]),
+    "pg3g_diff_set_params": (C.c_int, [_P, C.c_uint32, _FP, _FP, C.c_int]),
+    "pg3g_render_diff": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _FP]),
+    "pg3g_adjoint": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _FP, _DP, _DP, _FP]),
     "pg3g_build_workload_scene": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "pg3g_build_reference_scene": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "pg3g_scene_build_desc": (C.POINTER(SceneDesc), [_P]),

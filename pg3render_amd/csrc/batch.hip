@@ -12,8 +12,6 @@
 
 namespace pg3 {
 
-constexpr int kEmDynamic = -2;   // light spec read from each scene at run time
-
 // Wave-uniform scene of a global item index: the last scene whose first item is <= item.
 __device__ __forceinline__ uint32_t SceneOfItem(const BatchScene *__restrict__ info, uint32_t numScenes,
                                                 uint32_t item) {

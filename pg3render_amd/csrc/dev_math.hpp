@@ -346,6 +346,21 @@ PG3_HD F3 GgxCondEval(const DevMaterial &m, F3 wil, F3 wol, F3 h, float D, float
     const float G = SmithG1(wil, h, m.alpha) * SmithG1(wol, h, m.alpha);
     return m.glossy * ((F * G * D) / (4.0f * wil.z * wol.z));
 }
+// d log f / d alpha of the GGX conductor value at (wil, wol) (f = tint F G1(i) G1(o) D / (4 ci co),
+// alpha in D and both G1): d log D = 2/a - 4a / (a^2 + tan^2(h)), d log G1(v) = -a tan^2(v) /
+// (R (1 + R)) with R = sqrt(1 + a^2 tan^2(v)).  Zero where the value is zero.
+PG3_HD float GgxCondDLogDAlpha(const DevMaterial &m, F3 wil, F3 wol) {
+    if (wil.z <= 0.f || wol.z <= 0.f) return 0.f;
+    const F3 h = HalfwayReflection(wil, wol);
+    if (h.z <= 0.f || (wil.z * Dot(wil, h)) < 0.f || (wol.z * Dot(wol, h)) < 0.f) return 0.f;
+    const float a = m.alpha, a2 = a * a;
+    float d = 2.f / a - 4.f * a / (a2 + TanThetaSqr(h));
+    const float ti = TanThetaSqr(wil), to = TanThetaSqr(wol);
+    const float ri = sqrtf(1.f + a2 * ti), ro = sqrtf(1.f + a2 * to);
+    d -= a * ti / (ri * (1.f + ri));
+    d -= a * to / (ro * (1.f + ro));
+    return d;
+}
 struct DielCtx {
     F3 wolS, wilS, h;
     bool below, refl;

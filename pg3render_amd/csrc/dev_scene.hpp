@@ -67,6 +67,14 @@ struct SceneView {
     uint32_t minPL, maxPL;
     float iic;
     uint32_t k0, k1, sceneId;
+    // Differentiable rendering with frozen sampling: integrand values (BSDF values of Lambert and
+    // GGX conductor materials, EM radiance) come from evalMats / evalTexels, while directions,
+    // pdfs and MIS weights come from mats and the EM sampling tables.  Both pairs are equal unless
+    // parameters were changed without refreshing the sampling tables.  gradBase[m] is the first
+    // scalar gradient of material m (-1: not differentiated).
+    const DevMaterial *evalMats;
+    const float4 *evalTexels;
+    const int32_t *gradBase;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -233,6 +241,46 @@ PG3_DEV F3 EmEvalUV(const DevEm &E, float u, float v) {                         
     const F3 a = Texel(E, ax, ay) * (1 - xl) + Texel(E, bx, ay) * xl;
     const F3 b = Texel(E, ax, by) * (1 - xl) + Texel(E, bx, by) * xl;
     return a * (1 - yl) + b * yl;
+}
+// Texels and weights behind EmEvalUV at a direction, evaluated over any texel array with the
+// same arithmetic (differentiable rendering reads the radiance of evalTexels through it and
+// scatters texel gradients with the weights).
+struct EmFoot {
+    uint32_t a, b, c, d;     // texel indices y*w + x of (x0,y0), (x1,y0), (x0,y1), (x1,y1)
+    float xl, yl;            // bilinear weights; box filter: a only, xl = yl = 0
+};
+PG3_DEV EmFoot EmFootprint(const DevEm &E, F3 dir) {
+    float u, v;
+    Dir2LatLong(dir, u, v);
+    const float xf = u * (float)E.w, yf = v * (float)E.h;
+    EmFoot f;
+    if (!E.bilinear) {
+        const uint32_t x0 = min((uint32_t)xf, E.w - 1u), y0 = min((uint32_t)yf, E.h - 1u);
+        f.a = f.b = f.c = f.d = E.w * y0 + x0;
+        f.xl = f.yl = 0.f;
+        return f;
+    }
+    const int32_t cx = (int32_t)(xf + 0.5f), cy = (int32_t)(yf + 0.5f);
+    const int32_t x0 = cx - 1, y0 = cy - 1;
+    f.xl = xf - ((float)x0 + 0.5f);
+    f.yl = yf - ((float)y0 + 0.5f);
+    uint32_t ax, ay, bx, by;
+    EmNorm(E, x0, y0, ax, ay);
+    EmNorm(E, cx, cy, bx, by);
+    f.a = E.w * ay + ax;
+    f.b = E.w * ay + bx;
+    f.c = E.w * by + ax;
+    f.d = E.w * by + bx;
+    return f;
+}
+PG3_DEV F3 TexelAt(const float4 *tex, uint32_t i) {
+    const F4v t = ((GlobalPtr<F4v>)AsGlobal(tex))[i];
+    return mk(t.x, t.y, t.z);
+}
+PG3_DEV F3 EmEvalFoot(const float4 *tex, const EmFoot &f) {
+    const F3 a = TexelAt(tex, f.a) * (1 - f.xl) + TexelAt(tex, f.b) * f.xl;
+    const F3 b = TexelAt(tex, f.c) * (1 - f.xl) + TexelAt(tex, f.d) * f.xl;
+    return a * (1 - f.yl) + b * f.yl;
 }
 PG3_DEV F3 EmEval(const DevEm &E, F3 dir) {
     float u, v;

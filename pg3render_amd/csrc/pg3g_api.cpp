@@ -62,6 +62,17 @@ struct SceneSlot {
     pg3g_render_params params{};
     DevBuf tris, sphs, mats, lights, texels, margCdf, condCdf, rowInt, nodeW, leafVerts, fb;
     DevBuf viewDev;      // device copy of `view`, refreshed on upload and parameter changes
+    // differentiable rendering
+    std::vector<pg3g_material> hostMats;      // sampling materials as given (ctor arguments)
+    std::vector<pg3g_material> evalHostMats;  // evaluation materials
+    std::vector<float> hostTexels;            // sampling EM texels (w*h*3) for table rebuilds
+    std::vector<float> evalHostTexels;        // evaluation EM texels when they differ (evalTexDirty)
+    bool evalTexDirty = false;
+    pg3g_em_desc emDesc{};
+    DevBuf evalMats, evalTexels, gradBase, gradScal, gradEm, adjImg;
+    std::vector<uint32_t> diffMats;           // differentiated materials, scalar layout order
+    std::vector<int32_t> gradBaseHost;
+    uint32_t numScalars = 0;
     uint64_t iterations = 0;
     SteerableTables steer;
     bool hasSteer = false;
@@ -314,6 +325,11 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
     }
     std::vector<DevMaterial> mats(d->num_materials);
     for (uint32_t i = 0; i < d->num_materials; ++i) mats[i] = ToDev(d->materials[i]);
+    S.hostMats.assign(d->materials, d->materials + d->num_materials);
+    S.evalHostMats = S.hostMats;
+    S.diffMats.clear();
+    S.numScalars = 0;
+    S.gradBaseHost.assign(d->num_materials, -1);
     std::vector<DevLight> lights(d->num_lights);
     v.bgLight = -1;
     for (uint32_t i = 0; i < d->num_lights; ++i) {
@@ -351,6 +367,9 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
     v.numTris = d->num_triangles;
     v.numSphs = d->num_spheres;
     v.numLights = d->num_lights;
+    PG3_HIP(Upload(S.gradBase, S.gradBaseHost.data(), S.gradBaseHost.size(), st));
+    v.evalMats = v.mats;
+    v.gradBase = S.gradBase.as<int32_t>();
 
     // Environment map + sampling tables
     DevEm &E = v.em;
@@ -366,6 +385,9 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
             tex[i] = make_float4(d->em.texels[i * 3], d->em.texels[i * 3 + 1], d->em.texels[i * 3 + 2], 0.f);
         PG3_HIP(Upload(S.texels, tex.data(), tex.size(), st));
         E.texels = S.texels.as<float4>();
+        S.hostTexels.assign(d->em.texels, d->em.texels + (size_t)E.w * E.h * 3);
+        S.emDesc = d->em;
+        S.emDesc.texels = nullptr;
         {
             // The 2-D CDF is built for every image EM: besides the simple sampler it serves
             // EnvironmentMap::EstimateIrradiance's MC fallback (em.hxx:123-170).
@@ -396,6 +418,7 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
         }
     }
 
+    v.evalTexels = E.texels;
     CameraMatrix(d->camera, v.r2w);
     v.camPos = mk(d->camera.position[0], d->camera.position[1], d->camera.position[2]);
     v.resX = d->camera.res_x;
@@ -412,6 +435,53 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
     PG3_HIP(hipStreamSynchronize(st));
     S.valid = true;
     ++ctx->sceneEpoch;
+    return PG3G_OK;
+}
+
+// ---- differentiable rendering ------------------------------------------------------------------
+uint32_t DiffScalarCount(int32_t type) {
+    return type == PG3G_MAT_LAMBERT ? 3u : (type == PG3G_MAT_GGX_CONDUCTOR ? 4u : 0u);
+}
+
+int SyncView(pg3g_context *ctx, SceneSlot &S) {
+    PG3_HIP(hipMemcpyAsync(S.viewDev.p, &S.view, sizeof(SceneView), hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    ++ctx->sceneEpoch;
+    return PG3G_OK;
+}
+
+// Rebuilds the EM sampling tables of a slot from S.hostTexels (already uploaded to S.texels).
+int RebuildEmTables(pg3g_context *ctx, SceneSlot &S) {
+    DevEm &E = S.view.em;
+    PG3_HIP(LaunchCdfBuild(E, S.condCdf.as<float>(), S.margCdf.as<float>(), S.rowInt.as<float>(), ctx->stream));
+    if (E.sampler == PG3G_EM_STEERABLE) {
+        std::string err;
+        const pg3g_em_desc &d = S.emDesc;
+        const uint32_t mn = d.st_min_subdiv_level >= 0 ? (uint32_t)d.st_min_subdiv_level : 4u;
+        const uint32_t mx = d.st_max_subdiv_level >= 0 ? (uint32_t)d.st_max_subdiv_level : 9u;
+        SteerableTables t;
+        if (!BuildSteerable(S.hostTexels.data(), E.w, E.h, E.bilinear != 0, d.st_max_approx_error, mn, mx, t, err))
+            return Fail(PG3G_ERR_INVALID_ARG, err);
+        S.steer = std::move(t);
+        PG3_HIP(Upload(S.nodeW, S.steer.nodeW.data(), S.steer.nodeW.size(), ctx->stream));
+        PG3_HIP(Upload(S.leafVerts, S.steer.leafVerts.data(), S.steer.leafVerts.size(), ctx->stream));
+        E.nodeW = S.nodeW.as<float>();
+        E.leafVerts = S.leafVerts.as<float>();
+        E.numLevels = S.steer.numLevels;
+        for (uint32_t l = 0; l < S.steer.numLevels; ++l) {
+            E.levelBase[l] = S.steer.levelBase[l];
+            E.levelCount[l] = S.steer.levelCount[l];
+        }
+        for (int j = 0; j < 9; ++j) E.rootW[j] = S.steer.rootW[j];
+    }
+    return PG3G_OK;
+}
+
+int CheckDiffSlot(pg3g_context *ctx, uint32_t slot, SceneSlot *&S) {
+    S = GetSlot(ctx, slot);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
+    if (S->view.algorithm != PG3G_ALG_PATH_TRACING || S->view.maxPL == 0)
+        return Fail(PG3G_ERR_UNSUPPORTED, "differentiable rendering needs the path tracer (pt) with -maxpl > 0");
     return PG3G_OK;
 }
 
@@ -929,6 +999,198 @@ int pg3g_batch_autotune(pg3g_batch *batch, uint32_t max_candidates, float *best_
     if (int rc = pg3g_batch_set_config(batch, &bestCfg)) return rc;
     if (int rc = BatchRestart(batch)) return rc;
     if (best_pass_ms) *best_pass_ms = best;
+    return PG3G_OK;
+}
+
+int pg3g_diff_setup(pg3g_context *ctx, uint32_t slot, uint32_t num_materials, const uint32_t *material_ids,
+                    uint32_t *num_scalars) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (num_materials > 0 && !material_ids) return Fail(PG3G_ERR_INVALID_ARG, "material_ids is null");
+    std::vector<int32_t> base(S->hostMats.size(), -1);
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < num_materials; ++i) {
+        const uint32_t m = material_ids[i];
+        if (m >= S->hostMats.size()) return Fail(PG3G_ERR_INVALID_ARG, "material id out of range");
+        if (base[m] >= 0) return Fail(PG3G_ERR_INVALID_ARG, "material listed twice");
+        const uint32_t c = DiffScalarCount(S->hostMats[m].type);
+        if (c == 0)
+            return Fail(PG3G_ERR_UNSUPPORTED, "differentiable materials: Lambert (albedo) and GGX conductor (tint, alpha)");
+        base[m] = (int32_t)n;
+        n += c;
+    }
+    if (n > kMaxDiffScalars) return Fail(PG3G_ERR_INVALID_ARG, "too many differentiable parameters");
+    PG3_HIP(hipSetDevice(ctx->device));
+    S->diffMats.assign(material_ids, material_ids + num_materials);
+    S->gradBaseHost = base;
+    S->numScalars = n;
+    PG3_HIP(Upload(S->gradBase, base.data(), base.size(), ctx->stream));
+    S->view.gradBase = S->gradBase.as<int32_t>();
+    if (num_scalars) *num_scalars = n;
+    return SyncView(ctx, *S);
+}
+
+int pg3g_diff_get_params(pg3g_context *ctx, uint32_t slot, float *scalars, float *em_texels) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (scalars) {
+        for (uint32_t m : S->diffMats) {
+            const pg3g_material &d = S->evalHostMats[m];
+            float *o = scalars + S->gradBaseHost[m];
+            if (d.type == PG3G_MAT_LAMBERT) { o[0] = d.diffuse[0]; o[1] = d.diffuse[1]; o[2] = d.diffuse[2]; }
+            else { o[0] = d.glossy[0]; o[1] = d.glossy[1]; o[2] = d.glossy[2]; o[3] = d.alpha; }
+        }
+    }
+    if (em_texels) {
+        if (S->view.em.w == 0) return Fail(PG3G_ERR_INVALID_ARG, "the scene has no EM image");
+        const size_t n = (size_t)S->view.em.w * S->view.em.h;
+        std::vector<float4> tex(n);
+        PG3_HIP(hipSetDevice(ctx->device));
+        PG3_HIP(hipMemcpyAsync(tex.data(), S->view.evalTexels, sizeof(float4) * n, hipMemcpyDeviceToHost, ctx->stream));
+        PG3_HIP(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < n; ++i) { em_texels[i * 3] = tex[i].x; em_texels[i * 3 + 1] = tex[i].y; em_texels[i * 3 + 2] = tex[i].z; }
+    }
+    return PG3G_OK;
+}
+
+int pg3g_diff_set_params(pg3g_context *ctx, uint32_t slot, const float *scalars, const float *em_texels,
+                         int refresh_sampling) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (em_texels && S->view.em.w == 0) return Fail(PG3G_ERR_INVALID_ARG, "the scene has no EM image");
+    PG3_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    SceneView &v = S->view;
+    if (scalars) {
+        for (uint32_t m : S->diffMats) {
+            pg3g_material &d = S->evalHostMats[m];
+            const float *p = scalars + S->gradBaseHost[m];
+            for (int c = 0; c < 3; ++c) {
+                if (!(p[c] >= 0.f)) return Fail(PG3G_ERR_INVALID_ARG, "albedo / tint must be >= 0");
+                (d.type == PG3G_MAT_LAMBERT ? d.diffuse[c] : d.glossy[c]) = p[c];
+            }
+            if (d.type == PG3G_MAT_GGX_CONDUCTOR) {
+                if (!(p[3] > 0.f)) return Fail(PG3G_ERR_INVALID_ARG, "alpha must be > 0");
+                d.alpha = p[3];
+            }
+        }
+    }
+    std::vector<DevMaterial> em(S->evalHostMats.size());
+    for (size_t i = 0; i < em.size(); ++i) em[i] = ToDev(S->evalHostMats[i]);
+    PG3_HIP(Upload(S->evalMats, em.data(), em.size(), st));
+    const size_t n = (size_t)v.em.w * v.em.h;
+    if (em_texels) {
+        std::vector<float4> tex(n);
+        for (size_t i = 0; i < n; ++i) {
+            if (!(em_texels[i * 3] >= 0.f && em_texels[i * 3 + 1] >= 0.f && em_texels[i * 3 + 2] >= 0.f))
+                return Fail(PG3G_ERR_INVALID_ARG, "EM texels must be finite and >= 0");
+            tex[i] = make_float4(em_texels[i * 3], em_texels[i * 3 + 1], em_texels[i * 3 + 2], 0.f);
+        }
+        PG3_HIP(Upload(S->evalTexels, tex.data(), n, st));
+        S->evalHostTexels.assign(em_texels, em_texels + n * 3);
+        S->evalTexDirty = true;
+    } else if (v.em.w > 0 && !S->evalTexels.p) {
+        PG3_HIP(S->evalTexels.Alloc(S->texels.bytes));
+        PG3_HIP(hipMemcpyAsync(S->evalTexels.p, S->texels.p, S->texels.bytes, hipMemcpyDeviceToDevice, st));
+    }
+    if (refresh_sampling) {
+        // the evaluation parameters become the sampling parameters
+        S->hostMats = S->evalHostMats;
+        PG3_HIP(Upload(S->mats, em.data(), em.size(), st));
+        v.mats = S->mats.as<DevMaterial>();
+        v.evalMats = v.mats;
+        if (v.em.w > 0) {
+            if (S->evalTexDirty) {
+                S->hostTexels = S->evalHostTexels;
+                PG3_HIP(hipMemcpyAsync(S->texels.p, S->evalTexels.p, sizeof(float4) * n, hipMemcpyDeviceToDevice, st));
+                if (int rc = RebuildEmTables(ctx, *S)) return rc;
+                S->evalTexDirty = false;
+            }
+            v.evalTexels = v.em.texels;
+        }
+    } else {
+        v.evalMats = S->evalMats.as<DevMaterial>();
+        if (v.em.w > 0) v.evalTexels = S->evalTexels.as<float4>();
+    }
+    return SyncView(ctx, *S);
+}
+
+int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, float *elapsed_ms) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if ((uint64_t)first_iter + n_iter > 0xFFFFFFFFull) return Fail(PG3G_ERR_INVALID_ARG, "iteration range overflows");
+    PG3_HIP(hipSetDevice(ctx->device));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (elapsed_ms) {
+        PG3_HIP(hipEventCreate(&e0));
+        PG3_HIP(hipEventCreate(&e1));
+        PG3_HIP(hipEventRecord(e0, ctx->stream));
+    }
+    hipError_t e = LaunchDiff(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(), nullptr,
+                              nullptr, nullptr, 0, ctx->counters.as<unsigned long long>(), ctx->stream);
+    if (elapsed_ms) {
+        if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *elapsed_ms = ms;
+    }
+    PG3_HIP(e);
+    S->iterations += n_iter;
+    return PG3G_OK;
+}
+
+int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, const float *dloss_dimage,
+                 double *grad_scalars, double *grad_em, float *elapsed_ms) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (!dloss_dimage) return Fail(PG3G_ERR_INVALID_ARG, "dloss_dimage is null");
+    if (n_iter == 0) return Fail(PG3G_ERR_INVALID_ARG, "n_iter must be >= 1");
+    if ((uint64_t)first_iter + n_iter > 0xFFFFFFFFull) return Fail(PG3G_ERR_INVALID_ARG, "iteration range overflows");
+    const SceneView &v = S->view;
+    PG3_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t npx = (size_t)v.resX * v.resY;
+    PG3_HIP(Upload(S->adjImg, dloss_dimage, npx * 3, st));
+    PG3_HIP(S->gradScal.Alloc(sizeof(double) * kMaxDiffScalars));
+    PG3_HIP(hipMemsetAsync(S->gradScal.p, 0, sizeof(double) * kMaxDiffScalars, st));
+    const size_t emN = (size_t)v.em.w * v.em.h * 3;
+    double *gEm = nullptr;
+    if (emN > 0) {
+        PG3_HIP(S->gradEm.Alloc(sizeof(double) * emN));
+        PG3_HIP(hipMemsetAsync(S->gradEm.p, 0, sizeof(double) * emN, st));
+        gEm = S->gradEm.as<double>();
+    }
+    hipEvent_t e0, e1;
+    PG3_HIP(hipEventCreate(&e0));
+    PG3_HIP(hipEventCreate(&e1));
+    hipError_t e = hipEventRecord(e0, st);
+    if (e == hipSuccess)
+        e = LaunchDiff(v, S->viewDev.as<SceneView>(), first_iter, n_iter, nullptr, S->adjImg.as<float>(), gEm,
+                       S->gradScal.as<double>(), S->numScalars, ctx->counters.as<unsigned long long>(), st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    PG3_HIP(e);
+    if (elapsed_ms) *elapsed_ms = ms;
+    if (grad_scalars && S->numScalars > 0) {
+        PG3_HIP(hipMemcpy(grad_scalars, S->gradScal.p, sizeof(double) * S->numScalars, hipMemcpyDeviceToHost));
+        // alpha is clamped to [0.001, 1] by the material (materials.hxx:805): no gradient outside
+        for (uint32_t m : S->diffMats)
+            if (S->evalHostMats[m].type == PG3G_MAT_GGX_CONDUCTOR) {
+                const float a = S->evalHostMats[m].alpha;
+                if (a < 0.001f || a > 1.0f) grad_scalars[S->gradBaseHost[m] + 3] = 0.0;
+            }
+    }
+    if (grad_em) {
+        if (emN == 0) return Fail(PG3G_ERR_INVALID_ARG, "the scene has no EM image");
+        PG3_HIP(hipMemcpy(grad_em, S->gradEm.p, sizeof(double) * emN, hipMemcpyDeviceToHost));
+    }
     return PG3G_OK;
 }
 

@@ -43,6 +43,15 @@ struct BatchLaunch {
 };
 hipError_t LaunchBatchPass(const BatchLaunch &L, hipStream_t stream);
 
+// Differentiable render (diff.hip).  adjImg == nullptr: forward with frozen sampling into fb.
+// Otherwise the adjoint of iterations [firstIter, firstIter + nIter): adjImg = dLoss/dImage
+// (res_x*res_y*3, image = sum / nIter), gradients added to gradEm (w*h*3, may be null when the
+// scene has no EM) and gradScal (numScal <= kMaxDiffScalars).
+constexpr uint32_t kMaxDiffScalars = 64;
+hipError_t LaunchDiff(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
+                      const float *adjImg, double *gradEm, double *gradScal, uint32_t numScal,
+                      unsigned long long *counters, hipStream_t stream);
+
 hipError_t LaunchBsdfEval(const DevMaterial &m, uint32_t n, const float *wil, const float *wol, float *f, float *pdf,
                           float *comp, hipStream_t s);
 hipError_t LaunchBsdfSample(const DevMaterial &m, uint32_t n, const float *wol, const float *u, float *wil, float *f,

@@ -13,6 +13,8 @@ namespace pg3 {
 
 constexpr uint32_t kMaxPickLights = 8;    // per-vertex light-contribution cache kept in registers
 
+constexpr int kEmDynamic = -2;   // kernel template value: light spec read from the scene at run time
+
 // Sum over the 64 lanes of a wavefront.
 __device__ __forceinline__ uint32_t WaveSum(uint32_t v) {
 #pragma unroll
