@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-vs-plain measurement")
     ap.add_argument("--batch-tune", type=int, default=5, help="autotuner candidates for the batch (0 = all)")
+    ap.add_argument("--no-diff", action="store_true", help="skip the adjoint and inverse-rendering measurements")
+    ap.add_argument("--inverse-steps", type=int, default=200, help="Adam steps of the inverse-rendering run")
     return ap.parse_args()
 
 
@@ -100,6 +102,37 @@ def batch_measure(ctx, tune_candidates):
             "plain_ms_per_pass": round(plain_s * 1e3, 3), "autotuned_config": cfg, "autotune_s": round(tune_s, 3)}
 
 
+def diff_measure(ctx):
+    """Differentiation workload: box with two GGX-conductor spheres (tint + roughness) and a
+    128x64 EM, 256x256, 32 spp; forward (frozen-sampling render) vs path-replay adjoint of an L2
+    loss, both timed with HIP events on the library stream."""
+    import numpy as np
+    from pg3render_amd import SceneBuild
+    from pg3render_amd.diff import DiffScene
+    sb = SceneBuild.workload(4, seed=2024)
+    ctx.upload(sb, 70)
+    ds = DiffScene(ctx, sb, [3, 4], slot=70)
+    spp = sb.spp
+    theta, em = ds.get_params()
+    target = ds.render(1 << 20, spp)
+    ds.set_params(theta * 1.1, em * 0.9, refresh_sampling=True)
+    img = ds.render(0, spp)
+    dl = 2.0 * (img - target) / img.size
+    ds.adjoint(0, spp, dl)                            # warm-up
+    fwd = min(ds.render(0, spp, timed=True)[1] for _ in range(3))
+    # the forward keeps each sample's radiance, so the adjoint of the same samples needs one replay
+    adj = min(ds.adjoint(0, spp, dl, timed=True)[2] for _ in range(3))
+    # standalone adjoint (no stored radiance: primal replay + gradient replay)
+    ds.set_params(theta * 1.1, em * 0.9, refresh_sampling=True)
+    adj2 = min(ds.adjoint(0, spp, dl, timed=True)[2] for _ in range(2))
+    w, h = sb.resolution
+    return {"workload": "box + 2 GGX-conductor spheres (tint, roughness) + 128x64 EM (simple sampler), "
+                        f"{w}x{h}, {spp} spp, maxpl 8; L2 image loss", "parameters": 8 + int(em.size),
+            "forward_ms": round(fwd, 3), "adjoint_ms": round(adj, 3), "adjoint_over_forward": round(adj / fwd, 3),
+            "adjoint_two_replays_ms": round(adj2, 3), "adjoint_two_replays_over_forward": round(adj2 / fwd, 3),
+            "forward_msamples_per_s": round(w * h * spp / fwd / 1e3, 3)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -156,6 +189,13 @@ def main():
            "rays_per_s": round(rays_per_s, 1), "roofline": roofline}
     if not args.no_batch:
         out["batched"] = batch_measure(ctx, args.batch_tune)
+    if not args.no_diff:
+        out["differentiation"] = diff_measure(ctx)
+        if args.inverse_steps > 0:
+            from pg3render_amd.inverse import inverse_rendering
+            inv = inverse_rendering(ctx, steps=args.inverse_steps)
+            inv.pop("loss_history")
+            out["inverse_rendering"] = inv
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sb, spp, args.cpu_seconds)
     print(json.dumps(out))

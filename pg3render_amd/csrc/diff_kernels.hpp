@@ -24,12 +24,38 @@
 
 namespace pg3 {
 
+// EM-texel gradients are gathered per workgroup in an LDS cache of FP64 sums before they reach the
+// global FP64 buffer: texel t owns slot t % kEmCacheSlots once a lane claims it (compare-and-swap
+// on the slot key); a texel whose slot belongs to another texel goes straight to global memory.
+// Importance sampling sends most light samples to a few bright texels, whose global atomics
+// would otherwise serialise; the cache is flushed once per workgroup.
+constexpr uint32_t kEmCacheSlots = 4096;
+
 struct GradSink {
     F3 adj;            // dLoss/dI(pixel) / N, per channel
     F3 total;          // the sample's radiance from the primal replay
     double *em;        // EM texel gradients (w*h*3), FP64
     double *scal;      // the wavefront's LDS slice of scalar gradients
+    int32_t *cacheKeys;   // workgroup LDS cache (kEmCacheSlots keys, -1 = free), may be null
+    double *cacheVals;    // kEmCacheSlots x 3 FP64 sums
 };
+
+PG3_DEV void EmGradAdd(const GradSink &g, uint32_t t, F3 v) {
+    if (g.cacheKeys) {
+        const uint32_t s = t & (kEmCacheSlots - 1u);
+        const int32_t old = atomicCAS(g.cacheKeys + s, -1, (int32_t)t);
+        if (old == -1 || old == (int32_t)t) {
+            atomicAdd(g.cacheVals + s * 3 + 0, (double)v.x);
+            atomicAdd(g.cacheVals + s * 3 + 1, (double)v.y);
+            atomicAdd(g.cacheVals + s * 3 + 2, (double)v.z);
+            return;
+        }
+    }
+    double *p = g.em + (size_t)t * 3;
+    atomicAdd(p + 0, (double)v.x);
+    atomicAdd(p + 1, (double)v.y);
+    atomicAdd(p + 2, (double)v.z);
+}
 
 PG3_DEV bool IsDiffMaterial(int32_t type) { return type == kMatLambert || type == kMatGgxCond; }
 
@@ -43,10 +69,7 @@ PG3_DEV void ScatterEm(const GradSink &g, const EmFoot &f, F3 W) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (wt[k] == 0.f) continue;
-        double *p = g.em + (size_t)idx[k] * 3;
-        atomicAdd(p + 0, (double)(a.x * wt[k]));
-        atomicAdd(p + 1, (double)(a.y * wt[k]));
-        atomicAdd(p + 2, (double)(a.z * wt[k]));
+        EmGradAdd(g, idx[k], a * wt[k]);
     }
 }
 
@@ -211,11 +234,14 @@ PG3_DEV bool DiffVertex(const SceneView &S, const LightSpec &spec, const RngKey 
 // A work item (8x8 tile x iterations) of the differentiable forward (kAdjoint false: per-pixel
 // sums into fb) or of the adjoint (kAdjoint true: each sample is traced twice, the primal replay
 // giving its radiance and the second replay the gradients).  Samples are handed to lanes as in
-// TraceTile.
+// TraceTile.  sampleL (may be null) holds one radiance per sample of the launch, index
+// (iteration - firstIter) * pixels + pixel: the forward writes it, and the adjoint then reads the
+// radiance instead of running the primal replay.
 template <bool kAdjoint>
 PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x0, uint32_t y0, uint32_t itBegin,
                            uint32_t itEnd, double *acc, double *fb, const float *adjImg, float invN, double *gradEm,
-                           double *gradScal, uint32_t &segments, uint32_t &samples) {
+                           double *gradScal, float *sampleL, uint32_t firstIter, int32_t *cacheKeys,
+                           double *cacheVals, uint32_t &segments, uint32_t &samples) {
     const uint32_t lane = __lane_id();
     if (!kAdjoint) {
         acc[lane * 3 + 0] = 0.0;
@@ -236,6 +262,8 @@ PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x
     g.total = mk(0.f);
     g.em = gradEm;
     g.scal = gradScal;
+    g.cacheKeys = cacheKeys;
+    g.cacheVals = cacheVals;
     for (;;) {
         const uint64_t idle = __ballot(!busy);
         if (next < total && idle != 0) {
@@ -254,6 +282,12 @@ PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x
                     if (kAdjoint) {
                         const GlobalPtr<float> a = AsGlobal(adjImg + (size_t)pixel * 3);
                         g.adj = mk(a[0], a[1], a[2]) * invN;
+                        if (sampleL) {
+                            const size_t si = ((size_t)(iter - firstIter) * S.resX * S.resY + pixel) * 3;
+                            const GlobalPtr<float> l = AsGlobal(sampleL + si);
+                            g.total = mk(l[0], l[1], l[2]);
+                            second = true;
+                        }
                     }
                 }
             }
@@ -272,6 +306,12 @@ PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x
             else cont = DiffVertex<false>(S, spec, key, ps, hit, h, g, segments);
             if (!cont) {
                 if (!kAdjoint) {
+                    if (sampleL) {
+                        float *l = sampleL + ((size_t)(iter - firstIter) * S.resX * S.resY + pixel) * 3;
+                        l[0] = ps.L.x;
+                        l[1] = ps.L.y;
+                        l[2] = ps.L.z;
+                    }
                     atomicAdd(acc + pix * 3 + 0, (double)ps.L.x);
                     atomicAdd(acc + pix * 3 + 1, (double)ps.L.y);
                     atomicAdd(acc + pix * 3 + 2, (double)ps.L.z);

@@ -69,7 +69,12 @@ struct SceneSlot {
     std::vector<float> evalHostTexels;        // evaluation EM texels when they differ (evalTexDirty)
     bool evalTexDirty = false;
     pg3g_em_desc emDesc{};
-    DevBuf evalMats, evalTexels, gradBase, gradScal, gradEm, adjImg;
+    DevBuf evalMats, evalTexels, gradBase, gradScal, gradEm, adjImg, adjWork;
+    DevBuf sampleL;                           // per-sample radiance of the last differentiable forward
+    bool sampleValid = false;
+    uint32_t sampleFirst = 0, sampleCount = 0;
+    uint64_t sampleEpoch = 0;
+    uint64_t epoch = 0;                       // bumped by every change of this slot's scene view
     std::vector<uint32_t> diffMats;           // differentiated materials, scalar layout order
     std::vector<int32_t> gradBaseHost;
     uint32_t numScalars = 0;
@@ -435,6 +440,8 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
     PG3_HIP(hipStreamSynchronize(st));
     S.valid = true;
     ++ctx->sceneEpoch;
+    ++S.epoch;
+    S.sampleValid = false;
     return PG3G_OK;
 }
 
@@ -447,6 +454,7 @@ int SyncView(pg3g_context *ctx, SceneSlot &S) {
     PG3_HIP(hipMemcpyAsync(S.viewDev.p, &S.view, sizeof(SceneView), hipMemcpyHostToDevice, ctx->stream));
     PG3_HIP(hipStreamSynchronize(ctx->stream));
     ++ctx->sceneEpoch;
+    ++S.epoch;
     return PG3G_OK;
 }
 
@@ -657,10 +665,7 @@ int pg3g_set_params(pg3g_context *ctx, uint32_t slot, const pg3g_render_params *
     S->params = *p;
     ApplyParams(S->view, *p);
     PG3_HIP(hipSetDevice(ctx->device));
-    PG3_HIP(hipMemcpyAsync(S->viewDev.p, &S->view, sizeof(SceneView), hipMemcpyHostToDevice, ctx->stream));
-    PG3_HIP(hipStreamSynchronize(ctx->stream));
-    ++ctx->sceneEpoch;
-    return PG3G_OK;
+    return SyncView(ctx, *S);
 }
 
 int pg3g_set_kernel_mode(pg3g_context *ctx, int mode) {
@@ -1126,8 +1131,17 @@ int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
         PG3_HIP(hipEventCreate(&e1));
         PG3_HIP(hipEventRecord(e0, ctx->stream));
     }
+    // keep the per-sample radiance for the adjoint of the same samples (up to 2^26 samples)
+    const uint64_t ns = (uint64_t)n_iter * S->view.resX * S->view.resY;
+    float *sl = nullptr;
+    S->sampleValid = false;
+    if (ns <= (1ull << 26)) {
+        PG3_HIP(S->sampleL.Alloc(sizeof(float) * 3 * ns));
+        sl = S->sampleL.as<float>();
+    }
     hipError_t e = LaunchDiff(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(), nullptr,
-                              nullptr, nullptr, 0, ctx->counters.as<unsigned long long>(), ctx->stream);
+                              nullptr, nullptr, 0, sl, nullptr, ctx->numCUs, ctx->counters.as<unsigned long long>(),
+                              ctx->stream);
     if (elapsed_ms) {
         if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -1139,6 +1153,12 @@ int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
     }
     PG3_HIP(e);
     S->iterations += n_iter;
+    if (sl) {
+        S->sampleValid = true;
+        S->sampleFirst = first_iter;
+        S->sampleCount = n_iter;
+        S->sampleEpoch = S->epoch;
+    }
     return PG3G_OK;
 }
 
@@ -1155,6 +1175,8 @@ int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t
     const size_t npx = (size_t)v.resX * v.resY;
     PG3_HIP(Upload(S->adjImg, dloss_dimage, npx * 3, st));
     PG3_HIP(S->gradScal.Alloc(sizeof(double) * kMaxDiffScalars));
+    PG3_HIP(S->adjWork.Alloc(sizeof(uint32_t)));
+    PG3_HIP(hipMemsetAsync(S->adjWork.p, 0, sizeof(uint32_t), st));
     PG3_HIP(hipMemsetAsync(S->gradScal.p, 0, sizeof(double) * kMaxDiffScalars, st));
     const size_t emN = (size_t)v.em.w * v.em.h * 3;
     double *gEm = nullptr;
@@ -1163,13 +1185,17 @@ int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t
         PG3_HIP(hipMemsetAsync(S->gradEm.p, 0, sizeof(double) * emN, st));
         gEm = S->gradEm.as<double>();
     }
+    // radiance of the same samples kept by the last differentiable forward (same parameters)
+    const bool stored = S->sampleValid && S->sampleFirst == first_iter && S->sampleCount == n_iter &&
+                        S->sampleEpoch == S->epoch;
     hipEvent_t e0, e1;
     PG3_HIP(hipEventCreate(&e0));
     PG3_HIP(hipEventCreate(&e1));
     hipError_t e = hipEventRecord(e0, st);
     if (e == hipSuccess)
         e = LaunchDiff(v, S->viewDev.as<SceneView>(), first_iter, n_iter, nullptr, S->adjImg.as<float>(), gEm,
-                       S->gradScal.as<double>(), S->numScalars, ctx->counters.as<unsigned long long>(), st);
+                       S->gradScal.as<double>(), S->numScalars, stored ? S->sampleL.as<float>() : nullptr,
+                       S->adjWork.as<uint32_t>(), ctx->numCUs, ctx->counters.as<unsigned long long>(), st);
     if (e == hipSuccess) e = hipEventRecord(e1, st);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.f;

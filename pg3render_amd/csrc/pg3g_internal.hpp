@@ -46,11 +46,14 @@ hipError_t LaunchBatchPass(const BatchLaunch &L, hipStream_t stream);
 // Differentiable render (diff.hip).  adjImg == nullptr: forward with frozen sampling into fb.
 // Otherwise the adjoint of iterations [firstIter, firstIter + nIter): adjImg = dLoss/dImage
 // (res_x*res_y*3, image = sum / nIter), gradients added to gradEm (w*h*3, may be null when the
-// scene has no EM) and gradScal (numScal <= kMaxDiffScalars).
+// scene has no EM) and gradScal (numScal <= kMaxDiffScalars).  sampleL (may be null): per-sample
+// radiance, nIter * res_x * res_y * 3 floats, written by the forward and read by the adjoint in
+// place of its primal replay.  The adjoint is a persistent kernel (numCUs workgroups) taking work
+// items from *work, which must be zero at launch.
 constexpr uint32_t kMaxDiffScalars = 64;
 hipError_t LaunchDiff(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
-                      const float *adjImg, double *gradEm, double *gradScal, uint32_t numScal,
-                      unsigned long long *counters, hipStream_t stream);
+                      const float *adjImg, double *gradEm, double *gradScal, uint32_t numScal, float *sampleL,
+                      uint32_t *work, int numCUs, unsigned long long *counters, hipStream_t stream);
 
 hipError_t LaunchBsdfEval(const DevMaterial &m, uint32_t n, const float *wil, const float *wol, float *f, float *pdf,
                           float *comp, hipStream_t s);
