@@ -20,6 +20,9 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 (vector) peak
 TRI_FLOP, SPH_FLOP, SHADE_FLOP = 62, 35, 400
 BYTES_PER_SAMPLE, BYTES_PER_SEGMENT = 112, 304
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 KiB, factors from a 512 MiB copy): profiles/render_pt_box_steerable_em_512px_64spp.md
+MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 9147551 + 7032478) * 1024)}
 
 
 def parse():
@@ -34,6 +37,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-vs-plain measurement")
     ap.add_argument("--batch-tune", type=int, default=5, help="autotuner candidates for the batch (0 = all)")
     ap.add_argument("--no-diff", action="store_true", help="skip the adjoint and inverse-rendering measurements")
+    ap.add_argument("--copy-probe", action="store_true", help="only run the HBM-counter calibration copy")
     ap.add_argument("--inverse-steps", type=int, default=200, help="Adam steps of the inverse-rendering run")
     return ap.parse_args()
 
@@ -43,6 +47,8 @@ def cpu_baseline(sb, spp, target_s):
     from oracle import oracle as O
     w, h = sb.resolution
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():       # the host share given to this job
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
     orc = O.OracleScene(sb)
     # calibrate on 4 rows, then size the band to about target_s seconds
     probe_rows = 4
@@ -57,10 +63,10 @@ def cpu_baseline(sb, spp, target_s):
 
 
 def batch_measure(ctx, tune_candidates):
-    """64-scene batch workload: per-scene megakernel launches vs one persistent batched pass.
-
-    Both render the same samples (each scene's spp iterations 0..spp-1, same RNG streams), so
-    they reach the same accuracy; the ratio of their rates is the batching speedup."""
+    """64-scene batch workload: one persistent batched pass vs per-scene rendering, both of the
+    same samples (each scene's spp iterations 0..spp-1, same RNG streams), so at equal accuracy.
+    Two per-scene baselines: one launch per scene rendering all its iterations (`speedup`), and
+    the drop-in RunIteration loop, one launch per iteration (`speedup_vs_run_iteration`)."""
     from pg3render_amd import Batch, SceneBuild, _ffi
     scenes = [SceneBuild.workload(3, seed=2024, variant=v) for v in range(64)]
     slots = list(range(1, 65))
@@ -78,8 +84,20 @@ def batch_measure(ctx, tune_candidates):
         ctx.synchronize()
         return time.perf_counter() - t0
 
+    def run_iteration_pass():
+        # the drop-in renderer as PG3Render's Render() loop drives it: RunIteration, one sample per
+        # pixel per call (pg3render.cxx:84-128), each call one launch
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for s, n in zip(slots, spp):
+            for it in range(n):
+                ctx.render(it, 1, s)
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
     plain_pass()                                   # warm-up (code objects, caches)
     plain_s = min(plain_pass() for _ in range(2))
+    run_it_s = run_iteration_pass()
     batch = Batch(ctx, slots, spp)
     t0 = time.perf_counter()
     batch.autotune(tune_candidates)
@@ -97,8 +115,10 @@ def batch_measure(ctx, tune_candidates):
     return {"workload": "64 seeded box scenes (2 area-light triangles + 256x128 sky/sun EM, steerable), "
                         "side 64..512, spp in {4,16,64,256}, maxpl 8", "samples_per_pass": total,
             "plain_per_scene_msamples_per_s": round(total / plain_s / 1e6, 3),
+            "run_iteration_msamples_per_s": round(total / run_it_s / 1e6, 3),
             "batched_msamples_per_s": round(total / batched_s / 1e6, 3),
-            "speedup": round(plain_s / batched_s, 3), "batched_gpu_ms_per_pass": round(min(gpu_ms), 3),
+            "speedup": round(plain_s / batched_s, 3), "speedup_vs_run_iteration": round(run_it_s / batched_s, 3),
+            "batched_gpu_ms_per_pass": round(min(gpu_ms), 3),
             "plain_ms_per_pass": round(plain_s * 1e3, 3), "autotuned_config": cfg, "autotune_s": round(tune_s, 3)}
 
 
@@ -122,9 +142,12 @@ def diff_measure(ctx):
     fwd = min(ds.render(0, spp, timed=True)[1] for _ in range(3))
     # the forward keeps each sample's radiance, so the adjoint of the same samples needs one replay
     adj = min(ds.adjoint(0, spp, dl, timed=True)[2] for _ in range(3))
-    # standalone adjoint (no stored radiance: primal replay + gradient replay)
-    ds.set_params(theta * 1.1, em * 0.9, refresh_sampling=True)
-    adj2 = min(ds.adjoint(0, spp, dl, timed=True)[2] for _ in range(2))
+    # standalone adjoint (no stored radiance: the adjoint runs its own forward pass first)
+    adj2 = []
+    for _ in range(2):
+        ds.set_params(theta * 1.1, em * 0.9, refresh_sampling=True)      # drops the stored radiance
+        adj2.append(ds.adjoint(0, spp, dl, timed=True)[2])
+    adj2 = min(adj2)
     w, h = sb.resolution
     return {"workload": "box + 2 GGX-conductor spheres (tint, roughness) + 128x64 EM (simple sampler), "
                         f"{w}x{h}, {spp} spp, maxpl 8; L2 image loss", "parameters": 8 + int(em.size),
@@ -133,8 +156,24 @@ def diff_measure(ctx):
             "forward_msamples_per_s": round(w * h * spp / fwd / 1e3, 3)}
 
 
+def copy_probe():
+    """Calibration kernel for the HBM counters: copies 512 MiB device-to-device 4 times (torch's
+    copy kernel reads and writes exactly 512 MiB per launch)."""
+    import torch
+    n = 512 * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_launch": n * 4, "launches": 4}))
+
+
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build", file=sys.stderr)
         sys.exit(2)
@@ -175,7 +214,8 @@ def main():
     gbs = bytes_alg / kernel_s / 1e9
     rays_per_s = (samples + 2 * segments) / wall
     roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                "traffic": MEASURED_TRAFFIC.get((args.workload, args.kernel)),
                 "compute": "fp32 VALU (no matrix work on this path); peak = MI355X FP32 157.3 TFLOP/s",
                 "hbm_model_gbs": round(gbs, 1), "hbm_model_frac": round(gbs / HBM_PEAK_GBS, 4),
                 "kernel_ms_per_step": round(sum(kernel_ms) / len(kernel_ms), 3),

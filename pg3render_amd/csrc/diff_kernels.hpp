@@ -335,7 +335,7 @@ PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const uint32_t x = x0 + (lane & 7u), y = y0 + (lane >> 3);
-        if (x < S.resX && y < S.resY) {
+        if (fb && x < S.resX && y < S.resY) {
             double *p = fb + (size_t)(y * S.resX + x) * 3;
             atomicAdd(p + 0, acc[lane * 3 + 0]);
             atomicAdd(p + 1, acc[lane * 3 + 1]);

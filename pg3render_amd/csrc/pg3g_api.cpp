@@ -1185,13 +1185,25 @@ int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t
         PG3_HIP(hipMemsetAsync(S->gradEm.p, 0, sizeof(double) * emN, st));
         gEm = S->gradEm.as<double>();
     }
-    // radiance of the same samples kept by the last differentiable forward (same parameters)
-    const bool stored = S->sampleValid && S->sampleFirst == first_iter && S->sampleCount == n_iter &&
-                        S->sampleEpoch == S->epoch;
+    // radiance of the same samples kept by the last differentiable forward (same parameters);
+    // without it a forward pass computes it first (cheaper than interleaving a primal replay with
+    // the gradient replay in every wavefront)
+    bool stored = S->sampleValid && S->sampleFirst == first_iter && S->sampleCount == n_iter &&
+                  S->sampleEpoch == S->epoch;
+    const uint64_t ns = (uint64_t)n_iter * v.resX * v.resY;
+    bool primal = false;
+    if (!stored && ns <= (1ull << 26)) {
+        PG3_HIP(S->sampleL.Alloc(sizeof(float) * 3 * ns));
+        S->sampleValid = false;
+        stored = primal = true;
+    }
     hipEvent_t e0, e1;
     PG3_HIP(hipEventCreate(&e0));
     PG3_HIP(hipEventCreate(&e1));
     hipError_t e = hipEventRecord(e0, st);
+    if (e == hipSuccess && primal)
+        e = LaunchDiff(v, S->viewDev.as<SceneView>(), first_iter, n_iter, nullptr, nullptr, nullptr, nullptr, 0,
+                       S->sampleL.as<float>(), nullptr, ctx->numCUs, ctx->counters.as<unsigned long long>(), st);
     if (e == hipSuccess)
         e = LaunchDiff(v, S->viewDev.as<SceneView>(), first_iter, n_iter, nullptr, S->adjImg.as<float>(), gEm,
                        S->gradScal.as<double>(), S->numScalars, stored ? S->sampleL.as<float>() : nullptr,
@@ -1203,6 +1215,12 @@ int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     PG3_HIP(e);
+    if (primal) {
+        S->sampleValid = true;
+        S->sampleFirst = first_iter;
+        S->sampleCount = n_iter;
+        S->sampleEpoch = S->epoch;
+    }
     if (elapsed_ms) *elapsed_ms = ms;
     if (grad_scalars && S->numScalars > 0) {
         PG3_HIP(hipMemcpy(grad_scalars, S->gradScal.p, sizeof(double) * S->numScalars, hipMemcpyDeviceToHost));
