@@ -66,6 +66,7 @@ struct SceneView {
     int32_t algorithm;
     uint32_t minPL, maxPL;
     float iic;
+    float splitBudget, splitLevel, lightRatio;   // -sb, splitting level, light-to-BSDF sample ratio
     uint32_t k0, k1, sceneId;
     // Differentiable rendering with frozen sampling: integrand values (BSDF values of Lambert and
     // GGX conductor materials, EM radiance) come from evalMats / evalTexels, while directions,

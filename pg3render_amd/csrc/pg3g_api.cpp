@@ -273,10 +273,13 @@ int ValidateParams(const pg3g_render_params &p) {
     if (p.algorithm < PG3G_ALG_DIRECT_BSDF || p.algorithm > PG3G_ALG_PATH_TRACING)
         return Fail(PG3G_ERR_UNSUPPORTED, "algorithm is not on the GPU path (supported: dbs dlsa dlss dmis ptn pt)");
     if (p.algorithm == PG3G_ALG_PATH_TRACING) {
-        if (std::lround(p.splitting_budget) != 1 || p.splitting_budget != 1.f)
-            return Fail(PG3G_ERR_UNSUPPORTED, "the GPU path tracer supports splitting budget 1 (-sb 1)");
-        if (p.indirect_clipping > 0.f)
-            return Fail(PG3G_ERR_UNSUPPORTED, "the GPU path tracer supports -iic 0 (indirect clipping off)");
+        if (!(p.splitting_budget >= 1.f && p.splitting_budget <= 64.f))
+            return Fail(PG3G_ERR_UNSUPPORTED, "the GPU path tracer supports splitting budgets 1..64 (-sb)");
+        if (!(p.splitting_level >= 0.f && p.splitting_level <= 1.f))
+            return Fail(PG3G_ERR_INVALID_ARG, "splitting level must be in [0, 1]");
+        if (!(p.light_to_bsdf_ratio > 0.f && p.light_to_bsdf_ratio <= 16.f))
+            return Fail(PG3G_ERR_UNSUPPORTED, "the GPU path tracer supports light-to-BSDF sample ratios in (0, 16]");
+        if (!(p.indirect_clipping >= 0.f)) return Fail(PG3G_ERR_INVALID_ARG, "indirect clipping must be >= 0");
     }
     if (p.max_path_length > 0 && p.min_path_length > p.max_path_length)
         return Fail(PG3G_ERR_INVALID_ARG, "min path length exceeds max path length");
@@ -288,6 +291,9 @@ void ApplyParams(SceneView &v, const pg3g_render_params &p) {
     v.minPL = p.min_path_length;
     v.maxPL = p.max_path_length;
     v.iic = p.indirect_clipping;
+    v.splitBudget = p.splitting_budget;
+    v.splitLevel = p.splitting_level;
+    v.lightRatio = p.light_to_bsdf_ratio;
     v.k0 = (uint32_t)p.seed;
     v.k1 = (uint32_t)(p.seed >> 32);
     v.sceneId = p.scene_id & 0xFFFFu;
@@ -488,8 +494,9 @@ int RebuildEmTables(pg3g_context *ctx, SceneSlot &S) {
 int CheckDiffSlot(pg3g_context *ctx, uint32_t slot, SceneSlot *&S) {
     S = GetSlot(ctx, slot);
     if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
-    if (S->view.algorithm != PG3G_ALG_PATH_TRACING || S->view.maxPL == 0)
-        return Fail(PG3G_ERR_UNSUPPORTED, "differentiable rendering needs the path tracer (pt) with -maxpl > 0");
+    if (S->view.algorithm != PG3G_ALG_PATH_TRACING || S->view.maxPL == 0 || S->view.splitBudget != 1.f ||
+        S->view.iic > 0.f)
+        return Fail(PG3G_ERR_UNSUPPORTED, "differentiable rendering needs the path tracer (pt) with -maxpl > 0 -sb 1 -iic 0");
     return PG3G_OK;
 }
 
@@ -925,8 +932,8 @@ int pg3g_batch_create(pg3g_context *ctx, uint32_t num_scenes, const uint32_t *sl
         SceneSlot *S = GetSlot(ctx, slots[i]);
         if (!S) return Fail(PG3G_ERR_NO_SCENE, "batch names an empty slot");
         if (spp[i] == 0) return Fail(PG3G_ERR_INVALID_ARG, "spp must be >= 1");
-        if (S->view.algorithm != PG3G_ALG_PATH_TRACING)
-            return Fail(PG3G_ERR_UNSUPPORTED, "batched rendering supports the path tracer (pt) only");
+        if (S->view.algorithm != PG3G_ALG_PATH_TRACING || S->view.splitBudget != 1.f || S->view.iic > 0.f)
+            return Fail(PG3G_ERR_UNSUPPORTED, "batched rendering supports the path tracer (pt) with -sb 1 -iic 0");
         for (uint32_t j = 0; j < i; ++j)
             if (slots[j] == slots[i]) return Fail(PG3G_ERR_INVALID_ARG, "a slot appears twice in the batch");
         const LightSpec ls = SceneLightSpec(S->view);

@@ -92,6 +92,51 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     }
 }
 
+// Path tracer with splitting (-sb > 1) or indirect clipping (-iic > 0): one thread per (pixel,
+// iteration chunk), each sample traced by RadiancePTTree (its vertex stack lives in scratch).
+__global__ void __launch_bounds__(256) RenderTreeKernel(SceneView S, uint32_t firstIter, uint32_t nIter,
+                                                         uint32_t itersPerThread, uint32_t numChunks,
+                                                         double *__restrict__ fb,
+                                                         unsigned long long *__restrict__ counters) {
+    const LightSpec spec = SceneLightSpec(S);
+    const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
+    const uint32_t numTiles = tilesX * tilesY;
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = (uint32_t)(gid & 63u);
+    const uint64_t waveId = gid >> 6;
+    const uint32_t tile = (uint32_t)(waveId % numTiles);
+    const uint32_t chunk = (uint32_t)(waveId / numTiles);
+    const uint32_t x = (tile % tilesX) * 8u + (lane & 7u);
+    const uint32_t y = (tile / tilesX) * 8u + (lane >> 3);
+    uint32_t segments = 0, samples = 0;
+    if (chunk < numChunks && x < S.resX && y < S.resY) {
+        const uint32_t pixel = y * S.resX + x;
+        const uint32_t itBegin = firstIter + chunk * itersPerThread;
+        const uint32_t itEnd = min(firstIter + nIter, itBegin + itersPerThread);
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        for (uint32_t it = itBegin; it < itEnd; ++it) {
+            RngKey key;
+            F3 org, dir;
+            CameraSample(S, pixel, it, key, org, dir);
+            const F3 L = RadiancePTTree(S, spec, key, org, dir, segments);
+            acc0 += L.x;
+            acc1 += L.y;
+            acc2 += L.z;
+            ++samples;
+        }
+        if (itEnd > itBegin) {
+            atomicAdd(fb + (size_t)pixel * 3 + 0, acc0);
+            atomicAdd(fb + (size_t)pixel * 3 + 1, acc1);
+            atomicAdd(fb + (size_t)pixel * 3 + 2, acc2);
+        }
+    }
+    const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
+    if (lane == 0 && counters) {
+        atomicAdd(counters + 0, (unsigned long long)segSum);
+        atomicAdd(counters + 1, (unsigned long long)smpSum);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Function-level kernels (tier-1 parity of BSDF and EM functions)
 __global__ void BsdfEvalKernel(DevMaterial m, uint32_t n, const float *__restrict__ wil, const float *__restrict__ wol,
@@ -283,8 +328,12 @@ hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t 
     const uint32_t tiles = ((S.resX + 7u) / 8u) * ((S.resY + 7u) / 8u);
     const uint64_t threads = (uint64_t)tiles * 64u * numChunks;
     const uint32_t blocks = (uint32_t)((threads + 255) / 256);
-    const PTKernelFn pt = (S.algorithm == 7) ? PickPTKernel(S) : nullptr;
+    const bool chain = (S.algorithm == 7) && S.splitBudget == 1.f && !(S.iic > 0.f);   // -sb 1 -iic 0
+    const PTKernelFn pt = chain ? PickPTKernel(S) : nullptr;
     if (pt) hipLaunchKernelGGL(pt, dim3(blocks), dim3(256), 0, stream, Sdev, firstIter, nIter, ipt, numChunks, fb, counters);
+    else if (S.algorithm == 7)
+        hipLaunchKernelGGL(RenderTreeKernel, dim3(blocks), dim3(256), 0, stream, S, firstIter, nIter, ipt, numChunks, fb,
+                           counters);
     else hipLaunchKernelGGL(RenderMegakernel, dim3(blocks), dim3(256), 0, stream, S, firstIter, nIter, ipt, numChunks, fb,
                             counters);
     return hipGetLastError();

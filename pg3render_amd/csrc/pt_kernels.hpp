@@ -405,6 +405,221 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
     return Le + Lo;
 }
 
+// ---------------------------------------------------------------------------------------------
+// EstimateIncomingRadiancePT with splitting (-sb > 1) and indirect clipping (-iic > 0): the
+// recursion of path_tracer.hxx:167-430 run as a post-order walk over an explicit stack of
+// vertices, since clipping needs each child's reflected estimate before it is weighted.  Paths
+// are cut at kTreeDepth vertices (the reference cuts at 1200, kMaxPathLengthHard); with
+// -maxpl <= kTreeDepth the two agree exactly.
+constexpr uint32_t kTreeDepth = 48;
+
+// MISWeight2Power with sample counts, path_tracer_base.hxx:522-541
+PG3_DEV float MisPowerN(float p1, uint32_t n1, float p2, uint32_t n2) {
+    const float s1 = n1 * p1, s2 = n2 * p2;
+    const float q1 = s1 * s1, q2 = s2 * s2;
+    return q1 / (q1 + q2) / n1;
+}
+// Vec3Base::ClipProportionally, math.hxx:538-549
+PG3_DEV F3 ClipProportionally(F3 v, float maxAllowed) {
+    const float m = MaxC(v);
+    if (m > maxAllowed) {
+        const float s = maxAllowed / m;
+        v.x *= s;
+        v.y *= s;
+        v.z *= s;
+    }
+    return v;
+}
+
+struct TreeFrame {              // one vertex of the recursion, in its BSDF-sample loop
+    F3 sp, n, wol;              // position, shading normal, outgoing direction (local)
+    F3 refl;                    // oReflectedRadianceEstimate being accumulated
+    F3 em;                      // emission found by the ray that reached this vertex (for the parent)
+    F3 att;                     // record of the BSDF sample whose subtree is being traced
+    float cosAbs, pdfW, comp;
+    float nextBudget, rr, lcSum, lpdf;
+    int32_t mat, lid;
+    uint32_t len, branch, nB, nL, s, flags;   // flags: 1 cut (RR), 2 finite, sides << 4
+};
+
+PG3_DEV F3 RadiancePTTree(const SceneView &S, const LightSpec &spec, const RngKey &key0, F3 org, F3 dir,
+                          uint32_t &segments) {
+    TreeFrame stk[kTreeDepth];
+    int32_t top = -1;
+    // pending call EstimateIncomingRadiancePT(ray, len, computeRefl, budget) on branch `cBranch`
+    F3 cOrg = org, cDir = dir;
+    float cTmin = 0.f, cBudget = S.splitBudget;
+    uint32_t cLen = 1, cBranch = 0;
+    bool cRefl = true;
+    // values returned by the last finished call
+    F3 rEm = mk(0.f), rRefl = mk(0.f);
+    float rPdf = 0.f;
+    int32_t rLid = -1;
+    enum { kCall, kNext, kReturn };
+    int mode = kCall;
+    for (;;) {
+        if (mode == kCall) {
+            RngKey key = key0;
+            key.c3 = (key0.c3 & 0xFFFF0000u) | cBranch;
+            rEm = mk(0.f);
+            rRefl = mk(0.f);
+            rPdf = 0.f;
+            rLid = -1;
+            mode = kReturn;
+            Hit h;
+            h.dist = 1e36f;
+            if (Intersect(S, cOrg, cDir, cTmin, h)) {
+                const F3 sp = cOrg + cDir * h.dist;
+                const Frame fr = FrameFromZ(h.n);
+                const F3 wol = ToLocal(fr, -cDir);
+                const DevMaterial mat = LoadGlobal(S.mats + h.mat);
+                if (mat.lightId >= 0 && cLen >= S.minPL) {                            // :198-210
+                    const DevLight Lt = LoadGlobal(S.lights + mat.lightId);
+                    if (Lt.type == kLightArea) {
+                        if (top >= 0) rPdf = AreaEmissionPdf(Lt, sp, cOrg);
+                        if (wol.z > 0.f) rEm = Lt.radiance;
+                    } else if (Lt.type == kLightPoint) {
+                        rPdf = kInf;
+                    }
+                    rLid = mat.lightId;
+                }
+                const bool stop = !cRefl || ReflectanceZero(mat) || (S.maxPL > 0 && cLen >= S.maxPL) ||
+                                  cLen >= 1200u || top + 1 >= (int32_t)kTreeDepth;
+                if (!stop) {
+                    ++segments;
+                    TreeFrame &F = stk[top + 1];
+                    // ComputeSplittingCounts, :432-454
+                    const uint32_t nB = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * cBudget);
+                    const uint32_t nL = (uint32_t)lroundf(fmaxf(S.lightRatio * nB, 1.f));
+                    F.sp = sp;
+                    F.n = fr.z;
+                    F.wol = wol;
+                    F.refl = mk(0.f);
+                    F.em = rEm;
+                    F.lpdf = rPdf;
+                    F.lid = rLid;
+                    F.mat = h.mat;
+                    F.len = cLen;
+                    F.branch = cBranch;
+                    F.nB = nB;
+                    F.nL = nL;
+                    F.s = 0;
+                    F.flags = mat.sides << 4;
+                    F.nextBudget = 1 + ((cBudget - nB) / (float)nB);
+                    LightCache lc;
+                    lc.sum = 0.f;
+                    if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, key, cLen, lc);
+                    F.lcSum = lc.sum;
+                    if ((cLen + 1) >= S.minPL) {                                      // :252-265
+                        for (uint32_t l = 0; l < nL; ++l) {
+                            float prob = 1.f;
+                            const int32_t li = PickLight(spec, lc, RngGet(key, cLen, l, kDimLightPick), prob);
+                            LightSample ls;
+                            if (li >= 0 &&
+                                SampleIllumination(S, spec, (uint32_t)li, sp, fr, mat.sides, RngGet(key, cLen, l, kDimLight0),
+                                                   RngGet(key, cLen, l, kDimLight1), RngGet(key, cLen, l, kDimLight2), ls)) {
+                                ls.prob = prob;                                       // :350-425
+                                if (MaxC(ls.sample) > 0.f && !Occluded(S, sp, ls.wig, ls.dist)) {
+                                    BsdfRec r;
+                                    EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
+                                    if (ls.pdfW != kInf) {
+                                        const float bsdfPdf = r.pdfW * r.compProb;
+                                        const float lightPdf = ls.pdfW * ls.prob;
+                                        F.refl += (ls.sample * r.att * MisPowerN(lightPdf, nL, bsdfPdf, nB)) / lightPdf;
+                                    } else {
+                                        F.refl += (ls.sample * r.att) / (ls.prob * nL);
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    F.rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
+                    ++top;
+                    mode = kNext;
+                }
+            } else if (cLen >= S.minPL && S.bgLight >= 0) {                           // :408-422
+                rEm = BgRadiance(S, spec, cDir);
+                if (top >= 0) rPdf = BgPdf(S, spec, cDir, FrameFromZ(stk[top].n), stk[top].flags >> 4);
+                rLid = S.bgLight;
+            }
+        } else if (mode == kNext) {                                                   // :281-404
+            TreeFrame &F = stk[top];
+            mode = kReturn;
+            RngKey key = key0;
+            key.c3 = (key0.c3 & 0xFFFF0000u) | F.branch;
+            const DevMaterial mat = LoadGlobal(S.mats + F.mat);
+            while (F.s < F.nB) {
+                const uint32_t s = F.s;
+                const bool cut = (S.maxPL == 0) && (RngGet(key, F.len, s, kDimRR) > F.rr);
+                BsdfRec r;
+                SampleBsdf(mat, F.wol, RngGet(key, F.len, s, kDimBsdf0), RngGet(key, F.len, s, kDimBsdf1),
+                           RngGet(key, F.len, s, kDimBsdf2), r);
+                if (IsBlocker(r)) {
+                    ++F.s;
+                    continue;
+                }
+                const bool finite = (r.pdfW != kInf);
+                F.att = r.att;
+                F.cosAbs = fabsf(r.wil.z);
+                F.pdfW = r.pdfW;
+                F.comp = r.compProb;
+                F.flags = (F.flags & ~3u) | (cut ? 1u : 0u) | (finite ? 2u : 0u);
+                cOrg = F.sp;
+                cDir = ToWorld(FrameFromZ(F.n), r.wil);
+                cTmin = EpsRayCos(F.cosAbs);
+                cLen = F.len + 1;
+                cRefl = !cut;
+                cBudget = F.nextBudget;
+                cBranch = (F.nB > 1) ? ((F.branch * 7u + s + 1u) & 0xFFFFu) : F.branch;
+                mode = kCall;
+                break;
+            }
+            if (mode == kReturn) {            // every BSDF sample done: this vertex returns
+                rEm = F.em;
+                rRefl = F.refl;
+                rPdf = F.lpdf;
+                rLid = F.lid;
+                --top;
+            }
+        } else {                                                                      // kReturn
+            if (top < 0) return rEm + rRefl;
+            TreeFrame &P = stk[top];
+            const bool finite = P.flags & 2u, cut = P.flags & 1u;
+            if (!IsZero(rEm) && rLid >= 0) {
+                if (finite) {
+                    float pick = 1.f;
+                    if (spec.numLights == 0) {
+                        pick = 0.f;
+                    } else if (spec.numLights > 1) {
+                        if (P.lcSum > 0.f) {
+                            RngKey key = key0;
+                            key.c3 = (key0.c3 & 0xFFFF0000u) | P.branch;
+                            pick = EstimateContribution(S, spec, (uint32_t)rLid, P.sp, FrameFromZ(P.n), P.flags >> 4,
+                                                        key, P.len) / P.lcSum;
+                        } else {
+                            pick = 1.f / spec.numLights;
+                        }
+                    }
+                    const float lp = rPdf * pick;
+                    const float bp = P.pdfW * P.comp;
+                    P.refl += (P.att * P.cosAbs * rEm * MisPowerN(bp, P.nB, lp, P.nL)) / bp;
+                } else {
+                    P.refl += (P.att * rEm) / (static_cast<float>(P.nB) * P.comp);
+                }
+            }
+            if (!cut && !IsZero(rRefl)) {
+                F3 ind;
+                if (finite) ind = (P.att * P.cosAbs * rRefl) / (P.pdfW * P.nB * P.rr * P.comp);
+                else ind = (P.att * rRefl) / (P.nB * P.rr * P.comp);
+                if (S.iic > 0.f) ind = ClipProportionally(ind, S.iic);
+                P.refl += ind;
+            }
+            ++P.s;
+            mode = kNext;
+        }
+    }
+}
+
 // Camera sample `iteration` of `pixel`: jitter, primary ray (PathTracerBase::RunIteration,
 // path_tracer_base.hxx:66-80) and the RNG key of its path.
 PG3_DEV void CameraSample(const SceneView &S, uint32_t pixel, uint32_t iteration, RngKey &key, F3 &org, F3 &dir) {

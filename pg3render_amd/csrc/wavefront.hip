@@ -294,7 +294,7 @@ void DestroyWavefrontContext(WavefrontContext *w) { delete w; }
 hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
                            unsigned long long *counters, WavefrontContext *W, hipStream_t stream) {
     if (!W) return hipErrorInvalidValue;
-    if (S.algorithm != 7 || S.maxPL == 0) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, stream);
+    if (S.algorithm != 7 || S.maxPL == 0 || S.splitBudget != 1.f || S.iic > 0.f) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, stream);
     if (nIter == 0) return hipSuccess;
     if (W->gridBlocks == 0) {
         int dev = 0, cus = 256;

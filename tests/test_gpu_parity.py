@@ -279,7 +279,59 @@ def test_iterations_accumulate_like_run_iteration(ctx):
 
 def test_unsupported_parameters_are_rejected(ctx):
     sb = SceneBuild.workload(1, seed=1, res=16)
-    sb.set_params(splitting_budget=4.0)
+    sb.set_params(splitting_budget=100.0)
     with pytest.raises(_ffi.Pg3gError) as e:
         ctx.upload(sb)
     assert e.value.code == _ffi.PG3G_ERR_UNSUPPORTED
+    sb.set_params(splitting_budget=0.5)
+    with pytest.raises(_ffi.Pg3gError) as e:
+        ctx.upload(sb)
+    assert e.value.code == _ffi.PG3G_ERR_UNSUPPORTED
+
+
+# Splitting (-sb) and indirect clipping (-iic), path_tracer.hxx:167-454: the GPU walks the same
+# recursion (explicit vertex stack) with the same per-branch RNG streams as the oracle.
+SPLIT_CASES = [
+    pytest.param(dict(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=8), id="sb4_iic16_maxpl8"),
+    pytest.param(dict(splitting_budget=4.0, indirect_clipping=0.0, max_path_length=0), id="sb4_rr"),
+    pytest.param(dict(splitting_budget=1.0, indirect_clipping=0.5, max_path_length=8), id="sb1_iic0.5"),
+    pytest.param(dict(splitting_budget=16.0, splitting_level=0.5, light_to_bsdf_ratio=2.0, indirect_clipping=4.0,
+                      max_path_length=6), id="sb16_level0.5_ratio2"),
+    pytest.param(dict(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0), id="defaults_sb4_iic16_rr"),
+]
+
+
+@pytest.mark.parametrize("params", SPLIT_CASES)
+def test_splitting_and_clipping_parity(ctx, params):
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(1, seed=31, res=32)
+    sb.set_params(**params)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+
+
+def test_splitting_on_reference_scene_with_em(ctx):
+    """Reference scene 2 (box, spheres, EM background) with the reference's defaults -sb 4 -iic 16
+    and Russian roulette."""
+    ctx.set_kernel_mode(_ffi.KERNEL_WAVEFRONT)   # falls back to the tree kernel for these options
+    sb = SceneBuild.reference(2, resolution=32)
+    sb.set_params(algorithm="pt", splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+
+
+def test_splitting_with_iic0_sb1_equals_chain_kernel(ctx):
+    """-sb 1 -iic 0 takes the chain kernel; the tree walker reduces to it (same samples): render the
+    chain kernel and the tree kernel (forced with a clipping threshold no radiance reaches)."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(1, seed=33, res=32)
+    ctx.upload(sb)
+    ctx.render(0, 4)
+    a = ctx.framebuffer(0, sb.resolution)[0]
+    sb.set_params(indirect_clipping=1e30)
+    ctx.upload(sb)
+    ctx.render(0, 4)
+    b = ctx.framebuffer(0, sb.resolution)[0]
+    rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-6)
+    assert np.mean(rel > 1e-5) < 0.02
+    assert abs(a.sum() - b.sum()) / a.sum() < 1e-4
