@@ -383,43 +383,59 @@ PG3_DEV float BasePdf(const DevEm &E, F3 dir, const SH9 &c) {                   
     const float whole = WholeIntegral(E, c);
     return IsTiny(whole) ? 0.f : (lum * cc) / whole;
 }
+// Inner levels of the steerable tree staged in LDS by a kernel: levels >= minLevel (down from the
+// root) are read from `nodes` (node index - baseNode) instead of E.nodeW.
+struct TreeCache {
+    const float *nodes = nullptr;
+    uint32_t minLevel = 0xFFFFFFFFu;
+    uint32_t baseNode = 0;
+};
+// One level of PickTriangle: children [first, first + cc) with weights at w (12 floats per node).
+template <class P>
+PG3_DEV uint32_t PickChild(P w, uint32_t cc, const SH9 &c, float &u) {
+    float cs[7];
+    cs[0] = 0.f;
+#pragma unroll
+    for (uint32_t i = 0; i < 6; ++i) {
+        const float ci = (i < cc) ? Dot9(w + i * 12u, c) : 0.f;
+        cs[i + 1] = cs[i] + ci;
+    }
+    float whole = cs[0];
+#pragma unroll
+    for (uint32_t i = 1; i <= 6; ++i) if (i == cc) whole = cs[i];
+    if (whole > 1E-10f) {
+        const float inv = 1.f / whole;
+#pragma unroll
+        for (uint32_t i = 1; i <= 6; ++i) cs[i] *= inv;
+    }
+    const float ut = u * 0.999999f;
+    // upper_bound over cs[0..cc]; past-the-end clamps to the last element
+    uint32_t ub = cc;
+#pragma unroll
+    for (int32_t i = 6; i >= 0; --i) if ((uint32_t)i <= cc && cs[i] > ut) ub = (uint32_t)i;
+    if (ub == 0) ub = 1;   // cs[0] = 0 <= ut, unreachable; kept for safety
+    float low = 0.f, high = 0.f;
+#pragma unroll
+    for (uint32_t i = 0; i <= 6; ++i) {
+        if (i == ub - 1u) low = cs[i];
+        if (i == ub) high = cs[i];
+    }
+    const float nu = (ut - low) / fmaxf(high - low, 0.00001f);
+    u = fminf(nu, 1.f);
+    return ub - 1u;
+}
 // PickTriangle, :3798-3875.  Level l node i has children [6i, min(6i+6, count(l-1))) on level l-1.
-PG3_DEV uint32_t PickLeaf(const DevEm &E, const SH9 &c, float &u) {
+PG3_DEV uint32_t PickLeaf(const DevEm &E, const SH9 &c, float &u, const TreeCache &tc = TreeCache{}) {
     uint32_t node = 0;
     for (int32_t lvl = (int32_t)E.numLevels - 1; lvl > 0; --lvl) {
         const uint32_t first = node * 6u;
         const uint32_t cc = min(6u, E.levelCount[lvl - 1] - first);
-        const GlobalPtr<float> w = AsGlobal(E.nodeW + (size_t)(E.levelBase[lvl - 1] + first) * 12u);
-        float cs[7];
-        cs[0] = 0.f;
-#pragma unroll
-        for (uint32_t i = 0; i < 6; ++i) {
-            const float ci = (i < cc) ? Dot9(w + i * 12u, c) : 0.f;
-            cs[i + 1] = cs[i] + ci;
-        }
-        float whole = cs[0];
-#pragma unroll
-        for (uint32_t i = 1; i <= 6; ++i) if (i == cc) whole = cs[i];
-        if (whole > 1E-10f) {
-            const float inv = 1.f / whole;
-#pragma unroll
-            for (uint32_t i = 1; i <= 6; ++i) cs[i] *= inv;
-        }
-        const float ut = u * 0.999999f;
-        // upper_bound over cs[0..cc]; past-the-end clamps to the last element
-        uint32_t ub = cc;
-#pragma unroll
-        for (int32_t i = 6; i >= 0; --i) if ((uint32_t)i <= cc && cs[i] > ut) ub = (uint32_t)i;
-        if (ub == 0) ub = 1;   // cs[0] = 0 <= ut, unreachable; kept for safety
-        const uint32_t child = ub - 1u;
-        float low = 0.f, high = 0.f;
-#pragma unroll
-        for (uint32_t i = 0; i <= 6; ++i) {
-            if (i == ub - 1u) low = cs[i];
-            if (i == ub) high = cs[i];
-        }
-        const float nu = (ut - low) / fmaxf(high - low, 0.00001f);
-        u = fminf(nu, 1.f);
+        const uint32_t idx = E.levelBase[lvl - 1] + first;
+        uint32_t child;
+        if ((uint32_t)(lvl - 1) >= tc.minLevel)     // wave-uniform: the level is a loop counter
+            child = PickChild(tc.nodes + (size_t)(idx - tc.baseNode) * 12u, cc, c, u);
+        else
+            child = PickChild(AsGlobal(E.nodeW + (size_t)idx * 12u), cc, c, u);
         node = first + child;
     }
     return node;
@@ -450,9 +466,9 @@ PG3_DEV void SampleTriBilinear(float ux, float uy, float va, float vb, float vc,
 }
 // SampleHemisphereImpl, :2162-2217
 PG3_DEV bool SteerSampleHemisphere(const DevEm &E, F3 n, const SH9 &c, float u0, float u1, F3 &dir,
-                                   float &pdf, F3 &radCos) {
+                                   float &pdf, F3 &radCos, const TreeCache &tc = TreeCache{}) {
     float ux = u0;
-    const uint32_t leaf = PickLeaf(E, c, ux);
+    const uint32_t leaf = PickLeaf(E, c, ux, tc);
     const GlobalPtr<float> lv = AsGlobal(E.leafVerts + (size_t)leaf * 36u);
     const float v0 = Dot9(lv + 3, c), v1 = Dot9(lv + 15, c), v2 = Dot9(lv + 27, c);
     float b0, b1;
@@ -470,26 +486,26 @@ PG3_DEV bool SteerSampleHemisphere(const DevEm &E, F3 n, const SH9 &c, float u0,
     radCos = EmEval(E, dir) * cs;
     return true;
 }
-// SampleImpl, :2084-2157
+// SampleImpl, :2084-2157.  One call site of the hemisphere sampler: a lane's hemisphere (normal
+// and its clamped-cosine coefficients) is selected first, so lanes of a wavefront that pick
+// different hemispheres run one tree descent, not two.
 PG3_DEV bool SteerSample(const DevEm &E, F3 n, uint32_t sides, float uSide, float u0, float u1, F3 &dir,
-                         float &pdf, F3 &radCos) {
+                         float &pdf, F3 &radCos, const TreeCache &tc = TreeCache{}) {
     const bool front = sides & kSideFront, back = sides & kSideBack;
+    if (!front && !back) return false;
+    float ui = 0.f, li = 0.f, wi = 0.f;
+    bool up = front;
     if (front && back) {
-        const SH9 uc = ClampedCos(n), lc = ClampedCos(-n);
-        const float ui = WholeIntegral(E, uc), li = WholeIntegral(E, lc), wi = ui + li;
+        ui = WholeIntegral(E, ClampedCos(n));
+        li = WholeIntegral(E, ClampedCos(-n));
+        wi = ui + li;
         if (IsTiny(wi)) return false;
-        if (uSide * wi < ui) {
-            SteerSampleHemisphere(E, n, uc, u0, u1, dir, pdf, radCos);
-            pdf *= ui / wi;
-        } else {
-            SteerSampleHemisphere(E, -n, lc, u0, u1, dir, pdf, radCos);
-            pdf *= li / wi;
-        }
-        return true;
+        up = uSide * wi < ui;
     }
-    if (front) return SteerSampleHemisphere(E, n, ClampedCos(n), u0, u1, dir, pdf, radCos);
-    if (back) return SteerSampleHemisphere(E, -n, ClampedCos(-n), u0, u1, dir, pdf, radCos);
-    return false;
+    const F3 hn = up ? n : -n;
+    SteerSampleHemisphere(E, hn, ClampedCos(hn), u0, u1, dir, pdf, radCos, tc);
+    if (front && back) pdf *= (up ? ui : li) / wi;
+    return true;
 }
 PG3_DEV float SteerPdfHemisphere(const DevEm &E, F3 dir, F3 n, const SH9 &c) {       // :2281-2299
     if (Dot(dir, n) < 0.f) return 0.f;
@@ -497,16 +513,19 @@ PG3_DEV float SteerPdfHemisphere(const DevEm &E, F3 dir, F3 n, const SH9 &c) {  
 }
 PG3_DEV float SteerPdf(const DevEm &E, F3 dir, F3 n, uint32_t sides) {               // :2220-2278
     const bool front = sides & kSideFront, back = sides & kSideBack;
+    if (!front && !back) return 0.f;
+    float ui = 0.f, li = 0.f, wi = 0.f;
+    bool up = front;
     if (front && back) {
-        const SH9 uc = ClampedCos(n), lc = ClampedCos(-n);
-        const float ui = WholeIntegral(E, uc), li = WholeIntegral(E, lc), wi = ui + li;
+        ui = WholeIntegral(E, ClampedCos(n));
+        li = WholeIntegral(E, ClampedCos(-n));
+        wi = ui + li;
         if (IsTiny(wi)) return 0.f;
-        if (Dot(dir, n) >= 0.f) return SteerPdfHemisphere(E, dir, n, uc) * (ui / wi);
-        return SteerPdfHemisphere(E, dir, -n, lc) * (li / wi);
+        up = Dot(dir, n) >= 0.f;
     }
-    if (front) return SteerPdfHemisphere(E, dir, n, ClampedCos(n));
-    if (back) return SteerPdfHemisphere(E, dir, -n, ClampedCos(-n));
-    return 0.f;
+    const F3 hn = up ? n : -n;
+    const float p = SteerPdfHemisphere(E, dir, hn, ClampedCos(hn));
+    return (front && back) ? p * ((up ? ui : li) / wi) : p;
 }
 PG3_DEV float SteerIrradiance(const DevEm &E, F3 n, uint32_t sides) {               // :2304-2358
     const float ui = WholeIntegral(E, ClampedCos(n)), li = WholeIntegral(E, ClampedCos(-n));
@@ -526,9 +545,9 @@ PG3_DEV float EmPdf(const DevEm &E, int kind, F3 dir, const Frame &fr, uint32_t 
 }
 // EnvironmentMapSampler::Sample of the active sampler (em_sampler.hxx:49-64)
 PG3_DEV bool EmSample(const DevEm &E, int kind, const Frame &fr, uint32_t sides, float u0, float u1, float u2, F3 &dir,
-                      float &pdf, F3 &radCos) {
+                      float &pdf, F3 &radCos, const TreeCache &tc = TreeCache{}) {
     if (kind == kEmSimple) { SimpleSample(E, fr, sides, u0, u1, dir, pdf, radCos); return true; }
-    if (kind == kEmSteerable) return SteerSample(E, fr.z, sides, u2, u0, u1, dir, pdf, radCos);
+    if (kind == kEmSteerable) return SteerSample(E, fr.z, sides, u2, u0, u1, dir, pdf, radCos, tc);
     const F3 w = SampleCosSphereParamPdfW(u0, u1, u2, sides & kSideFront, sides & kSideBack, pdf);
     dir = ToWorld(fr, w);
     radCos = EmEval(E, dir) * fabsf(w.z);
@@ -575,6 +594,7 @@ PG3_DEV void PointCompute(const DevLight &L, F3 sp, F3 n, uint32_t sides, LightS
 struct LightSpec {
     int em;
     uint32_t numLights;
+    TreeCache tree;      // steerable tree levels staged in LDS by the kernel, if any
 };
 PG3_HD LightSpec SceneLightSpec(const SceneView &S) {
     return LightSpec{S.em.w > 0 ? S.em.sampler : (int)kEmAbsent, S.numLights};
@@ -655,7 +675,7 @@ PG3_DEV bool SampleIllumination(const SceneView &S, const LightSpec &spec, uint3
     if (L.type == kLightPoint) { PointCompute(L, sp, fr.z, sides, o); return true; }
     o.prob = 1.0f;
     o.dist = 3.402823466e+38f;   // em_sampler.hxx:57
-    if (spec.em != kEmAbsent) return EmSample(S.em, spec.em, fr, sides, u0, u1, u2, o.wig, o.pdfW, o.sample);
+    if (spec.em != kEmAbsent) return EmSample(S.em, spec.em, fr, sides, u0, u1, u2, o.wig, o.pdfW, o.sample, spec.tree);
     const F3 w = SampleCosSphereParamPdfW(u0, u1, u2, sides & kSideFront, sides & kSideBack, o.pdfW);  // em_cosine_sampler.hxx:12-33
     o.wig = ToWorld(fr, w);
     o.sample = L.radiance * fabsf(w.z);

@@ -90,6 +90,7 @@ struct pg3g_context {
     hipStream_t stream = nullptr;
     std::vector<std::unique_ptr<SceneSlot>> slots;
     DevBuf counters;   // [segments, camera samples]
+    DevBuf work;       // item counter of the persistent path-tracing kernel
     int kernelMode = PG3G_KERNEL_WAVEFRONT;
     DevBuf scratch[8];
     WavefrontContext *wavefront = nullptr;
@@ -637,6 +638,7 @@ int pg3g_create(pg3g_context **ctx, int device) {
     PG3_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     PG3_HIP(hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device));
     PG3_HIP(c->counters.Alloc(sizeof(unsigned long long) * 2));
+    PG3_HIP(c->work.Alloc(sizeof(uint32_t)));
     PG3_HIP(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned long long) * 2, c->stream));
     PG3_HIP(hipStreamSynchronize(c->stream));
     *ctx = c.release();
@@ -651,6 +653,7 @@ int pg3g_destroy(pg3g_context *ctx) {
     DestroyWavefrontContext(ctx->wavefront);
     ctx->wavefront = nullptr;
     ctx->counters.Free();
+    ctx->work.Free();
     for (auto &b : ctx->scratch) b.Free();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -693,9 +696,11 @@ int pg3g_render_slot(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
         if (!ctx->wavefront) ctx->wavefront = CreateWavefrontContext();
         if (!ctx->wavefront) return Fail(PG3G_ERR_OUT_OF_MEMORY, "host allocation failed");
         PG3_HIP(LaunchWavefront(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(),
-                                ctx->counters.as<unsigned long long>(), ctx->wavefront, ctx->stream));
+                                ctx->counters.as<unsigned long long>(), ctx->work.as<uint32_t>(), ctx->numCUs,
+                                ctx->wavefront, ctx->stream));
     } else {
-        PG3_HIP(LaunchMegakernel(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(), ctx->counters.as<unsigned long long>(),
+        PG3_HIP(LaunchMegakernel(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(),
+                                 ctx->counters.as<unsigned long long>(), ctx->work.as<uint32_t>(), ctx->numCUs,
                                  ctx->stream));
     }
     S->iterations += n_iter;

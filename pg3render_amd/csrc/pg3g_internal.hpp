@@ -14,13 +14,15 @@ namespace pg3 {
 
 // S: host copy of the scene view (launch geometry, generic kernel argument); Sdev: the same view
 // in device memory (read by the specialised path-tracer kernels).
+// work: one device word for the persistent kernels' item counter (reset by the launch).
 hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
-                            unsigned long long *counters, hipStream_t stream);
+                            unsigned long long *counters, uint32_t *work, int numCUs, hipStream_t stream);
 struct WavefrontContext;   // queue storage of the wavefront path tracer (one per pg3g_context)
 WavefrontContext *CreateWavefrontContext();
 void DestroyWavefrontContext(WavefrontContext *w);
 hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
-                           unsigned long long *counters, WavefrontContext *queues, hipStream_t stream);
+                           unsigned long long *counters, uint32_t *work, int numCUs, WavefrontContext *queues,
+                           hipStream_t stream);
 // Batched multi-scene pass (batch.hip).  Scene s owns global work items
 // [itemBegin, itemBegin + numTiles * ceil(spp / itersPerItem)); item = chunk * numTiles + tile.
 struct BatchScene {

@@ -63,28 +63,63 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
 #ifndef PG3_PT_WAVES_PER_EU
 #define PG3_PT_WAVES_PER_EU 4
 #endif
+// Persistent: each CU holds 16 wavefronts for the whole launch, and every wavefront takes work
+// items (8x8 tile x iteration chunk) from a device counter until none is left, so a wavefront
+// that finishes early starts the next item instead of waiting for its workgroup.  With the
+// steerable sampler the workgroup is 1024 lanes (one per CU) and stages the inner levels of the
+// EM's triangle tree in LDS once (kTreeLdsNodes nodes, 60 KiB): the tree descent of every light
+// sample then gathers from LDS instead of from the L2.
+constexpr uint32_t kTreeLdsNodes = 1280;
+template <int kEmKind>
+constexpr uint32_t PTBlock() { return kEmKind == kEmSteerable ? 1024u : 256u; }
+
 template <int kEmKind, bool kOneLight>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PG3_PT_WAVES_PER_EU)))
+__global__ void __launch_bounds__(PTBlock<kEmKind>()) __attribute__((amdgpu_waves_per_eu(PG3_PT_WAVES_PER_EU)))
 RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nIter, uint32_t itersPerThread,
-               uint32_t numChunks, double *__restrict__ fb, unsigned long long *__restrict__ counters) {
+               uint32_t numChunks, double *__restrict__ fb, uint32_t *__restrict__ work,
+               unsigned long long *__restrict__ counters) {
     // The scene is read from a device copy rather than from kernel arguments: held in SGPRs,
     // its ~450 bytes of fields spill into VGPR lanes of this register-heavy kernel.
     const SceneView &S = *Sp;
-    const LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
-    __shared__ double accLds[4][64 * 3];
+    LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
+    __shared__ double accLds[PTBlock<kEmKind>() / 64u][64 * 3];
+    if constexpr (kEmKind == kEmSteerable) {
+        __shared__ F4v treeLds[kTreeLdsNodes * 3];
+        // the levels from the root down that fit (contiguous at the end of nodeW)
+        const DevEm &E = S.em;
+        uint32_t minLevel = E.numLevels, count = 0;
+        for (int32_t l = (int32_t)E.numLevels - 1; l >= 1; --l) {
+            if (count + E.levelCount[l] > kTreeLdsNodes) break;
+            count += E.levelCount[l];
+            minLevel = (uint32_t)l;
+        }
+        if (minLevel < E.numLevels) {
+            const uint32_t base = E.levelBase[minLevel];
+            const GlobalPtr<F4v> src = (GlobalPtr<F4v>)AsGlobal(E.nodeW + (size_t)base * 12u);
+            for (uint32_t i = threadIdx.x; i < count * 3u; i += blockDim.x) treeLds[i] = src[i];
+            spec.tree.nodes = reinterpret_cast<const float *>(treeLds);
+            spec.tree.minLevel = minLevel;
+            spec.tree.baseNode = base;
+        }
+        __syncthreads();
+    }
     const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
     const uint32_t numTiles = tilesX * tilesY;
-    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t lane = (uint32_t)(gid & 63u);
-    const uint32_t waveId = (uint32_t)(gid >> 6);
-    const uint32_t tile = waveId % numTiles;
-    const uint32_t chunk = waveId / numTiles;
+    const uint32_t numItems = numTiles * numChunks;
+    const uint32_t lane = __lane_id();
     uint32_t segments = 0, samples = 0;
-    const uint32_t it = firstIter + chunk * itersPerThread;
-    const uint32_t itEnd = min(firstIter + nIter, it + itersPerThread);
-    if (chunk < numChunks && it < itEnd)
-        TraceTile(S, spec, (tile % tilesX) * 8u, (tile / tilesX) * 8u, it, itEnd, accLds[threadIdx.x >> 6], fb, segments,
-                  samples);
+    for (;;) {
+        uint32_t item = 0;
+        if (lane == 0) item = atomicAdd(work, 1u);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= numItems) break;   // every wave leaves once the counter passes the last item
+        const uint32_t tile = item % numTiles, chunk = item / numTiles;
+        const uint32_t it = firstIter + chunk * itersPerThread;
+        const uint32_t itEnd = min(firstIter + nIter, it + itersPerThread);
+        if (it < itEnd)
+            TraceTile(S, spec, (tile % tilesX) * 8u, (tile / tilesX) * 8u, it, itEnd, accLds[threadIdx.x >> 6], fb,
+                      segments, samples);
+    }
     const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
     if (lane == 0 && counters) {
         atomicAdd(counters + 0, (unsigned long long)segSum);
@@ -299,7 +334,12 @@ __global__ void __launch_bounds__(256) CdfMarginalKernel(uint32_t h, const float
 
 // ---------------------------------------------------------------------------------------------
 // Launchers (host)
-using PTKernelFn = void (*)(const SceneView *, uint32_t, uint32_t, uint32_t, uint32_t, double *, unsigned long long *);
+using PTKernelFn = void (*)(const SceneView *, uint32_t, uint32_t, uint32_t, uint32_t, double *, uint32_t *,
+                           unsigned long long *);
+
+__global__ void ResetWorkCounter(uint32_t *work) {
+    if (threadIdx.x == 0) work[0] = 0u;
+}
 
 // Specialised path-tracer kernel for a scene (`pt` only).
 static PTKernelFn PickPTKernel(const SceneView &S) {
@@ -316,7 +356,7 @@ static PTKernelFn PickPTKernel(const SceneView &S) {
 }
 
 hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
-                            unsigned long long *counters, hipStream_t stream) {
+                            unsigned long long *counters, uint32_t *work, int numCUs, hipStream_t stream) {
     if (nIter == 0) return hipSuccess;
     const uint32_t npix = S.resX * S.resY;
     // Enough lanes to fill 256 CUs several times over; each thread keeps >= 1 iteration.
@@ -330,7 +370,15 @@ hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t 
     const uint32_t blocks = (uint32_t)((threads + 255) / 256);
     const bool chain = (S.algorithm == 7) && S.splitBudget == 1.f && !(S.iic > 0.f);   // -sb 1 -iic 0
     const PTKernelFn pt = chain ? PickPTKernel(S) : nullptr;
-    if (pt) hipLaunchKernelGGL(pt, dim3(blocks), dim3(256), 0, stream, Sdev, firstIter, nIter, ipt, numChunks, fb, counters);
+    const uint32_t ptBlock = (SceneLightSpec(S).em == kEmSteerable) ? PTBlock<kEmSteerable>() : PTBlock<kEmSimple>();
+    if (pt) {
+        if (!work) return hipErrorInvalidValue;
+        // 16 resident wavefronts per CU (4 per SIMD, the kernel's register budget)
+        const uint32_t grid = (uint32_t)max(1, numCUs) * (1024u / ptBlock);
+        hipLaunchKernelGGL(ResetWorkCounter, dim3(1), dim3(64), 0, stream, work);
+        hipLaunchKernelGGL(pt, dim3(grid), dim3(ptBlock), 0, stream, Sdev, firstIter, nIter, ipt, numChunks, fb, work,
+                           counters);
+    }
     else if (S.algorithm == 7)
         hipLaunchKernelGGL(RenderTreeKernel, dim3(blocks), dim3(256), 0, stream, S, firstIter, nIter, ipt, numChunks, fb,
                            counters);

@@ -292,9 +292,10 @@ WavefrontContext *CreateWavefrontContext() { return new (std::nothrow) Wavefront
 void DestroyWavefrontContext(WavefrontContext *w) { delete w; }
 
 hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
-                           unsigned long long *counters, WavefrontContext *W, hipStream_t stream) {
+                           unsigned long long *counters, uint32_t *work, int numCUs, WavefrontContext *W,
+                           hipStream_t stream) {
     if (!W) return hipErrorInvalidValue;
-    if (S.algorithm != 7 || S.maxPL == 0 || S.splitBudget != 1.f || S.iic > 0.f) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, stream);
+    if (S.algorithm != 7 || S.maxPL == 0 || S.splitBudget != 1.f || S.iic > 0.f) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, work, numCUs, stream);
     if (nIter == 0) return hipSuccess;
     if (W->gridBlocks == 0) {
         int dev = 0, cus = 256;
@@ -310,7 +311,7 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
     if (e != hipSuccess) return e;
     const WfQueues &Q = W->st.q;
     const ShadeFn shade = PickShadeKernel(S);
-    if (!shade) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, stream);
+    if (!shade) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, work, numCUs, stream);
     const dim3 grid(W->gridBlocks), block(256);
     for (uint64_t base = 0; base < total; base += cap) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(cap, total - base);
