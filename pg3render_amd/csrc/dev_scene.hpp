@@ -377,11 +377,23 @@ PG3_DEV float ShDotCoeffs(F3 d, const SH9 &c) {               // Dot(GenerateSph
     return r;
 }
 PG3_DEV float WholeIntegral(const DevEm &E, const SH9 &c) { return Dot9(E.rootW, c); }  // :2416-2432
-PG3_DEV float BasePdf(const DevEm &E, F3 dir, const SH9 &c) {                          // :2387-2413
-    const float lum = Luminance(EmEval(E, dir));
+// GetBasePdf, :2387-2413, with the EM radiance at dir given
+PG3_DEV float BasePdfLe(const DevEm &E, F3 dir, F3 Le, const SH9 &c) {
+    const float lum = Luminance(Le);
     const float cc = ShDotCoeffs(dir, c);
     const float whole = WholeIntegral(E, c);
     return IsTiny(whole) ? 0.f : (lum * cc) / whole;
+}
+// EM radiance at a direction and at its antipode with one lat-long mapping: the antipode of
+// (u, v) is (u +- 1/2, 1 - v) (the steerable sampler's pdf and sample always need both).
+struct EmPair {
+    F3 at, opp;
+};
+PG3_DEV EmPair EmEvalPair(const DevEm &E, F3 dir) {
+    float u, v;
+    Dir2LatLong(dir, u, v);
+    const float uo = (u < 0.5f) ? u + 0.5f : u - 0.5f, vo = 1.f - v;
+    return EmPair{EmEvalUV(E, u, v), EmEvalUV(E, uo, vo)};
 }
 // Inner levels of the steerable tree staged in LDS by a kernel: levels >= minLevel (down from the
 // root) are read from `nodes` (node index - baseNode) instead of E.nodeW.
@@ -480,10 +492,12 @@ PG3_DEV bool SteerSampleHemisphere(const DevEm &E, F3 n, const SH9 &c, float u0,
     const float whole = WholeIntegral(E, c);
     pdf = IsTiny(whole) ? 0.f : val / whole;
     const F3 fl = -dir;
-    pdf += BasePdf(E, fl, c);
+    const EmPair le = EmEvalPair(E, dir);
+    pdf += BasePdfLe(E, fl, le.opp, c);
     float cs = Dot(dir, n);
-    if (cs < 0.f) { dir = fl; cs *= -1.f; }
-    radCos = EmEval(E, dir) * cs;
+    const bool flip = cs < 0.f;
+    if (flip) { dir = fl; cs *= -1.f; }
+    radCos = (flip ? le.opp : le.at) * cs;
     return true;
 }
 // SampleImpl, :2084-2157.  One call site of the hemisphere sampler: a lane's hemisphere (normal
@@ -507,11 +521,13 @@ PG3_DEV bool SteerSample(const DevEm &E, F3 n, uint32_t sides, float uSide, floa
     if (front && back) pdf *= (up ? ui : li) / wi;
     return true;
 }
-PG3_DEV float SteerPdfHemisphere(const DevEm &E, F3 dir, F3 n, const SH9 &c) {       // :2281-2299
+// pre (may be null): the EM radiance pair of dir, when the caller has it already
+PG3_DEV float SteerPdfHemisphere(const DevEm &E, F3 dir, F3 n, const SH9 &c, const EmPair *pre) {  // :2281-2299
     if (Dot(dir, n) < 0.f) return 0.f;
-    return BasePdf(E, dir, c) + BasePdf(E, -dir, c);
+    const EmPair le = pre ? *pre : EmEvalPair(E, dir);
+    return BasePdfLe(E, dir, le.at, c) + BasePdfLe(E, -dir, le.opp, c);
 }
-PG3_DEV float SteerPdf(const DevEm &E, F3 dir, F3 n, uint32_t sides) {               // :2220-2278
+PG3_DEV float SteerPdf(const DevEm &E, F3 dir, F3 n, uint32_t sides, const EmPair *pre = nullptr) {  // :2220-2278
     const bool front = sides & kSideFront, back = sides & kSideBack;
     if (!front && !back) return 0.f;
     float ui = 0.f, li = 0.f, wi = 0.f;
@@ -524,7 +540,7 @@ PG3_DEV float SteerPdf(const DevEm &E, F3 dir, F3 n, uint32_t sides) {          
         up = Dot(dir, n) >= 0.f;
     }
     const F3 hn = up ? n : -n;
-    const float p = SteerPdfHemisphere(E, dir, hn, ClampedCos(hn));
+    const float p = SteerPdfHemisphere(E, dir, hn, ClampedCos(hn), pre);
     return (front && back) ? p * ((up ? ui : li) / wi) : p;
 }
 PG3_DEV float SteerIrradiance(const DevEm &E, F3 n, uint32_t sides) {               // :2304-2358
@@ -538,9 +554,9 @@ PG3_DEV float SteerIrradiance(const DevEm &E, F3 n, uint32_t sides) {           
 
 // EnvironmentMap::PdfW of the active sampler (em.hxx:113-120)
 // `kind` is the sampler (E.sampler at run time, or a constant in specialised kernels).
-PG3_DEV float EmPdf(const DevEm &E, int kind, F3 dir, const Frame &fr, uint32_t sides) {
+PG3_DEV float EmPdf(const DevEm &E, int kind, F3 dir, const Frame &fr, uint32_t sides, const EmPair *pre = nullptr) {
     if (kind == kEmSimple) return SimplePdf(E, dir);
-    if (kind == kEmSteerable) return SteerPdf(E, dir, fr.z, sides);
+    if (kind == kEmSteerable) return SteerPdf(E, dir, fr.z, sides, pre);
     return CosSpherePdfW(sides & kSideFront, sides & kSideBack, ToLocal(fr, dir));
 }
 // EnvironmentMapSampler::Sample of the active sampler (em_sampler.hxx:49-64)
@@ -605,8 +621,9 @@ PG3_DEV F3 BgRadiance(const SceneView &S, const LightSpec &ls, F3 dir) {
     return (ls.em != kEmAbsent) ? EmEval(S.em, dir) : LoadGlobal(S.lights + S.bgLight).radiance;
 }
 // ... and its PDF w.r.t. a shading frame/material
-PG3_DEV float BgPdf(const SceneView &S, const LightSpec &ls, F3 dir, const Frame &fr, uint32_t sides) {
-    if (ls.em != kEmAbsent) return EmPdf(S.em, ls.em, dir, fr, sides);
+PG3_DEV float BgPdf(const SceneView &S, const LightSpec &ls, F3 dir, const Frame &fr, uint32_t sides,
+                    const EmPair *pre = nullptr) {
+    if (ls.em != kEmAbsent) return EmPdf(S.em, ls.em, dir, fr, sides, pre);
     return CosSpherePdfW(sides & kSideFront, sides & kSideBack, ToLocal(fr, dir));
 }
 // AreaLight::GetEmmision PDF part, lights.hxx:88-99

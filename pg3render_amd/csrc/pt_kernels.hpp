@@ -144,12 +144,16 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey
     const bool hasPrev = ps.flags & kHasPrev, prevFinite = ps.flags & kPrevFinite;
     if (!hit) {
         if (len >= S.minPL && S.bgLight >= 0) {                                       // :408-422
-            const F3 Le = BgRadiance(S, spec, ps.dir);
+            // the steerable pdf needs the EM at the direction and at its antipode: one mapping
+            const bool pair = (spec.em == kEmSteerable);
+            EmPair le;
+            if (pair) le = EmEvalPair(S.em, ps.dir);
+            const F3 Le = pair ? le.at : BgRadiance(S, spec, ps.dir);
             if (!hasPrev) {
                 ps.L += Le;
             } else if (!IsZero(Le)) {
                 if (prevFinite) {
-                    const float lp = BgPdf(S, spec, ps.dir, FrameFromZ(ps.prevN), ps.flags >> 4) *
+                    const float lp = BgPdf(S, spec, ps.dir, FrameFromZ(ps.prevN), ps.flags >> 4, pair ? &le : nullptr) *
                                      PrevPickProb(S, spec, key, ps, (uint32_t)S.bgLight);
                     ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
                 } else {
