@@ -42,6 +42,18 @@ PG3_HD F3 Cross(F3 a, F3 b) { return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x *
 PG3_HD float LenSqr(F3 a) { return Dot(a, a); }
 PG3_HD float Length(F3 a) { return sqrtf(Dot(a, a)); }
 PG3_HD F3 Normalize(F3 a) { return a / sqrtf(Dot(a, a)); }            // math.hxx:653-658
+// Division by the hardware reciprocal (<= 2 ulp; `__fdividef`).  Used where a one-ulp change only
+// moves a sample within its pdf (the steerable sampler's arithmetic, path throughput updates);
+// BSDF values and pdfs keep correctly rounded division, which their 1e-5 parity needs.
+#if defined(__HIP_DEVICE_COMPILE__)
+PG3_HD float FastDiv(float a, float b) { return __fdividef(a, b); }
+#else
+PG3_HD float FastDiv(float a, float b) { return a / b; }
+#endif
+PG3_HD F3 FastDiv(F3 a, float b) {
+    const float r = FastDiv(1.0f, b);
+    return F3{a.x * r, a.y * r, a.z * r};
+}
 PG3_HD float MaxC(F3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }        // math.hxx:535
 PG3_HD bool IsZero(F3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
 PG3_HD float Luminance(F3 a) { return 0.212671f * a.x + 0.715160f * a.y + 0.072169f * a.z; }  // spectrum.hxx:74-80

@@ -125,7 +125,8 @@ PG3_DEV bool SphHit(const DevSphere &s, F3 org, F3 dir, float tmin, Hit &h) {
     else return false;
     h.dist = resT;
     h.mat = s.mat;
-    h.n = Normalize(o + mk(resT) * dir);
+    const F3 nn = o + mk(resT) * dir;
+    h.n = FastDiv(nn, sqrtf(Dot(nn, nn)));
     return true;
 }
 // Primitive records are read through the constant address space: every lane of a wavefront
@@ -382,7 +383,7 @@ PG3_DEV float BasePdfLe(const DevEm &E, F3 dir, F3 Le, const SH9 &c) {
     const float lum = Luminance(Le);
     const float cc = ShDotCoeffs(dir, c);
     const float whole = WholeIntegral(E, c);
-    return IsTiny(whole) ? 0.f : (lum * cc) / whole;
+    return IsTiny(whole) ? 0.f : FastDiv(lum * cc, whole);
 }
 // EM radiance at a direction and at its antipode with one lat-long mapping: the antipode of
 // (u, v) is (u +- 1/2, 1 - v) (the steerable sampler's pdf and sample always need both).
@@ -416,7 +417,7 @@ PG3_DEV uint32_t PickChild(P w, uint32_t cc, const SH9 &c, float &u) {
 #pragma unroll
     for (uint32_t i = 1; i <= 6; ++i) if (i == cc) whole = cs[i];
     if (whole > 1E-10f) {
-        const float inv = 1.f / whole;
+        const float inv = FastDiv(1.f, whole);
 #pragma unroll
         for (uint32_t i = 1; i <= 6; ++i) cs[i] *= inv;
     }
@@ -432,7 +433,7 @@ PG3_DEV uint32_t PickChild(P w, uint32_t cc, const SH9 &c, float &u) {
         if (i == ub - 1u) low = cs[i];
         if (i == ub) high = cs[i];
     }
-    const float nu = (ut - low) / fmaxf(high - low, 0.00001f);
+    const float nu = FastDiv(ut - low, fmaxf(high - low, 0.00001f));
     u = fminf(nu, 1.f);
     return ub - 1u;
 }
@@ -454,9 +455,9 @@ PG3_DEV uint32_t PickLeaf(const DevEm &E, const SH9 &c, float &u, const TreeCach
 }
 // SampleTriangleFFunction / GFunction / Bilinear, :3709-3747, :3882-3905 (math.hxx:284-342)
 PG3_DEV void SampleTriBilinear(float ux, float uy, float va, float vb, float vc, float &b0, float &b1) {
-    const float x = (vb - va) / 3 + (vc - vb) / 6;
-    const float y = va / 2;
-    const float a = x / (x + y), b = y / (x + y);
+    const float x = FastDiv(vb - va, 3.f) + FastDiv(vc - vb, 6.f);
+    const float y = va * 0.5f;
+    const float a = FastDiv(x, x + y), b = FastDiv(y, x + y);
     float s = 0.5f;
     bool bail = false;
 #pragma unroll
@@ -465,14 +466,14 @@ PG3_DEV void SampleTriBilinear(float ux, float uy, float va, float vb, float vc,
         const float d = s * (s * 3.f * a + 2.f * b) + 0.f;
         if (!bail) {
             if (fabsf(d) < 0.0001f) { bail = true; }
-            else s = Clampf(s - f / d, 0.f, 1.f);
+            else s = Clampf(s - FastDiv(f, d), 0.f, 1.f);
         }
     }
     if (bail) s = 0.5f;
     const float t0 = (s * (vc - vb)) + (2 * (1 - s) * va) + (s * vb);
-    const float gamma = s * (vc - vb) / t0;
-    const float rho = 2 * ((1 - s) * va + s * vb) / t0;
-    const float t = (2 * uy) / (rho + SafeSqrt(rho * rho + 4 * gamma * uy));
+    const float gamma = FastDiv(s * (vc - vb), t0);
+    const float rho = FastDiv(2 * ((1 - s) * va + s * vb), t0);
+    const float t = FastDiv(2 * uy, rho + SafeSqrt(rho * rho + 4 * gamma * uy));
     b0 = 1 - s;
     b1 = s * (1 - t);
 }
@@ -488,9 +489,9 @@ PG3_DEV bool SteerSampleHemisphere(const DevEm &E, F3 n, const SH9 &c, float u0,
     const float b2 = 1.0f - b0 - b1;
     const float val = b0 * v0 + b1 * v1 + b2 * v2;
     const F3 p = mk(lv[0], lv[1], lv[2]) * b0 + mk(lv[12], lv[13], lv[14]) * b1 + mk(lv[24], lv[25], lv[26]) * b2;
-    dir = p / Length(p);
+    dir = FastDiv(p, Length(p));
     const float whole = WholeIntegral(E, c);
-    pdf = IsTiny(whole) ? 0.f : val / whole;
+    pdf = IsTiny(whole) ? 0.f : FastDiv(val, whole);
     const F3 fl = -dir;
     const EmPair le = EmEvalPair(E, dir);
     pdf += BasePdfLe(E, fl, le.opp, c);
@@ -518,7 +519,7 @@ PG3_DEV bool SteerSample(const DevEm &E, F3 n, uint32_t sides, float uSide, floa
     }
     const F3 hn = up ? n : -n;
     SteerSampleHemisphere(E, hn, ClampedCos(hn), u0, u1, dir, pdf, radCos, tc);
-    if (front && back) pdf *= (up ? ui : li) / wi;
+    if (front && back) pdf *= FastDiv(up ? ui : li, wi);
     return true;
 }
 // pre (may be null): the EM radiance pair of dir, when the caller has it already
@@ -541,7 +542,7 @@ PG3_DEV float SteerPdf(const DevEm &E, F3 dir, F3 n, uint32_t sides, const EmPai
     }
     const F3 hn = up ? n : -n;
     const float p = SteerPdfHemisphere(E, dir, hn, ClampedCos(hn), pre);
-    return (front && back) ? p * ((up ? ui : li) / wi) : p;
+    return (front && back) ? p * FastDiv(up ? ui : li, wi) : p;
 }
 PG3_DEV float SteerIrradiance(const DevEm &E, F3 n, uint32_t sides) {               // :2304-2358
     const float ui = WholeIntegral(E, ClampedCos(n)), li = WholeIntegral(E, ClampedCos(-n));

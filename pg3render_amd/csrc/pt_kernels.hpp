@@ -231,8 +231,8 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey
     const bool finite = (r.pdfW != kInf);
     if (finite) {
         ps.prevBsdfPdf = r.pdfW * r.compProb;
-        ps.eThr = ps.beta * r.att * cosAbs / ps.prevBsdfPdf;
-        if (!cut) ps.beta = ps.beta * ((r.att * cosAbs) / (r.pdfW * 1u * rr * r.compProb));
+        ps.eThr = FastDiv(ps.beta * r.att * cosAbs, ps.prevBsdfPdf);
+        if (!cut) ps.beta = ps.beta * FastDiv(r.att * cosAbs, r.pdfW * 1u * rr * r.compProb);
     } else {
         ps.eThr = ps.beta * r.att / (1.f * r.compProb);
         if (!cut) ps.beta = ps.beta * (r.att / (1u * rr * r.compProb));
