@@ -38,6 +38,7 @@ struct DevEm {
     int32_t bilinear;
     int32_t sampler;
     const float4 *texels;      // w*h RGB texels padded to 16 B (one load per texel)
+    const float *lum;          // w*h texel luminances (the steerable pdf reads luminance only)
     // simple spherical sampler (em_simple_spherical_sampler.hxx): FP32 CDFs built from FP64 sums
     const float *margCdf;      // h+1
     const float *condCdf;      // h*(w+1)
@@ -284,6 +285,29 @@ PG3_DEV F3 EmEvalFoot(const float4 *tex, const EmFoot &f) {
     const F3 b = TexelAt(tex, f.c) * (1 - f.xl) + TexelAt(tex, f.d) * f.xl;
     return a * (1 - f.yl) + b * f.yl;
 }
+// Luminance of EmEvalUV from the luminance map (same texels and weights; luminance is linear).
+PG3_DEV float LumAt(const DevEm &E, uint32_t x, uint32_t y) { return AsGlobal(E.lum)[(size_t)E.w * y + x]; }
+PG3_DEV float EmLumUV(const DevEm &E, float u, float v) {
+    const float xf = u * (float)E.w, yf = v * (float)E.h;
+    if (!E.bilinear) {
+        const uint32_t x0 = min((uint32_t)xf, E.w - 1u), y0 = min((uint32_t)yf, E.h - 1u);
+        return LumAt(E, x0, y0);
+    }
+    const int32_t cx = (int32_t)(xf + 0.5f), cy = (int32_t)(yf + 0.5f);
+    const int32_t x0 = cx - 1, y0 = cy - 1;
+    const float xl = xf - ((float)x0 + 0.5f);
+    const float yl = yf - ((float)y0 + 0.5f);
+    uint32_t ax, ay, bx, by;
+    EmNorm(E, x0, y0, ax, ay);
+    EmNorm(E, cx, cy, bx, by);
+    const float a = LumAt(E, ax, ay) * (1 - xl) + LumAt(E, bx, ay) * xl;
+    const float b = LumAt(E, ax, by) * (1 - xl) + LumAt(E, bx, by) * xl;
+    return a * (1 - yl) + b * yl;
+}
+PG3_DEV void Antipode(float u, float v, float &uo, float &vo) {   // lat-long of -dir from that of dir
+    uo = (u < 0.5f) ? u + 0.5f : u - 0.5f;
+    vo = 1.f - v;
+}
 PG3_DEV F3 EmEval(const DevEm &E, F3 dir) {
     float u, v;
     Dir2LatLong(dir, u, v);
@@ -378,23 +402,23 @@ PG3_DEV float ShDotCoeffs(F3 d, const SH9 &c) {               // Dot(GenerateSph
     return r;
 }
 PG3_DEV float WholeIntegral(const DevEm &E, const SH9 &c) { return Dot9(E.rootW, c); }  // :2416-2432
-// GetBasePdf, :2387-2413, with the EM radiance at dir given
-PG3_DEV float BasePdfLe(const DevEm &E, F3 dir, F3 Le, const SH9 &c) {
-    const float lum = Luminance(Le);
+// GetBasePdf, :2387-2413, with the EM luminance at dir given
+PG3_DEV float BasePdfLum(const DevEm &E, F3 dir, float lum, const SH9 &c) {
     const float cc = ShDotCoeffs(dir, c);
     const float whole = WholeIntegral(E, c);
     return IsTiny(whole) ? 0.f : FastDiv(lum * cc, whole);
 }
-// EM radiance at a direction and at its antipode with one lat-long mapping: the antipode of
-// (u, v) is (u +- 1/2, 1 - v) (the steerable sampler's pdf and sample always need both).
+// EM radiance at a direction and luminance at its antipode with one lat-long mapping: the
+// antipode of (u, v) is (u +- 1/2, 1 - v) (the steerable pdf always needs both directions).
 struct EmPair {
-    F3 at, opp;
+    F3 at;
+    float lumOpp;
 };
 PG3_DEV EmPair EmEvalPair(const DevEm &E, F3 dir) {
-    float u, v;
+    float u, v, uo, vo;
     Dir2LatLong(dir, u, v);
-    const float uo = (u < 0.5f) ? u + 0.5f : u - 0.5f, vo = 1.f - v;
-    return EmPair{EmEvalUV(E, u, v), EmEvalUV(E, uo, vo)};
+    Antipode(u, v, uo, vo);
+    return EmPair{EmEvalUV(E, u, v), EmLumUV(E, uo, vo)};
 }
 // Inner levels of the steerable tree staged in LDS by a kernel: levels >= minLevel (down from the
 // root) are read from `nodes` (node index - baseNode) instead of E.nodeW.
@@ -493,12 +517,17 @@ PG3_DEV bool SteerSampleHemisphere(const DevEm &E, F3 n, const SH9 &c, float u0,
     const float whole = WholeIntegral(E, c);
     pdf = IsTiny(whole) ? 0.f : FastDiv(val, whole);
     const F3 fl = -dir;
-    const EmPair le = EmEvalPair(E, dir);
-    pdf += BasePdfLe(E, fl, le.opp, c);
     float cs = Dot(dir, n);
     const bool flip = cs < 0.f;
     if (flip) { dir = fl; cs *= -1.f; }
-    radCos = (flip ? le.opp : le.at) * cs;
+    // radiance at the returned direction; the base pdf needs the luminance at -(sampled
+    // direction), which is the returned direction when it was flipped and its antipode otherwise
+    float u, v, uo, vo;
+    Dir2LatLong(dir, u, v);
+    const F3 Le = EmEvalUV(E, u, v);
+    Antipode(u, v, uo, vo);
+    pdf += BasePdfLum(E, fl, flip ? Luminance(Le) : EmLumUV(E, uo, vo), c);
+    radCos = Le * cs;
     return true;
 }
 // SampleImpl, :2084-2157.  One call site of the hemisphere sampler: a lane's hemisphere (normal
@@ -525,8 +554,18 @@ PG3_DEV bool SteerSample(const DevEm &E, F3 n, uint32_t sides, float uSide, floa
 // pre (may be null): the EM radiance pair of dir, when the caller has it already
 PG3_DEV float SteerPdfHemisphere(const DevEm &E, F3 dir, F3 n, const SH9 &c, const EmPair *pre) {  // :2281-2299
     if (Dot(dir, n) < 0.f) return 0.f;
-    const EmPair le = pre ? *pre : EmEvalPair(E, dir);
-    return BasePdfLe(E, dir, le.at, c) + BasePdfLe(E, -dir, le.opp, c);
+    float la, lo;
+    if (pre) {
+        la = Luminance(pre->at);
+        lo = pre->lumOpp;
+    } else {
+        float u, v, uo, vo;
+        Dir2LatLong(dir, u, v);
+        Antipode(u, v, uo, vo);
+        la = EmLumUV(E, u, v);
+        lo = EmLumUV(E, uo, vo);
+    }
+    return BasePdfLum(E, dir, la, c) + BasePdfLum(E, -dir, lo, c);
 }
 PG3_DEV float SteerPdf(const DevEm &E, F3 dir, F3 n, uint32_t sides, const EmPair *pre = nullptr) {  // :2220-2278
     const bool front = sides & kSideFront, back = sides & kSideBack;

@@ -60,7 +60,7 @@ struct SceneSlot {
     bool valid = false;
     SceneView view{};
     pg3g_render_params params{};
-    DevBuf tris, sphs, mats, lights, texels, margCdf, condCdf, rowInt, nodeW, leafVerts, fb;
+    DevBuf tris, sphs, mats, lights, texels, lum, margCdf, condCdf, rowInt, nodeW, leafVerts, fb;
     DevBuf viewDev;      // device copy of `view`, refreshed on upload and parameter changes
     // differentiable rendering
     std::vector<pg3g_material> hostMats;      // sampling materials as given (ctor arguments)
@@ -397,6 +397,9 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
             tex[i] = make_float4(d->em.texels[i * 3], d->em.texels[i * 3 + 1], d->em.texels[i * 3 + 2], 0.f);
         PG3_HIP(Upload(S.texels, tex.data(), tex.size(), st));
         E.texels = S.texels.as<float4>();
+        PG3_HIP(S.lum.Alloc(sizeof(float) * tex.size()));
+        E.lum = S.lum.as<float>();
+        PG3_HIP(LaunchEmLum(E.texels, S.lum.as<float>(), (uint32_t)tex.size(), st));
         S.hostTexels.assign(d->em.texels, d->em.texels + (size_t)E.w * E.h * 3);
         S.emDesc = d->em;
         S.emDesc.texels = nullptr;
@@ -468,6 +471,7 @@ int SyncView(pg3g_context *ctx, SceneSlot &S) {
 // Rebuilds the EM sampling tables of a slot from S.hostTexels (already uploaded to S.texels).
 int RebuildEmTables(pg3g_context *ctx, SceneSlot &S) {
     DevEm &E = S.view.em;
+    PG3_HIP(LaunchEmLum(E.texels, S.lum.as<float>(), E.w * E.h, ctx->stream));
     PG3_HIP(LaunchCdfBuild(E, S.condCdf.as<float>(), S.margCdf.as<float>(), S.rowInt.as<float>(), ctx->stream));
     if (E.sampler == PG3G_EM_STEERABLE) {
         std::string err;

@@ -411,6 +411,18 @@ hipError_t LaunchEmEval(const SceneView &S, uint32_t n, const float *dirs, float
     hipLaunchKernelGGL(EmEvalKernel, dim3((n + 255) / 256), dim3(256), 0, s, S, n, dirs, rgb);
     return hipGetLastError();
 }
+__global__ void EmLumKernel(const float4 *__restrict__ tex, float *__restrict__ lum, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float4 t = tex[i];
+        lum[i] = Luminance(mk(t.x, t.y, t.z));
+    }
+}
+hipError_t LaunchEmLum(const float4 *tex, float *lum, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(EmLumKernel, dim3((n + 255) / 256), dim3(256), 0, s, tex, lum, n);
+    return hipGetLastError();
+}
 hipError_t LaunchCdfBuild(const DevEm &E, float *condCdf, float *margCdf, float *rowIntegral, hipStream_t s) {
     hipLaunchKernelGGL(CdfRowsKernel, dim3(E.h), dim3(256), 0, s, E, condCdf, rowIntegral);
     hipError_t e = hipGetLastError();
