@@ -237,6 +237,7 @@ typedef struct pg3g_batch_config {
 
 int pg3g_batch_create(pg3g_context *ctx, uint32_t num_scenes, const uint32_t *slots,
                       const uint32_t *spp, pg3g_batch **batch);
+/* A batch refers to its context: destroy every batch before pg3g_destroy of its context. */
 int pg3g_batch_destroy(pg3g_batch *batch);
 int pg3g_batch_set_config(pg3g_batch *batch, const pg3g_batch_config *config);
 int pg3g_batch_get_config(const pg3g_batch *batch, pg3g_batch_config *config);

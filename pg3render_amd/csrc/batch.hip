@@ -3,7 +3,7 @@
 // scene per process (pg3render.cxx:54-174, one AbstractRenderer per thread); here every scene is
 // a set of work items (one 8x8 pixel tile x a run of iterations) that resident wavefronts take
 // from a device counter until none is left.  Each item is traced exactly as the per-scene kernel
-// traces it (TracePixel: same samples, same RNG streams), so a batched pass and per-scene renders
+// traces it (TraceItems: same samples, same RNG streams), so a batched pass and per-scene renders
 // of the same iterations produce the same per-pixel sums.
 #include <hip/hip_runtime.h>
 
@@ -24,32 +24,48 @@ __device__ __forceinline__ uint32_t SceneOfItem(const BatchScene *__restrict__ i
     return lo;
 }
 
+// Item source of a batch pass: global item i belongs to the last scene whose first item is <= i;
+// within the scene, item = chunk * numTiles + tile as in the per-scene kernel.
+template <int kEmKind, bool kOneLight>
+struct BatchItems {
+    const SceneView *views;
+    const BatchScene *info;
+    uint32_t numScenes, numItems, pass;
+    uint32_t *work;
+    PG3_DEV bool Next(TileItem &t) {
+        uint32_t item = 0;
+        if (__lane_id() == 0) item = atomicAdd(&work[0], 1u);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= numItems) return false;   // every wave leaves once the counter passes the last item
+        const uint32_t s = SceneOfItem(info, numScenes, item);
+        const BatchScene bi = info[s];
+        const uint32_t local = item - bi.itemBegin;
+        const uint32_t tile = local % bi.numTiles, chunk = local / bi.numTiles;
+        const uint32_t base = pass * bi.spp;
+        t.scene = s;
+        t.x0 = (tile % bi.tilesX) * 8u;
+        t.y0 = (tile / bi.tilesX) * 8u;
+        t.it = base + chunk * bi.itersPerItem;
+        t.total = 64u * (min(bi.spp, (chunk + 1u) * bi.itersPerItem) - chunk * bi.itersPerItem);
+        t.fb = bi.fb;
+        return true;
+    }
+    PG3_DEV const SceneView *View(uint32_t s) const { return views + s; }
+    PG3_DEV LightSpec Spec(uint32_t s) const {
+        const SceneView &S = views[s];
+        return (kEmKind == kEmDynamic) ? SceneLightSpec(S) : LightSpec{kEmKind, kOneLight ? 1u : S.numLights};
+    }
+};
+
 template <int kEmKind, bool kOneLight>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 BatchKernel(const SceneView *__restrict__ views, const BatchScene *__restrict__ info, uint32_t numScenes,
             uint32_t numItems, uint32_t *__restrict__ work, unsigned long long *__restrict__ counters) {
     const uint32_t lane = __lane_id();
-    const uint32_t pass = work[1];
-    __shared__ double accLds[4][64 * 3];
+    __shared__ double accLds[4][2 * 64 * 3];
     uint32_t segments = 0, samples = 0;
-    for (;;) {
-        uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(&work[0], 1u);
-        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
-        if (item >= numItems) break;   // every wave leaves once the counter passes the last item
-        const uint32_t s = SceneOfItem(info, numScenes, item);
-        const BatchScene bi = info[s];
-        const SceneView &S = views[s];
-        const LightSpec spec = (kEmKind == kEmDynamic) ? SceneLightSpec(S)
-                                                       : LightSpec{kEmKind, kOneLight ? 1u : S.numLights};
-        const uint32_t local = item - bi.itemBegin;
-        const uint32_t tile = local % bi.numTiles, chunk = local / bi.numTiles;
-        const uint32_t base = pass * bi.spp;
-        const uint32_t itBegin = base + chunk * bi.itersPerItem;
-        const uint32_t itEnd = base + min(bi.spp, (chunk + 1u) * bi.itersPerItem);
-        TraceTile(S, spec, (tile % bi.tilesX) * 8u, (tile / bi.tilesX) * 8u, itBegin, itEnd, accLds[threadIdx.x >> 6],
-                  bi.fb, segments, samples);
-    }
+    BatchItems<kEmKind, kOneLight> src{views, info, numScenes, numItems, work[1], work};
+    TraceItems(src, accLds[threadIdx.x >> 6], segments, samples);
     const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
     if (lane == 0 && counters) {
         atomicAdd(counters + 0, (unsigned long long)segSum);

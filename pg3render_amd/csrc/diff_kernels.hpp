@@ -233,8 +233,8 @@ PG3_DEV bool DiffVertex(const SceneView &S, const LightSpec &spec, const RngKey 
 
 // A work item (8x8 tile x iterations) of the differentiable forward (kAdjoint false: per-pixel
 // sums into fb) or of the adjoint (kAdjoint true: each sample is traced twice, the primal replay
-// giving its radiance and the second replay the gradients).  Samples are handed to lanes as in
-// TraceTile.  sampleL (may be null) holds one radiance per sample of the launch, index
+// giving its radiance and the second replay the gradients).  Samples are handed out in order, one
+// to each lane whose path has ended.  sampleL (may be null) holds one radiance per sample of the launch, index
 // (iteration - firstIter) * pixels + pixel: the forward writes it, and the adjoint then reads the
 // radiance instead of running the primal replay.
 template <bool kAdjoint>

@@ -1005,7 +1005,7 @@ int pg3g_batch_autotune(pg3g_batch *batch, uint32_t max_candidates, float *best_
     if (int rc = BatchCheck(batch)) return rc;
     // Work-item length (iterations of one tile) x persistent wavefronts per CU; the first
     // candidates are the ones most often best, so a small max_candidates still covers them.
-    static const uint32_t kCand[][2] = {{8, 16}, {4, 16}, {16, 16}, {2, 16}, {32, 16}, {8, 8}, {8, 32}, {4, 32}};
+    static const uint32_t kCand[][2] = {{2, 16}, {4, 16}, {1, 16}, {8, 16}, {16, 16}, {2, 8}, {4, 8}, {8, 32}};
     const uint32_t n = (max_candidates == 0) ? 8u : std::min<uint32_t>(max_candidates, 8u);
     const uint32_t useGraph = batch->cfg.use_graph;
     float best = 3.4e38f;

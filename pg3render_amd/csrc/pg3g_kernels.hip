@@ -82,7 +82,7 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     // its ~450 bytes of fields spill into VGPR lanes of this register-heavy kernel.
     const SceneView &S = *Sp;
     LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
-    __shared__ double accLds[PTBlock<kEmKind>() / 64u][64 * 3];
+    __shared__ double accLds[PTBlock<kEmKind>() / 64u][2 * 64 * 3];
     if constexpr (kEmKind == kEmSteerable) {
         __shared__ F4v treeLds[kTreeLdsNodes * 3];
         // the levels from the root down that fit (contiguous at the end of nodeW)
@@ -103,23 +103,11 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
         }
         __syncthreads();
     }
-    const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
-    const uint32_t numTiles = tilesX * tilesY;
-    const uint32_t numItems = numTiles * numChunks;
+    const uint32_t tilesX = (S.resX + 7u) / 8u, numTiles = tilesX * ((S.resY + 7u) / 8u);
     const uint32_t lane = __lane_id();
     uint32_t segments = 0, samples = 0;
-    for (;;) {
-        uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(work, 1u);
-        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
-        if (item >= numItems) break;   // every wave leaves once the counter passes the last item
-        const uint32_t tile = item % numTiles, chunk = item / numTiles;
-        const uint32_t it = firstIter + chunk * itersPerThread;
-        const uint32_t itEnd = min(firstIter + nIter, it + itersPerThread);
-        if (it < itEnd)
-            TraceTile(S, spec, (tile % tilesX) * 8u, (tile / tilesX) * 8u, it, itEnd, accLds[threadIdx.x >> 6], fb,
-                      segments, samples);
-    }
+    SceneItems src{Sp, spec, work, fb, firstIter, nIter, itersPerThread, numTiles * numChunks, numTiles, tilesX};
+    TraceItems(src, accLds[threadIdx.x >> 6], segments, samples);
     const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
     if (lane == 0 && counters) {
         atomicAdd(counters + 0, (unsigned long long)segSum);
@@ -373,10 +361,17 @@ hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t 
     const uint32_t ptBlock = (SceneLightSpec(S).em == kEmSteerable) ? PTBlock<kEmSteerable>() : PTBlock<kEmSimple>();
     if (pt) {
         if (!work) return hipErrorInvalidValue;
+        // items of a few iterations: the wave overlaps items, so small items cost no tail, and
+        // they even out the waves' shares at the end of the launch
+        const uint32_t waves = (uint32_t)max(1, numCUs) * 16u;
+        const uint64_t perWave = (uint64_t)nIter * tiles / (16u * waves);
+        uint32_t ipi = perWave >= 2u ? 2u : 1u;
+        if (const char *e = getenv("PG3G_PT_ITEM_ITERS")) ipi = (uint32_t)max(1, atoi(e));
+        const uint32_t ptIpi = min(ipi, nIter), ptChunks = (nIter + ptIpi - 1) / ptIpi;
         // 16 resident wavefronts per CU (4 per SIMD, the kernel's register budget)
         const uint32_t grid = (uint32_t)max(1, numCUs) * (1024u / ptBlock);
         hipLaunchKernelGGL(ResetWorkCounter, dim3(1), dim3(64), 0, stream, work);
-        hipLaunchKernelGGL(pt, dim3(grid), dim3(ptBlock), 0, stream, Sdev, firstIter, nIter, ipt, numChunks, fb, work,
+        hipLaunchKernelGGL(pt, dim3(grid), dim3(ptBlock), 0, stream, Sdev, firstIter, nIter, ptIpi, ptChunks, fb, work,
                            counters);
     }
     else if (S.algorithm == 7)

@@ -633,76 +633,170 @@ PG3_DEV void CameraSample(const SceneView &S, uint32_t pixel, uint32_t iteration
     GenerateRay(S, float(x) + jx, float(y) + jy, org, dir);
 }
 
-// One work item of PathTracer -sb 1: the 8x8 pixel tile with corner (x0, y0) x iterations
-// [itBegin, itEnd), traced by one wavefront.  The item's samples q = 0 .. 64*(itEnd-itBegin)-1
-// (tile pixel q % 64, iteration itBegin + q / 64) are handed out in order, one to each lane whose
-// path has ended (path regeneration across pixels and iterations), so every lane stays busy
-// until the item's last samples are in flight.  Per-pixel FP64 sums are gathered in the
-// wavefront's LDS slice `acc` (64 pixels x 3) and added to the framebuffer once per pixel.
-// Which lane traces which sample changes nothing: a sample's path depends only on its
-// (pixel, iteration) RNG stream.
-PG3_DEV void TraceTile(const SceneView &S, const LightSpec &spec, uint32_t x0, uint32_t y0, uint32_t itBegin,
-                       uint32_t itEnd, double *acc, double *fb, uint32_t &segments, uint32_t &samples) {
+// Adds LDS slot `slot` (64 pixels x 3 sums of one tile) to the framebuffer and clears it.
+PG3_DEV void FlushSlot(double *slot, uint32_t x0, uint32_t y0, uint32_t resX, uint32_t resY, double *fb) {
     const uint32_t lane = __lane_id();
-    acc[lane * 3 + 0] = 0.0;
-    acc[lane * 3 + 1] = 0.0;
-    acc[lane * 3 + 2] = 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t total = 64u * (itEnd - itBegin);
-    uint32_t next = 0;       // first sample not handed out yet (wave-uniform)
-    bool busy = false;
-    uint32_t pix = 0;        // tile pixel of this lane's sample
-    RngKey key{};
-    PathState ps;
-    for (;;) {
-        const uint64_t idle = __ballot(!busy);
-        if (next < total && idle != 0) {
-            const uint32_t q = next + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-            if (!busy && q < total) {
-                pix = q & 63u;
-                const uint32_t x = x0 + (pix & 7u), y = y0 + (pix >> 3);
-                if (x < S.resX && y < S.resY) {
-                    F3 org, dir;
-                    CameraSample(S, y * S.resX + x, itBegin + (q >> 6), key, org, dir);
-                    InitPath(ps, org, dir);
-                    busy = true;
-                }
-            }
-            next += (uint32_t)__popcll(idle);
-        }
-        if (__ballot(busy) == 0) {
-            if (next >= total) break;
-            continue;
-        }
-        if (busy) {
-            Hit h;
-            h.dist = 1e36f;
-            const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
-            ShadowReq sr;
-            const bool cont = ShadeVertex(S, spec, key, ps, hit, h, sr, segments);
-            if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
-            if (!cont) {
-                atomicAdd(acc + pix * 3 + 0, (double)ps.L.x);
-                atomicAdd(acc + pix * 3 + 1, (double)ps.L.y);
-                atomicAdd(acc + pix * 3 + 2, (double)ps.L.z);
-                ++samples;
-                busy = false;
-            }
-        }
-    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint32_t x = x0 + (lane & 7u), y = y0 + (lane >> 3);
-    if (x < S.resX && y < S.resY) {
-        double *p = fb + (size_t)(y * S.resX + x) * 3;
-        atomicAdd(p + 0, acc[lane * 3 + 0]);
-        atomicAdd(p + 1, acc[lane * 3 + 1]);
-        atomicAdd(p + 2, acc[lane * 3 + 2]);
+    if (x < resX && y < resY) {
+        double *p = fb + (size_t)(y * resX + x) * 3;
+        atomicAdd(p + 0, slot[lane * 3 + 0]);
+        atomicAdd(p + 1, slot[lane * 3 + 1]);
+        atomicAdd(p + 2, slot[lane * 3 + 2]);
+    }
+    slot[lane * 3 + 0] = 0.0;
+    slot[lane * 3 + 1] = 0.0;
+    slot[lane * 3 + 2] = 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A work item of PathTracer -sb 1: the 8x8 pixel tile with corner (x0, y0) of scene `scene`
+// x iterations [it, it + total / 64).  Its samples q = 0 .. total-1 are (tile pixel q % 64,
+// iteration it + q / 64).
+struct TileItem {
+    uint32_t scene, x0, y0, it, total;
+    double *fb;
+};
+
+// Persistent wavefront loop over work items.  Samples are handed out in order, one to each lane
+// whose path has ended (path regeneration across pixels and iterations), and the wavefront keeps
+// two items open: once the newer item has handed out its last sample, lanes whose paths end
+// start samples of a further item in the other slot while the older item's last paths still
+// run, so no lane waits for the longest path of an item.  Each item's sums are gathered in its
+// LDS slot (`acc`: 2 x 64 pixels x 3 FP64) and added to its framebuffer as soon as its last
+// sample ends.  Open items share one scene: an item of another scene waits until the older item
+// has drained.  All slot bookkeeping is wave-uniform.  Which lane traces which sample changes
+// nothing: a sample's path depends only on its (pixel, iteration) RNG stream.
+//   src.Next(TileItem&) -> bool   fetches the next item (false: none left; wave-uniform)
+//   src.View(scene) -> const SceneView *, src.Spec(scene) -> LightSpec
+template <class Src>
+PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &samples) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (uint32_t k = 0; k < 6; ++k) acc[lane + 64u * k] = 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    TileItem A{}, B{};                 // the two slots (wave-uniform)
+    uint32_t penA = 0, penB = 0;       // samples of the slot in flight
+    bool liveA = false, liveB = false, more = true;
+    uint32_t cur = 0, next = 0;        // slot handing out samples, its first sample not handed out
+    bool held = false;                 // cur's scene differs from the scene of the running paths
+    uint32_t scene = 0;                // scene of the running paths
+    bool busy = false;
+    uint32_t pix = 0, slot = 0;        // this lane's sample: tile pixel and slot
+    RngKey key{};
+    PathState ps;
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        if (idle != 0 && more && !(liveA && liveB) && !((cur ? liveB : liveA) && next < (cur ? B.total : A.total))) {
+            TileItem t;
+            if (!src.Next(t)) more = false;
+            else {
+                cur = liveA ? 1u : 0u;
+                if (cur) B = t;
+                else A = t;
+                next = 0;
+                held = (t.scene != scene) && (liveA || liveB);
+                if (cur) { penB = 0; liveB = true; }
+                else { penA = 0; liveA = true; }
+            }
+        }
+        if (held && !(cur ? liveA : liveB)) held = false;   // the older item has drained
+        const bool curOpen = !held && (cur ? liveB : liveA) && next < (cur ? B.total : A.total);
+        if (curOpen) scene = cur ? B.scene : A.scene;
+        // the scene is looked up by index at each use: a pointer variable carried around the
+        // loop would lose its address space (flat loads in place of scalar and LDS loads)
+        const SceneView &S = *src.View(scene);
+        const LightSpec spec = src.Spec(scene);
+        if (idle != 0 && curOpen) {
+            const TileItem &T = cur ? B : A;
+            const uint32_t q = next + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+            bool start = false;
+            if (!busy && q < T.total) {
+                pix = q & 63u;
+                const uint32_t x = T.x0 + (pix & 7u), y = T.y0 + (pix >> 3);
+                if (x < S.resX && y < S.resY) {
+                    F3 org, dir;
+                    CameraSample(S, y * S.resX + x, T.it + (q >> 6), key, org, dir);
+                    InitPath(ps, org, dir);
+                    busy = true;
+                    slot = cur;
+                    start = true;
+                }
+            }
+            next += (uint32_t)__popcll(idle);
+            const uint32_t started = (uint32_t)__popcll(__ballot(start));
+            if (cur) penB += started;
+            else penA += started;
+        }
+        if (__ballot(busy) != 0) {
+            bool done = false;
+            if (busy) {
+                Hit h;
+                h.dist = 1e36f;
+                const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
+                ShadowReq sr;
+                const bool cont = ShadeVertex(S, spec, key, ps, hit, h, sr, segments);
+                if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
+                if (!cont) {
+                    double *a = acc + slot * 192u + pix * 3u;
+                    atomicAdd(a + 0, (double)ps.L.x);
+                    atomicAdd(a + 1, (double)ps.L.y);
+                    atomicAdd(a + 2, (double)ps.L.z);
+                    ++samples;
+                    busy = false;
+                    done = true;
+                }
+            }
+            const uint64_t fin = __ballot(done), finB = __ballot(done && slot != 0);
+            penB -= (uint32_t)__popcll(finB);
+            penA -= (uint32_t)__popcll(fin & ~finB);
+        }
+        // an item is complete once it has handed out every sample and none is in flight (a held
+        // item has handed out none)
+        if (liveA && penA == 0 && !(cur == 0 && (held || next < A.total))) {
+            const SceneView *V = src.View(A.scene);
+            FlushSlot(acc, A.x0, A.y0, V->resX, V->resY, A.fb);
+            liveA = false;
+        }
+        if (liveB && penB == 0 && !(cur == 1 && (held || next < B.total))) {
+            const SceneView *V = src.View(B.scene);
+            FlushSlot(acc + 192, B.x0, B.y0, V->resX, V->resY, B.fb);
+            liveB = false;
+        }
+        if (!more && !liveA && !liveB) break;
     }
 }
+
+// Item source of one scene: items chunk * numTiles + tile from the launch's work counter.
+struct SceneItems {
+    const SceneView *S;
+    LightSpec spec;
+    uint32_t *work;
+    double *fb;
+    uint32_t firstIter, nIter, itersPerItem, numItems, numTiles, tilesX;
+    PG3_DEV bool Next(TileItem &t) {
+        uint32_t item = 0;
+        if (__lane_id() == 0) item = atomicAdd(work, 1u);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= numItems) return false;   // every wave leaves once the counter passes the last item
+        const uint32_t tile = item % numTiles, chunk = item / numTiles;
+        t.scene = 0;
+        t.x0 = (tile % tilesX) * 8u;
+        t.y0 = (tile / tilesX) * 8u;
+        t.it = firstIter + chunk * itersPerItem;
+        t.total = 64u * (min(firstIter + nIter, t.it + itersPerItem) - t.it);
+        t.fb = fb;
+        return true;
+    }
+    PG3_DEV const SceneView *View(uint32_t) const { return S; }
+    PG3_DEV LightSpec Spec(uint32_t) const { return spec; }
+};
 
 PG3_DEV F3 RadianceSample(const SceneView &S, uint32_t pixel, uint32_t iteration, uint32_t &segments) {
     RngKey key;

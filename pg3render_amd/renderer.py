@@ -6,6 +6,7 @@
 `was_used()` is `WasUsed`.  Scenes come from the C++ builders in libpg3g (include/pg3g_scenes.h).
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -88,10 +89,13 @@ class Context:
     def __init__(self, device=0):
         self._lib = _ffi.load()
         self.ptr = C.c_void_p()
+        self._batches = weakref.WeakSet()   # closed before the context (they refer to it)
         check(self._lib.pg3g_create(C.byref(self.ptr), device), "pg3g_create")
 
     def close(self):
         if self.ptr:
+            for b in list(self._batches):
+                b.close()
             self._lib.pg3g_destroy(self.ptr)
             self.ptr = C.c_void_p()
 
@@ -257,6 +261,7 @@ class Batch:
         sp = (C.c_uint32 * n)(*self.spp)
         self.ptr = C.c_void_p()
         check(self._lib.pg3g_batch_create(ctx.ptr, n, sl, sp, C.byref(self.ptr)), "pg3g_batch_create")
+        ctx._batches.add(self)
 
     def close(self):
         if self.ptr:
