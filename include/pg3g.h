@@ -169,7 +169,7 @@ int pg3g_render_slot(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
 int pg3g_render_timed(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
                       float *elapsed_ms);
 
-/* Kernel organisation of pg3g_render*: wavefront queues (default) or one fused kernel. */
+/* Kernel organisation of pg3g_render*: one fused kernel (default) or wavefront queues. */
 #define PG3G_KERNEL_WAVEFRONT 0
 #define PG3G_KERNEL_MEGAKERNEL 1
 int pg3g_set_kernel_mode(pg3g_context *ctx, int mode);

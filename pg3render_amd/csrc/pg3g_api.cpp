@@ -91,7 +91,7 @@ struct pg3g_context {
     std::vector<std::unique_ptr<SceneSlot>> slots;
     DevBuf counters;   // [segments, camera samples]
     DevBuf work;       // item counter of the persistent path-tracing kernel
-    int kernelMode = PG3G_KERNEL_WAVEFRONT;
+    int kernelMode = PG3G_KERNEL_MEGAKERNEL;   // the fused kernel: 3-4x the wavefront pipeline
     DevBuf scratch[8];
     WavefrontContext *wavefront = nullptr;
     uint64_t sceneEpoch = 0;   // bumped by every upload / parameter change (invalidates batches)

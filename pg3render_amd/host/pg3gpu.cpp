@@ -6,7 +6,7 @@
 //
 //   pg3gpu [-s <scene_id>] [-w <workload>] [-a <pt|ptn|dbs|dlsa|dlss|dmis>] [-i <iterations>]
 //          [-minpl <n>] [-maxpl <n>] [-iic <f>] [-sb <f>] [-em <type>] [-r <resolution>]
-//          [-seed <n>] [-k <wavefront|megakernel>] [-o <file.hdr>]
+//          [-seed <n>] [-k <megakernel|wavefront>] [-o <file.hdr>]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -55,7 +55,7 @@ bool SaveHdr(const char *path, const std::vector<double> &img, uint32_t w, uint3
 }  // namespace
 
 int main(int argc, char **argv) {
-    int sceneId = -1, workload = 0, algorithm = PG3G_ALG_PATH_TRACING, kernel = PG3G_KERNEL_WAVEFRONT;
+    int sceneId = -1, workload = 0, algorithm = PG3G_ALG_PATH_TRACING, kernel = PG3G_KERNEL_MEGAKERNEL;
     uint32_t iterations = 1, emType = 0, res = 0;
     long minPL = -1, maxPL = -1;
     double iic = -1, sb = -1;
@@ -81,7 +81,7 @@ int main(int argc, char **argv) {
         else if (a == "-em") emType = (uint32_t)std::atol(need());
         else if (a == "-r") res = (uint32_t)std::atol(need());
         else if (a == "-seed") seed = std::strtoull(need(), nullptr, 10);
-        else if (a == "-k") kernel = (std::string(need()) == "megakernel") ? PG3G_KERNEL_MEGAKERNEL : PG3G_KERNEL_WAVEFRONT;
+        else if (a == "-k") kernel = (std::string(need()) == "wavefront") ? PG3G_KERNEL_WAVEFRONT : PG3G_KERNEL_MEGAKERNEL;
         else if (a == "-o") out = need();
         else { std::fprintf(stderr, "pg3gpu: unknown option %s\n", a.c_str()); return 2; }
     }
