@@ -63,15 +63,17 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
 #ifndef PG3_PT_WAVES_PER_EU
 #define PG3_PT_WAVES_PER_EU 4
 #endif
-// Persistent: each CU holds 16 wavefronts for the whole launch, and every wavefront takes work
+// Persistent: each CU holds kPTWavesPerCU (16) wavefronts for the whole launch, and every wavefront takes work
 // items (8x8 tile x iteration chunk) from a device counter until none is left, so a wavefront
 // that finishes early starts the next item instead of waiting for its workgroup.  With the
 // steerable sampler the workgroup is 1024 lanes (one per CU) and stages the inner levels of the
 // EM's triangle tree in LDS once (kTreeLdsNodes nodes, 60 KiB): the tree descent of every light
 // sample then gathers from LDS instead of from the L2.
 constexpr uint32_t kTreeLdsNodes = 1280;
+constexpr uint32_t kPTWavesPerCU = 4u * PG3_PT_WAVES_PER_EU;
 template <int kEmKind>
-constexpr uint32_t PTBlock() { return kEmKind == kEmSteerable ? 1024u : 256u; }
+constexpr uint32_t PTBlock() { return kEmKind == kEmSteerable ? 64u * kPTWavesPerCU : 256u; }
+static_assert(64u * kPTWavesPerCU <= 1024u, "one steerable workgroup per CU holds every resident wavefront");
 
 template <int kEmKind, bool kOneLight>
 __global__ void __launch_bounds__(PTBlock<kEmKind>()) __attribute__((amdgpu_waves_per_eu(PG3_PT_WAVES_PER_EU)))
@@ -363,13 +365,13 @@ hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t 
         if (!work) return hipErrorInvalidValue;
         // items of a few iterations: the wave overlaps items, so small items cost no tail, and
         // they even out the waves' shares at the end of the launch
-        const uint32_t waves = (uint32_t)max(1, numCUs) * 16u;
+        const uint32_t waves = (uint32_t)max(1, numCUs) * kPTWavesPerCU;
         const uint64_t perWave = (uint64_t)nIter * tiles / (16u * waves);
         uint32_t ipi = perWave >= 2u ? 2u : 1u;
         if (const char *e = getenv("PG3G_PT_ITEM_ITERS")) ipi = (uint32_t)max(1, atoi(e));
         const uint32_t ptIpi = min(ipi, nIter), ptChunks = (nIter + ptIpi - 1) / ptIpi;
-        // 16 resident wavefronts per CU (4 per SIMD, the kernel's register budget)
-        const uint32_t grid = (uint32_t)max(1, numCUs) * (1024u / ptBlock);
+        // kPTWavesPerCU resident wavefronts per CU (the kernel's register budget per SIMD x 4 SIMDs)
+        const uint32_t grid = (uint32_t)max(1, numCUs) * (64u * kPTWavesPerCU / ptBlock);
         hipLaunchKernelGGL(ResetWorkCounter, dim3(1), dim3(64), 0, stream, work);
         hipLaunchKernelGGL(pt, dim3(grid), dim3(ptBlock), 0, stream, Sdev, firstIter, nIter, ptIpi, ptChunks, fb, work,
                            counters);
