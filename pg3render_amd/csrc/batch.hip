@@ -28,6 +28,7 @@ __device__ __forceinline__ uint32_t SceneOfItem(const BatchScene *__restrict__ i
 // within the scene, item = chunk * numTiles + tile as in the per-scene kernel.
 template <int kEmKind, bool kOneLight>
 struct BatchItems {
+    static constexpr bool kMergeEm = kOneLight && kEmKind == kEmSteerable;
     const SceneView *views;
     const BatchScene *info;
     uint32_t numScenes, numItems, pass;

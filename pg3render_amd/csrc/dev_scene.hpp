@@ -502,6 +502,21 @@ PG3_DEV void SampleTriBilinear(float ux, float uy, float va, float vb, float vc,
     b1 = s * (1 - t);
 }
 // SampleHemisphereImpl, :2162-2217
+// Direction sampled from the linear approximation of a hemisphere (PickTriangle, then a point of
+// the leaf triangle with density proportional to the interpolated vertex values) and the
+// approximation's value there.
+PG3_DEV F3 SteerTriangleSample(const DevEm &E, const SH9 &c, float u0, float u1, const TreeCache &tc, float &val) {
+    float ux = u0;
+    const uint32_t leaf = PickLeaf(E, c, ux, tc);
+    const GlobalPtr<float> lv = AsGlobal(E.leafVerts + (size_t)leaf * 36u);
+    const float v0 = Dot9(lv + 3, c), v1 = Dot9(lv + 15, c), v2 = Dot9(lv + 27, c);
+    float b0, b1;
+    SampleTriBilinear(ux, u1, v0, v1, v2, b0, b1);
+    const float b2 = 1.0f - b0 - b1;
+    val = b0 * v0 + b1 * v1 + b2 * v2;
+    const F3 p = mk(lv[0], lv[1], lv[2]) * b0 + mk(lv[12], lv[13], lv[14]) * b1 + mk(lv[24], lv[25], lv[26]) * b2;
+    return FastDiv(p, Length(p));
+}
 PG3_DEV bool SteerSampleHemisphere(const DevEm &E, F3 n, const SH9 &c, float u0, float u1, F3 &dir,
                                    float &pdf, F3 &radCos, const TreeCache &tc = TreeCache{}) {
     float ux = u0;

@@ -108,7 +108,9 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     const uint32_t tilesX = (S.resX + 7u) / 8u, numTiles = tilesX * ((S.resY + 7u) / 8u);
     const uint32_t lane = __lane_id();
     uint32_t segments = 0, samples = 0;
-    SceneItems src{Sp, spec, work, fb, firstIter, nIter, itersPerThread, numTiles * numChunks, numTiles, tilesX};
+    SceneItems<kOneLight && kEmKind == kEmSteerable> src{Sp,        spec,     work,     fb,      firstIter,
+                                                         nIter,     itersPerThread, numTiles * numChunks, numTiles,
+                                                         tilesX};
     TraceItems(src, accLds[threadIdx.x >> 6], segments, samples);
     const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
     if (lane == 0 && counters) {

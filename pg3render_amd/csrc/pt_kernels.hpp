@@ -134,12 +134,229 @@ struct ShadowReq {
     bool valid;
 };
 
+// Emission of an area light hit by the previous BSDF sample (path_tracer.hxx:198-210, MIS as
+// :327-369) and the checks that end the path at this vertex (:212-238).  Returns true if the
+// vertex is shaded.
+PG3_DEV bool HitEmissionAndChecks(const SceneView &S, const LightSpec &spec, const RngKey &key, PathState &ps, F3 sp,
+                                  F3 wol, const DevMaterial &mat, uint32_t &segments) {
+    const uint32_t len = ps.len;
+    const bool hasPrev = ps.flags & kHasPrev, prevFinite = ps.flags & kPrevFinite;
+    if (mat.lightId >= 0 && len >= S.minPL) {
+        const DevLight Lt = LoadGlobal(S.lights + mat.lightId);
+        F3 Le = mk(0.f);
+        if (Lt.type == kLightArea && wol.z > 0.f) Le = Lt.radiance;
+        if (!hasPrev) {
+            ps.L += Le;
+        } else if (!IsZero(Le)) {
+            if (prevFinite) {
+                const float lp = AreaEmissionPdf(Lt, sp, ps.org) * PrevPickProb(S, spec, key, ps, (uint32_t)mat.lightId);
+                ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
+            } else {
+                ps.L += ps.eThr * Le;
+            }
+        }
+    }
+    if (!(ps.flags & kComputeRefl)) return false;
+    if (ReflectanceZero(mat)) return false;
+    if (S.maxPL > 0 && len >= S.maxPL) return false;
+    if (len >= 1200u) return false;
+    ++segments;
+    return true;
+}
+
+// Shadow-ray request of a light sample (AddMISLightSampleContribution, path_tracer_base.hxx:
+// 350-425): the contribution is added if the segment to the light is unoccluded.
+PG3_DEV void LightSampleRequest(const PathState &ps, const DevMaterial &mat, const Frame &fr, F3 wol, F3 sp,
+                                const LightSample &ls, ShadowReq &sr) {
+    if (MaxC(ls.sample) > 0.f) {
+        BsdfRec r;
+        EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
+        if (ls.pdfW != kInf) {
+            const float lightPdf = ls.pdfW * ls.prob;
+            const float bsdfPdf = r.pdfW * r.compProb;
+            sr.contrib = ps.beta * ((ls.sample * r.att * MisPower(lightPdf, bsdfPdf)) / lightPdf);
+        } else {
+            sr.contrib = ps.beta * ((ls.sample * r.att) / (ls.prob * 1u));
+        }
+        sr.p = sp;
+        sr.dir = ls.wig;
+        sr.tmax = ls.dist;
+        sr.valid = true;
+    }
+}
+
+// BSDF sample, Russian roulette and the path update (path_tracer.hxx:267-303).  Returns true if
+// the path continues.
+PG3_DEV bool ContinuePath(const SceneView &S, PathState &ps, F3 sp, const Frame &fr, F3 wol, const DevMaterial &mat,
+                          const VertexRng &vr, float lcSum) {
+    float rr = 1.0f;
+    bool cut = false;
+    if (S.maxPL == 0) {
+        rr = Clampf(RRProb(mat, wol), 0.0f, 0.997f);
+        cut = vr.u[kDimRR] > rr;
+    }
+    BsdfRec r;
+    SampleBsdf(mat, wol, vr.u[kDimBsdf0], vr.u[kDimBsdf1], vr.u[kDimBsdf2], r);
+    if (IsBlocker(r)) return false;
+    const float cosAbs = fabsf(r.wil.z);
+    const bool finite = (r.pdfW != kInf);
+    if (finite) {
+        ps.prevBsdfPdf = r.pdfW * r.compProb;
+        ps.eThr = FastDiv(ps.beta * r.att * cosAbs, ps.prevBsdfPdf);
+        if (!cut) ps.beta = ps.beta * FastDiv(r.att * cosAbs, r.pdfW * 1u * rr * r.compProb);
+    } else {
+        ps.eThr = ps.beta * r.att / (1.f * r.compProb);
+        if (!cut) ps.beta = ps.beta * (r.att / (1u * rr * r.compProb));
+    }
+    ps.flags = kHasPrev | (finite ? kPrevFinite : 0u) | (cut ? 0u : kComputeRefl) | (mat.sides << 4);
+    ps.prevN = fr.z;
+    ps.lcSum = lcSum;
+    ps.org = sp;
+    ps.dir = ToWorld(fr, r.wil);
+    ps.tmin = EpsRayCos(cosAbs);
+    ps.len = ps.len + 1;
+    return true;
+}
+
+// Vertex of the path tracer with a steerable EM; the same results as the generic vertex below,
+// in another order.  An escaped path (emission with MIS, path_tracer.hxx:408-422: the EM pdf of
+// its direction at the previous vertex) and a light sample of the EM (SampleImpl and
+// SampleHemisphereImpl, em_steerable_sampler.hxx:2084-2217) need the same lookups: the
+// hemisphere's clamped-cosine coefficients and integrals, the EM radiance at one direction and
+// the luminance at its antipode (one lat-long mapping), and the two GetBasePdf terms
+// (:2387-2413) of that direction and its antipode.  Lanes of both kinds run these lookups in one
+// pass, so a wavefront no longer issues them once for its escaped lanes and again for its
+// shading lanes.
+PG3_DEV bool ShadeVertexSteerable(const SceneView &S, const LightSpec &spec, const RngKey &key, PathState &ps,
+                                  bool hit, const Hit &h, ShadowReq &sr, uint32_t &segments) {
+    const DevEm &E = S.em;
+    const uint32_t len = ps.len;
+    F3 sp = mk(0.f), wol = mk(0.f);
+    Frame fr{};
+    DevMaterial mat{};
+    VertexRng vr{};
+    LightCache lc;
+    lc.sum = 0.f;
+    bool shade = false;            // hit lane that shades this vertex
+    bool emq = false, nee = false;  // lane runs the EM lookups; they serve a light sample
+    F3 qn = ps.prevN, qdir = ps.dir;
+    uint32_t qsides = ps.flags >> 4;
+    float uSide = 0.f, u0 = 0.f, u1 = 0.f, prob = 1.f;
+    if (hit) {
+        sp = ps.org + ps.dir * h.dist;
+        fr = FrameFromZ(h.n);
+        wol = ToLocal(fr, -ps.dir);
+        mat = LoadGlobal(S.mats + h.mat);
+        shade = HitEmissionAndChecks(S, spec, key, ps, sp, wol, mat, segments);
+        if (shade) {
+            RngVertex(key, len, vr);
+            if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, key, len, lc);
+            if ((len + 1) >= S.minPL) {                                               // :252-265
+                const int32_t li = PickLight(spec, lc, vr.u[kDimLightPick], prob);
+                // the EM belongs to the background light, so a scene with one light has no other
+                if (li >= 0 && (spec.numLights == 1 || li == S.bgLight)) {
+                    emq = nee = true;
+                    qn = fr.z;
+                    qsides = mat.sides;
+                    u0 = vr.u[kDimLight0];
+                    u1 = vr.u[kDimLight1];
+                    uSide = vr.u[kDimLight2];
+                } else if (li >= 0) {
+                    LightSample ls;
+                    if (SampleIllumination(S, spec, (uint32_t)li, sp, fr, mat.sides, vr.u[kDimLight0],
+                                           vr.u[kDimLight1], vr.u[kDimLight2], ls)) {
+                        ls.prob = prob;
+                        LightSampleRequest(ps, mat, fr, wol, sp, ls, sr);
+                    }
+                }
+            }
+        }
+    } else {
+        emq = len >= S.minPL && S.bgLight >= 0;
+    }
+    F3 Le = mk(0.f);
+    float emPdf = 0.f;   // escaped lane: pdf of its direction under the previous vertex's EM sampling
+    LightSample ls;
+    bool lsOk = false;
+    if (emq) {
+        const bool front = qsides & kSideFront, back = qsides & kSideBack;
+        bool ok = front || back;
+        float ui = 0.f, li = 0.f, wi = 0.f;
+        bool up = front;
+        if (front && back) {
+            ui = WholeIntegral(E, ClampedCos(qn));
+            li = WholeIntegral(E, ClampedCos(-qn));
+            wi = ui + li;
+            if (IsTiny(wi)) ok = false;
+            up = nee ? (uSide * wi < ui) : (Dot(qdir, qn) >= 0.f);
+        }
+        const F3 hn = up ? qn : -qn;
+        const SH9 c = ClampedCos(hn);
+        const float whole = WholeIntegral(E, c);
+        float triPdf = 0.f, cs = 0.f;
+        bool flip = false;
+        if (nee && ok) {
+            float val;
+            const F3 sdir = SteerTriangleSample(E, c, u0, u1, spec.tree, val);
+            triPdf = IsTiny(whole) ? 0.f : FastDiv(val, whole);
+            cs = Dot(sdir, hn);
+            flip = cs < 0.f;
+            qdir = sdir;
+            if (flip) { qdir = -sdir; cs *= -1.f; }
+        }
+        float u, v, uo, vo;
+        Dir2LatLong(qdir, u, v);
+        Le = EmEvalUV(E, u, v);
+        Antipode(u, v, uo, vo);
+        const float lumOpp = EmLumUV(E, uo, vo);
+        // GetBasePdf of qdir and of -qdir; a light sample needs the one of -(sampled direction)
+        const float t1 = BasePdfLum(E, qdir, Luminance(Le), c);
+        const float t2 = BasePdfLum(E, -qdir, lumOpp, c);
+        if (nee) {
+            lsOk = ok;
+            float pdf = triPdf;
+            pdf += flip ? t1 : t2;
+            if (front && back) pdf *= FastDiv(up ? ui : li, wi);
+            ls.wig = qdir;
+            ls.pdfW = pdf;
+            ls.sample = Le * cs;
+            ls.prob = prob;
+            ls.dist = 3.402823466e+38f;   // em_sampler.hxx:57
+        } else if (ok) {
+            const float p = (Dot(qdir, hn) < 0.f) ? 0.f : t1 + t2;
+            emPdf = (front && back) ? p * FastDiv(up ? ui : li, wi) : p;
+        }
+    }
+    if (!hit) {                                                                       // :408-422
+        if (emq) {
+            if (!(ps.flags & kHasPrev)) {
+                ps.L += Le;
+            } else if (!IsZero(Le)) {
+                if (ps.flags & kPrevFinite) {
+                    const float lp = emPdf * PrevPickProb(S, spec, key, ps, (uint32_t)S.bgLight);
+                    ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
+                } else {
+                    ps.L += ps.eThr * Le;
+                }
+            }
+        }
+        return false;
+    }
+    if (!shade) return false;
+    if (lsOk) LightSampleRequest(ps, mat, fr, wol, sp, ls, sr);
+    return ContinuePath(S, ps, sp, fr, wol, mat, vr, lc.sum);
+}
+
 // One vertex of EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with one light sample and
 // one BSDF sample: adds the MIS-weighted emission found by the previous BSDF sample, emits the
 // light-sample request and samples the BSDF.  Returns true if the path continues.
+// kMergeEm: kernels whose only light is a steerable EM take ShadeVertexSteerable (with several
+// lights its extra live state costs more register spill than it saves).
+template <bool kMergeEm = false>
 PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey &key, PathState &ps, bool hit, const Hit &h,
                          ShadowReq &sr, uint32_t &segments) {
     sr.valid = false;
+    if constexpr (kMergeEm) return ShadeVertexSteerable(S, spec, key, ps, hit, h, sr, segments);
     const uint32_t len = ps.len;
     const bool hasPrev = ps.flags & kHasPrev, prevFinite = ps.flags & kPrevFinite;
     if (!hit) {
@@ -741,7 +958,7 @@ PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &sam
                 h.dist = 1e36f;
                 const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
                 ShadowReq sr;
-                const bool cont = ShadeVertex(S, spec, key, ps, hit, h, sr, segments);
+                const bool cont = ShadeVertex<Src::kMergeEm>(S, spec, key, ps, hit, h, sr, segments);
                 if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
                 if (!cont) {
                     double *a = acc + slot * 192u + pix * 3u;
@@ -774,7 +991,9 @@ PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &sam
 }
 
 // Item source of one scene: items chunk * numTiles + tile from the launch's work counter.
+template <bool kMerge>
 struct SceneItems {
+    static constexpr bool kMergeEm = kMerge;
     const SceneView *S;
     LightSpec spec;
     uint32_t *work;
