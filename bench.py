@@ -42,13 +42,35 @@ def parse():
     return ap.parse_args()
 
 
+def host_threads():
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():       # the host share given to this job
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    return max(1, threads)
+
+
+def steerable_build_measure():
+    """Host build of the steerable tables of the 1024x512 EM of the default workload at the
+    reference's default build parameters (levels 4..9, max approximation error 5), on one thread
+    and on the host threads (the tables are identical)."""
+    from pg3render_amd import SceneBuild, steerable_build
+    sb = SceneBuild.workload(2, seed=1)
+    sb.desc.em.st_min_subdiv_level = -1
+    sb.desc.em.st_max_subdiv_level = -1
+    sb.desc.em.st_max_approx_error = -1
+    one, _, _ = steerable_build(sb.desc.em, threads=1, leaves=False)
+    n = host_threads()
+    many, _, _ = steerable_build(sb.desc.em, threads=n, leaves=False)
+    return {"em": "1024x512", "levels": "4..9, max error 5 (reference defaults)", "leaves": one["num_leaves"],
+            "ms_1_thread": round(one["build_ms"], 1), "threads": n, "ms_threads": round(many["build_ms"], 1),
+            "speedup": round(one["build_ms"] / max(many["build_ms"], 1e-6), 2)}
+
+
 def cpu_baseline(sb, spp, target_s):
     """Times the CPU oracle (all host threads) on a bounded band of image rows at full spp."""
     from oracle import oracle as O
     w, h = sb.resolution
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    if os.environ.get("OMP_NUM_THREADS", "").isdigit():       # the host share given to this job
-        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    threads = host_threads()
     orc = O.OracleScene(sb)
     # calibrate on 4 rows, then size the band to about target_s seconds
     probe_rows = 4
@@ -236,6 +258,7 @@ def main():
             inv = inverse_rendering(ctx, steps=args.inverse_steps)
             inv.pop("loss_history")
             out["inverse_rendering"] = inv
+    out["steerable_build"] = steerable_build_measure()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sb, spp, args.cpu_seconds)
     print(json.dumps(out))

@@ -218,6 +218,22 @@ int pg3g_steerable_leaves(pg3g_context *ctx, float *leaf_dirs);
 /* Root weight (9 SH floats) = sum of all leaf weights grouped as the 6-ary tree does. */
 int pg3g_steerable_root_weight(pg3g_context *ctx, float *weight9);
 
+/* Host-only build of the steerable sampler tables of an EM (no context, no GPU): the
+ * triangulation refined by the approximation-error test and the 6-ary tree.  Replaces
+ * EnvironmentMapSteerableSampler::Init's build (TriangulateEm / RefineEmTriangulation /
+ * BuildTriangleTree, src/em_steerable_sampler.hxx:2963-3472, 3755-3793), which refines one
+ * triangle at a time; here each level's subdivision tests run on num_threads host threads
+ * (0 = PG3G_BUILD_THREADS, else OMP_NUM_THREADS, else the hardware threads), with the same
+ * tables for any thread count.  The same build runs inside pg3g_upload_scene.  Outputs may be
+ * null: leaf_dirs needs num_leaves*9 floats (leaf triangles in tree order, as
+ * pg3g_steerable_leaves), root_weight9 9 floats; build_ms receives the wall time of the build. */
+typedef struct pg3g_steerable_build_info {
+    uint32_t num_leaves, num_vertices, num_levels, threads;
+    double   build_ms;
+} pg3g_steerable_build_info;
+int pg3g_steerable_build(const pg3g_em_desc *em, uint32_t num_threads, pg3g_steerable_build_info *info,
+                         float *leaf_dirs, uint32_t leaf_capacity, float *root_weight9);
+
 /* ---- batched multi-scene rendering ------------------------------------------------------ */
 /* No counterpart in the reference, which renders one scene per process (src/pg3render.cxx:
  * 54-174).  A batch renders many uploaded slots in ONE persistent launch per pass: every scene

@@ -6,6 +6,7 @@ raises.
 """
 from . import _ffi
 from .diff import DiffScene, render_function
-from .renderer import Batch, Context, GpuPathTracer, SceneBuild
+from .renderer import Batch, Context, GpuPathTracer, SceneBuild, steerable_build
 
-__all__ = ["Batch", "Context", "DiffScene", "GpuPathTracer", "SceneBuild", "_ffi", "render_function"]
+__all__ = ["Batch", "Context", "DiffScene", "GpuPathTracer", "SceneBuild", "_ffi", "render_function",
+           "steerable_build"]

@@ -70,6 +70,11 @@ class SceneDesc(C.Structure):
                 ("camera", CameraDesc), ("params", RenderParams)]
 
 
+class SteerableBuildInfo(C.Structure):
+    _fields_ = [("num_leaves", C.c_uint32), ("num_vertices", C.c_uint32), ("num_levels", C.c_uint32),
+                ("threads", C.c_uint32), ("build_ms", C.c_double)]
+
+
 class BatchConfig(C.Structure):
     _fields_ = [("iters_per_item", C.c_uint32), ("waves_per_cu", C.c_uint32), ("use_graph", C.c_uint32)]
 
@@ -107,6 +112,8 @@ _SIGNATURES = {
     "pg3g_steerable_info": (C.c_int, [_P, _U32P, _U32P, _U32P]),
     "pg3g_steerable_leaves": (C.c_int, [_P, _FP]),
     "pg3g_steerable_root_weight": (C.c_int, [_P, _FP]),
+    "pg3g_steerable_build": (C.c_int, [C.POINTER(EmDesc), C.c_uint32, C.POINTER(SteerableBuildInfo), _FP, C.c_uint32,
+                                       _FP]),
     "pg3g_batch_create": (C.c_int, [_P, C.c_uint32, _U32P, _U32P, C.POINTER(_P)]),
     "pg3g_batch_destroy": (C.c_int, [_P]),
     "pg3g_batch_set_config": (C.c_int, [_P, C.POINTER(BatchConfig)]),

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -923,6 +924,37 @@ int pg3g_steerable_root_weight(pg3g_context *ctx, float *weight9) {
     if (!S || !S->hasSteer) return Fail(PG3G_ERR_NO_SCENE, "slot 0 has no steerable sampler");
     if (!weight9) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
     for (int j = 0; j < 9; ++j) weight9[j] = S->steer.rootW[j];
+    return PG3G_OK;
+}
+
+int pg3g_steerable_build(const pg3g_em_desc *em, uint32_t num_threads, pg3g_steerable_build_info *info,
+                         float *leaf_dirs, uint32_t leaf_capacity, float *root_weight9) {
+    if (!em || !em->texels || em->width == 0 || em->height == 0) return Fail(PG3G_ERR_INVALID_ARG, "no EM image");
+    const uint32_t mn = em->st_min_subdiv_level >= 0 ? (uint32_t)em->st_min_subdiv_level : 4u;
+    const uint32_t mx = em->st_max_subdiv_level >= 0 ? (uint32_t)em->st_max_subdiv_level : 9u;
+    const uint32_t threads = num_threads ? num_threads : SteerableBuildThreads();
+    SteerableTables t;
+    std::string err;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!BuildSteerable(em->texels, em->width, em->height, em->bilinear != 0, em->st_max_approx_error, mn, mx, t, err,
+                        threads))
+        return Fail(PG3G_ERR_INVALID_ARG, err);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (info) {
+        info->num_leaves = t.numLeaves;
+        info->num_vertices = t.numVertices;
+        info->num_levels = t.numLevels;
+        info->threads = threads;
+        info->build_ms = ms;
+    }
+    if (leaf_dirs) {
+        if (leaf_capacity < t.numLeaves) return Fail(PG3G_ERR_INVALID_ARG, "leaf_dirs holds fewer leaves than the build");
+        for (uint32_t i = 0; i < t.numLeaves; ++i)
+            for (int k = 0; k < 3; ++k)
+                for (int c = 0; c < 3; ++c) leaf_dirs[(size_t)i * 9 + k * 3 + c] = t.leafVerts[(size_t)i * 36 + k * 12 + c];
+    }
+    if (root_weight9)
+        for (int j = 0; j < 9; ++j) root_weight9[j] = t.rootW[j];
     return PG3G_OK;
 }
 

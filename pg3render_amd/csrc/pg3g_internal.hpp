@@ -78,8 +78,11 @@ struct SteerableTables {
     std::vector<float> leafVerts;                  // 36 floats per leaf (3 x (dir, SH weight))
     float rootW[9];
 };
-// Builds the tables from an RGB image (w*h*3 floats) with tent or box filtering.
+// Builds the tables from an RGB image (w*h*3 floats) with tent or box filtering on `threads`
+// host threads (0: SteerableBuildThreads()); the tables do not depend on the thread count.
 bool BuildSteerable(const float *texels, uint32_t w, uint32_t h, bool bilinear, float maxErr, uint32_t minLevel,
-                    uint32_t maxLevel, SteerableTables &out, std::string &err);
+                    uint32_t maxLevel, SteerableTables &out, std::string &err, uint32_t threads = 0);
+// PG3G_BUILD_THREADS, else OMP_NUM_THREADS, else the hardware threads (at most 64).
+uint32_t SteerableBuildThreads();
 
 }  // namespace pg3

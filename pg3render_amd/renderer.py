@@ -83,6 +83,25 @@ class SceneBuild:
             pass
 
 
+def steerable_build(em_desc, threads=0, leaves=True):
+    """Host-only build of the steerable sampler tables of `em_desc` (an EmDesc, e.g.
+    SceneBuild.desc.em): returns (info dict, leaf directions (n, 3, 3) or None, root weight (9,))."""
+    lib = _ffi.load()
+    info = _ffi.SteerableBuildInfo()
+    root = np.zeros(9, np.float32)
+    check(lib.pg3g_steerable_build(C.byref(em_desc), threads, C.byref(info), None, 0,
+                                   root.ctypes.data_as(C.POINTER(C.c_float))), "pg3g_steerable_build")
+    out = None
+    if leaves:
+        out = np.zeros((info.num_leaves, 3, 3), np.float32)
+        check(lib.pg3g_steerable_build(C.byref(em_desc), threads, C.byref(info),
+                                       out.ctypes.data_as(C.POINTER(C.c_float)), info.num_leaves, None),
+              "pg3g_steerable_build")
+    d = {"num_leaves": int(info.num_leaves), "num_vertices": int(info.num_vertices),
+         "num_levels": int(info.num_levels), "threads": int(info.threads), "build_ms": float(info.build_ms)}
+    return d, out, root
+
+
 class Context:
     """A libpg3g context on one HIP device."""
 
