@@ -229,10 +229,46 @@ int pg3g_steerable_root_weight(pg3g_context *ctx, float *weight9);
  * pg3g_steerable_leaves), root_weight9 9 floats; build_ms receives the wall time of the build. */
 typedef struct pg3g_steerable_build_info {
     uint32_t num_leaves, num_vertices, num_levels, threads;
+    uint32_t from_cache;         /* 1: loaded from the steerable cache (pg3g_steerable_cache_dir) */
     double   build_ms;
 } pg3g_steerable_build_info;
 int pg3g_steerable_build(const pg3g_em_desc *em, uint32_t num_threads, pg3g_steerable_build_info *info,
                          float *leaf_dirs, uint32_t leaf_capacity, float *root_weight9);
+
+/* Directory of the on-disk cache of steerable tables (null or "": none; default: the
+ * PG3G_STEERABLE_CACHE_DIR environment variable).  Replaces the reference's .emssd files
+ * (EnvironmentMapSteerableSampler::SaveToDisk / LoadFromDisk, src/em_steerable_sampler.hxx:
+ * 1596-2030), which are keyed by the EM file name: entries here are keyed by a hash of the
+ * texels, the image size and filter, and every build parameter.  Used by pg3g_upload_scene,
+ * pg3g_diff_set_params (refreshed sampling) and pg3g_steerable_build.  Process-wide. */
+int pg3g_steerable_cache_dir(const char *dir);
+
+/* ---- image and environment-map files (host only) ---------------------------------------- */
+/* Reads a Radiance RGBE (.hdr: flat or run-length scanlines, -Y h +X w), PFM or raw float image
+ * (uint32 width, uint32 height, width*height RGB float32 top row first) into row-major RGB
+ * floats, top row first.  rgb may be null to query the size; capacity counts floats. */
+int pg3g_load_image(const char *path, uint32_t *width, uint32_t *height, float *rgb, uint64_t capacity);
+/* EnvironmentMapImage::LoadImage (src/em_image.hxx:51-93) without OpenEXR: an image as
+ * pg3g_load_image, rotated in azimuth by azimuth_rotation (fraction of a turn, taken modulo 1 as
+ * Math::FmodX) and multiplied by scale. */
+int pg3g_load_em(const char *path, float azimuth_rotation, float scale, uint32_t *width, uint32_t *height,
+                 float *rgb, uint64_t capacity);
+/* Framebuffer::SaveHDR / SaveBMP (src/frame_buffer.hxx:114-200) of an RGB FP64 image (row-major,
+ * top row first) times scale (1 / iterations for pg3g_read_framebuffer sums).  The format follows
+ * the extension: .hdr (RGBE, flat scanlines) or .bmp (24-bit, value^(1/gamma), clamped). */
+int pg3g_save_image(const char *path, const double *rgb, uint32_t width, uint32_t height, double scale,
+                    float gamma);
+/* Per-pixel comparison of two RGB images on luminance, the check of the reference's regression
+ * script (testing/full_test.sh:97, an external diff tool in relative-RMSE mode):
+ * pixel_rel_rmse = sqrt(mean(((L - Lref) / max(|Lref|, 1e-3 mean|Lref|))^2)),
+ * image_rel_rmse = sqrt(sum (L - Lref)^2 / sum Lref^2).  diff_rgb (may be null) receives
+ * |image - reference| per channel. */
+typedef struct pg3g_image_diff {
+    double pixel_rel_rmse, image_rel_rmse, max_pixel_rel;
+    double mean_luminance, mean_luminance_reference;
+} pg3g_image_diff;
+int pg3g_image_compare(const float *image, const float *reference, uint32_t width, uint32_t height,
+                       pg3g_image_diff *out, float *diff_rgb);
 
 /* ---- batched multi-scene rendering ------------------------------------------------------ */
 /* No counterpart in the reference, which renders one scene per process (src/pg3render.cxx:

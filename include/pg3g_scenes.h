@@ -33,11 +33,19 @@ typedef struct pg3g_scene_build pg3g_scene_build;
 int pg3g_build_workload_scene(int workload, uint64_t seed, uint32_t variant, uint32_t res_override,
                               pg3g_scene_build **out);
 
-/* The reference's Cornell-box scene IDs 0..37 (src/config.hxx:45-115, scene.hxx:176-956) with the
- * constant environment maps kEMConstBluish (0) and kEMConstSRGBWhite (1); image EMs are loaded by
- * the caller.  resolution 0 = 512x512 (config.hxx:568). */
+/* The reference's Cornell-box scene IDs 0..37 (src/config.hxx:45-115, scene.hxx:176-956) with
+ * environment map em_type (scene.hxx:158-170, 830-943): the constant kEMConstBluish (0) and
+ * kEMConstSRGBWhite (1), and the constant white images 8x4 (2) and 1024x512 (3), which are built in
+ * memory.  Image EMs use the reference's default simple spherical sampler (hard_config.hxx:17),
+ * tent filtering on.  resolution 0 = 512x512 (config.hxx:568).  em_type 4..12 (photographs)
+ * needs pg3g_build_reference_scene_em. */
 int pg3g_build_reference_scene(uint32_t scene_id, uint32_t em_type, uint32_t resolution,
                                pg3g_scene_build **out);
+/* As pg3g_build_reference_scene, with the EM of em_type 2..12 read from em_path (Radiance .hdr,
+ * PFM or raw float; see pg3g_load_em) and rotated and scaled as the reference loads that type
+ * (LoadEnvironmentMap arguments, scene.hxx:850-943), in place of its OpenEXR file. */
+int pg3g_build_reference_scene_em(uint32_t scene_id, uint32_t em_type, uint32_t resolution, const char *em_path,
+                                  pg3g_scene_build **out);
 
 /* Description view of a built scene (valid until the build is freed; callers may edit params and
  * the material/light arrays in place through pg3g_scene_build_mutable). */

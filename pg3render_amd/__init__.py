@@ -6,7 +6,8 @@ raises.
 """
 from . import _ffi
 from .diff import DiffScene, render_function
-from .renderer import Batch, Context, GpuPathTracer, SceneBuild, steerable_build
+from .renderer import (Batch, Context, GpuPathTracer, SceneBuild, compare_images, load_image, save_image,
+                       steerable_build, steerable_cache_dir)
 
-__all__ = ["Batch", "Context", "DiffScene", "GpuPathTracer", "SceneBuild", "_ffi", "render_function",
-           "steerable_build"]
+__all__ = ["Batch", "Context", "DiffScene", "GpuPathTracer", "SceneBuild", "_ffi", "compare_images", "load_image",
+           "render_function", "save_image", "steerable_build", "steerable_cache_dir"]

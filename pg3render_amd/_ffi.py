@@ -72,7 +72,12 @@ class SceneDesc(C.Structure):
 
 class SteerableBuildInfo(C.Structure):
     _fields_ = [("num_leaves", C.c_uint32), ("num_vertices", C.c_uint32), ("num_levels", C.c_uint32),
-                ("threads", C.c_uint32), ("build_ms", C.c_double)]
+                ("threads", C.c_uint32), ("from_cache", C.c_uint32), ("build_ms", C.c_double)]
+
+
+class ImageDiff(C.Structure):
+    _fields_ = [("pixel_rel_rmse", C.c_double), ("image_rel_rmse", C.c_double), ("max_pixel_rel", C.c_double),
+                ("mean_luminance", C.c_double), ("mean_luminance_reference", C.c_double)]
 
 
 class BatchConfig(C.Structure):
@@ -114,6 +119,12 @@ _SIGNATURES = {
     "pg3g_steerable_root_weight": (C.c_int, [_P, _FP]),
     "pg3g_steerable_build": (C.c_int, [C.POINTER(EmDesc), C.c_uint32, C.POINTER(SteerableBuildInfo), _FP, C.c_uint32,
                                        _FP]),
+    "pg3g_steerable_cache_dir": (C.c_int, [C.c_char_p]),
+    "pg3g_load_image": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _FP, C.c_uint64]),
+    "pg3g_load_em": (C.c_int, [C.c_char_p, C.c_float, C.c_float, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _FP,
+                               C.c_uint64]),
+    "pg3g_save_image": (C.c_int, [C.c_char_p, _DP, C.c_uint32, C.c_uint32, C.c_double, C.c_float]),
+    "pg3g_image_compare": (C.c_int, [_FP, _FP, C.c_uint32, C.c_uint32, C.POINTER(ImageDiff), _FP]),
     "pg3g_batch_create": (C.c_int, [_P, C.c_uint32, _U32P, _U32P, C.POINTER(_P)]),
     "pg3g_batch_destroy": (C.c_int, [_P]),
     "pg3g_batch_set_config": (C.c_int, [_P, C.POINTER(BatchConfig)]),
@@ -128,6 +139,7 @@ _SIGNATURES = {
     "pg3g_adjoint": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _FP, _DP, _DP, _FP]),
     "pg3g_build_workload_scene": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "pg3g_build_reference_scene": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    "pg3g_build_reference_scene_em": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_char_p, C.POINTER(_P)]),
     "pg3g_scene_build_desc": (C.POINTER(SceneDesc), [_P]),
     "pg3g_scene_build_mutable": (C.POINTER(SceneDesc), [_P]),
     "pg3g_scene_build_em_texels": (_FP, [_P]),

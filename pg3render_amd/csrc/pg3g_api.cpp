@@ -418,7 +418,8 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
             std::string err;
             const uint32_t mn = d->em.st_min_subdiv_level >= 0 ? (uint32_t)d->em.st_min_subdiv_level : 4u;
             const uint32_t mx = d->em.st_max_subdiv_level >= 0 ? (uint32_t)d->em.st_max_subdiv_level : 9u;
-            if (!BuildSteerable(d->em.texels, E.w, E.h, E.bilinear != 0, d->em.st_max_approx_error, mn, mx, S.steer, err))
+            if (!BuildSteerableCached(d->em.texels, E.w, E.h, E.bilinear != 0, d->em.st_max_approx_error, mn, mx, S.steer,
+                                      err))
                 return Fail(PG3G_ERR_INVALID_ARG, err);
             S.hasSteer = true;
             PG3_HIP(Upload(S.nodeW, S.steer.nodeW.data(), S.steer.nodeW.size(), st));
@@ -480,7 +481,7 @@ int RebuildEmTables(pg3g_context *ctx, SceneSlot &S) {
         const uint32_t mn = d.st_min_subdiv_level >= 0 ? (uint32_t)d.st_min_subdiv_level : 4u;
         const uint32_t mx = d.st_max_subdiv_level >= 0 ? (uint32_t)d.st_max_subdiv_level : 9u;
         SteerableTables t;
-        if (!BuildSteerable(S.hostTexels.data(), E.w, E.h, E.bilinear != 0, d.st_max_approx_error, mn, mx, t, err))
+        if (!BuildSteerableCached(S.hostTexels.data(), E.w, E.h, E.bilinear != 0, d.st_max_approx_error, mn, mx, t, err))
             return Fail(PG3G_ERR_INVALID_ARG, err);
         S.steer = std::move(t);
         PG3_HIP(Upload(S.nodeW, S.steer.nodeW.data(), S.steer.nodeW.size(), ctx->stream));
@@ -936,8 +937,9 @@ int pg3g_steerable_build(const pg3g_em_desc *em, uint32_t num_threads, pg3g_stee
     SteerableTables t;
     std::string err;
     const auto t0 = std::chrono::steady_clock::now();
-    if (!BuildSteerable(em->texels, em->width, em->height, em->bilinear != 0, em->st_max_approx_error, mn, mx, t, err,
-                        threads))
+    bool cached = false;
+    if (!BuildSteerableCached(em->texels, em->width, em->height, em->bilinear != 0, em->st_max_approx_error, mn, mx, t,
+                              err, threads, &cached))
         return Fail(PG3G_ERR_INVALID_ARG, err);
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (info) {
@@ -945,6 +947,7 @@ int pg3g_steerable_build(const pg3g_em_desc *em, uint32_t num_threads, pg3g_stee
         info->num_vertices = t.numVertices;
         info->num_levels = t.numLevels;
         info->threads = threads;
+        info->from_cache = cached ? 1u : 0u;
         info->build_ms = ms;
     }
     if (leaf_dirs) {
@@ -955,6 +958,81 @@ int pg3g_steerable_build(const pg3g_em_desc *em, uint32_t num_threads, pg3g_stee
     }
     if (root_weight9)
         for (int j = 0; j < 9; ++j) root_weight9[j] = t.rootW[j];
+    return PG3G_OK;
+}
+
+int pg3g_steerable_cache_dir(const char *dir) {
+    SetSteerableCacheDir(dir);
+    return PG3G_OK;
+}
+
+int pg3g_load_image(const char *path, uint32_t *width, uint32_t *height, float *rgb, uint64_t capacity) {
+    if (!path || !width || !height) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    std::vector<float> img;
+    uint32_t w = 0, h = 0;
+    std::string err;
+    if (!ReadImageFile(path, img, w, h, err)) return Fail(PG3G_ERR_INVALID_ARG, err);
+    *width = w;
+    *height = h;
+    if (rgb) {
+        if (capacity < img.size()) return Fail(PG3G_ERR_INVALID_ARG, "image buffer too small");
+        std::memcpy(rgb, img.data(), img.size() * sizeof(float));
+    }
+    return PG3G_OK;
+}
+
+int pg3g_load_em(const char *path, float azimuth_rotation, float scale, uint32_t *width, uint32_t *height, float *rgb,
+                 uint64_t capacity) {
+    if (!path || !width || !height) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    std::vector<float> img;
+    uint32_t w = 0, h = 0;
+    std::string err;
+    if (!LoadEmFile(path, azimuth_rotation, scale, img, w, h, err)) return Fail(PG3G_ERR_INVALID_ARG, err);
+    *width = w;
+    *height = h;
+    if (rgb) {
+        if (capacity < img.size()) return Fail(PG3G_ERR_INVALID_ARG, "image buffer too small");
+        std::memcpy(rgb, img.data(), img.size() * sizeof(float));
+    }
+    return PG3G_OK;
+}
+
+int pg3g_save_image(const char *path, const double *rgb, uint32_t width, uint32_t height, double scale, float gamma) {
+    if (!path || !rgb || width == 0 || height == 0) return Fail(PG3G_ERR_INVALID_ARG, "bad image arguments");
+    std::string err;
+    if (!WriteImageFile(path, rgb, width, height, scale, gamma, err)) return Fail(PG3G_ERR_INVALID_ARG, err);
+    return PG3G_OK;
+}
+
+int pg3g_image_compare(const float *image, const float *reference, uint32_t width, uint32_t height,
+                       pg3g_image_diff *out, float *diff_rgb) {
+    if (!image || !reference || !out || width == 0 || height == 0) return Fail(PG3G_ERR_INVALID_ARG, "bad arguments");
+    const size_t n = (size_t)width * height;
+    auto lum = [](const float *p) { return 0.212671 * p[0] + 0.715160 * p[1] + 0.072169 * p[2]; };
+    double meanRef = 0, meanImg = 0;
+    for (size_t i = 0; i < n; ++i) {
+        meanRef += std::fabs(lum(reference + i * 3));
+        meanImg += std::fabs(lum(image + i * 3));
+    }
+    meanRef /= (double)n;
+    meanImg /= (double)n;
+    const double floor = std::max(1e-3 * meanRef, 1e-12);
+    double sumRel2 = 0, sumAbs2 = 0, sumRef2 = 0, maxRel = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const double a = lum(image + i * 3), b = lum(reference + i * 3), d = a - b;
+        const double rel = d / std::max(std::fabs(b), floor);
+        sumRel2 += rel * rel;
+        sumAbs2 += d * d;
+        sumRef2 += b * b;
+        maxRel = std::max(maxRel, std::fabs(rel));
+        if (diff_rgb)
+            for (int c = 0; c < 3; ++c) diff_rgb[i * 3 + c] = std::fabs(image[i * 3 + c] - reference[i * 3 + c]);
+    }
+    out->pixel_rel_rmse = std::sqrt(sumRel2 / (double)n);
+    out->image_rel_rmse = sumRef2 > 0 ? std::sqrt(sumAbs2 / sumRef2) : std::sqrt(sumAbs2 / (double)n);
+    out->max_pixel_rel = maxRel;
+    out->mean_luminance = meanImg;
+    out->mean_luminance_reference = meanRef;
     return PG3G_OK;
 }
 

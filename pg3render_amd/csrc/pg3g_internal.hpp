@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "dev_scene.hpp"
+#include "image_io.hpp"
 
 namespace pg3 {
 
@@ -84,5 +85,12 @@ bool BuildSteerable(const float *texels, uint32_t w, uint32_t h, bool bilinear, 
                     uint32_t maxLevel, SteerableTables &out, std::string &err, uint32_t threads = 0);
 // PG3G_BUILD_THREADS, else OMP_NUM_THREADS, else the hardware threads (at most 64).
 uint32_t SteerableBuildThreads();
+// BuildSteerable through the on-disk cache (steerable_cache.cpp) when a cache directory is set
+// (SetSteerableCacheDir, else PG3G_STEERABLE_CACHE_DIR); *fromCache tells whether it was loaded.
+bool BuildSteerableCached(const float *texels, uint32_t w, uint32_t h, bool bilinear, float maxErr, uint32_t minLevel,
+                          uint32_t maxLevel, SteerableTables &out, std::string &err, uint32_t threads = 0,
+                          bool *fromCache = nullptr);
+void SetSteerableCacheDir(const char *dir);   // null or "": no cache
+
 
 }  // namespace pg3

@@ -9,6 +9,7 @@
 
 #include "pg3g.h"
 #include "pg3g_scenes.h"
+#include "image_io.hpp"
 
 struct pg3g_scene_build {
     std::vector<float> tris, sphs, em;
@@ -422,12 +423,22 @@ pg3g_material Dielectric(int type, float alpha, float inner, float outer, bool b
     return m;
 }
 
-int BuildReference(pg3g_scene_build &b, uint32_t id, uint32_t em, uint32_t res) {
+// Image EMs of the reference scenes (scene.hxx:850-943): azimuth rotation and scale given to
+// LoadEnvironmentMap for em types 2..12.  Types 2 and 3 are constant white images (1.0 in every
+// texel, 8x4 and 1024x512) and are built in memory; the others are photographs loaded from a file.
+struct RefEm { float rotation, scale; uint32_t w, h; };
+const RefEm kRefEms[13] = {{0, 0, 0, 0},       {0, 0, 0, 0},      {0.f, 1.f, 8, 4},   {0.f, 1.f, 1024, 512},
+                           {0.40f, 10.f, 0, 0}, {0.05f, 1.f, 0, 0}, {0.05f, 1.f, 0, 0}, {0.83f, 1.5f, 0, 0},
+                           {0.1f, 2.f, 0, 0},  {0.53f, 0.2f, 0, 0}, {-0.12f, 0.5f, 0, 0}, {-0.12f, 3.f, 0, 0},
+                           {0.f, 7500.f, 0, 0}};
+
+int BuildReference(pg3g_scene_build &b, uint32_t id, uint32_t em, uint32_t res, const char *emPath = nullptr) {
     if (id >= sizeof(kSceneConfigs) / sizeof(kSceneConfigs[0])) return PG3G_ERR_INVALID_ARG;
     const uint32_t M = kSceneConfigs[id];
     if (M & (kSpheresSraLayers | kLayeredSphere)) return PG3G_ERR_UNSUPPORTED;
     const bool ceiling = M & kLightCeiling, box = M & kLightBox, point = M & kLightPoint, env = M & kLightEnv;
-    if (env && em > 1) return PG3G_ERR_UNSUPPORTED;   // image EMs come from EXR files in the reference
+    if (env && em > 12) return PG3G_ERR_INVALID_ARG;
+    if (env && em > 3 && !emPath) return PG3G_ERR_UNSUPPORTED;   // photographs: the caller names the file
     const uint32_t n = res ? res : 512;
     const float pos[3] = {-0.0439815f, -4.12529f, 0.222539f}, fwd[3] = {0.00688625f, 0.998505f, -0.0542161f},
                 up[3] = {3.73896e-4f, 0.0542148f, 0.998529f};
@@ -537,7 +548,22 @@ int BuildReference(pg3g_scene_build &b, uint32_t id, uint32_t em, uint32_t res) 
     }
     if (env) {
         if (em == 0) b.lights.push_back(Background(135 / 255.f, 206 / 255.f, 250 / 255.f));
-        else b.lights.push_back(Background(0.87f, 0.87f, 0.87f));
+        else if (em == 1) b.lights.push_back(Background(0.87f, 0.87f, 0.87f));
+        else {
+            const RefEm &r = kRefEms[em];
+            uint32_t w = r.w, h = r.h;
+            if (emPath) {
+                std::string err;
+                if (!pg3::LoadEmFile(emPath, r.rotation, r.scale, b.em, w, h, err)) return PG3G_ERR_INVALID_ARG;
+            } else {
+                b.em.assign((size_t)w * h * 3, 1.f * r.scale);
+            }
+            b.lights.push_back(Background(0.f, 0.f, 0.f));
+            b.desc.em.width = w;
+            b.desc.em.height = h;
+            b.desc.em.bilinear = 1;                     // LoadEnvironmentMap default (lights.hxx:372)
+            b.desc.em.sampler = PG3G_EM_SIMPLE;         // the reference's default sampler (hard_config.hxx:17)
+        }
     }
     SetParams(b, 0, 1234, 0);
     b.desc.params.indirect_clipping = 16.f;   // Config defaults, config.hxx:567-578
@@ -568,6 +594,19 @@ int pg3g_build_reference_scene(uint32_t scene_id, uint32_t em_type, uint32_t res
     pg3g_scene_build *b = new (std::nothrow) pg3g_scene_build();
     if (!b) return PG3G_ERR_OUT_OF_MEMORY;
     const int rc = BuildReference(*b, scene_id, em_type, resolution);
+    if (rc != PG3G_OK) { delete b; return rc; }
+    b->Finish();
+    *out = b;
+    return PG3G_OK;
+}
+
+int pg3g_build_reference_scene_em(uint32_t scene_id, uint32_t em_type, uint32_t resolution, const char *em_path,
+                                  pg3g_scene_build **out) {
+    if (!out) return PG3G_ERR_INVALID_ARG;
+    *out = nullptr;
+    pg3g_scene_build *b = new (std::nothrow) pg3g_scene_build();
+    if (!b) return PG3G_ERR_OUT_OF_MEMORY;
+    const int rc = BuildReference(*b, scene_id, em_type, resolution, em_path);
     if (rc != PG3G_OK) { delete b; return rc; }
     b->Finish();
     *out = b;

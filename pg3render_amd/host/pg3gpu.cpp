@@ -5,8 +5,11 @@
 // divided by the iteration count exactly as GetFramebuffer does (src/renderer.hxx:255-261).
 //
 //   pg3gpu [-s <scene_id>] [-w <workload>] [-a <pt|ptn|dbs|dlsa|dlss|dmis>] [-i <iterations>]
-//          [-minpl <n>] [-maxpl <n>] [-iic <f>] [-sb <f>] [-em <type>] [-r <resolution>]
-//          [-seed <n>] [-k <megakernel|wavefront>] [-o <file.hdr>]
+//          [-minpl <n>] [-maxpl <n>] [-iic <f>] [-sb <f>] [-em <type>] [-emfile <file>]
+//          [-r <resolution>] [-seed <n>] [-k <megakernel|wavefront>] [-o <file.hdr|file.bmp>]
+//          [-gamma <g>]
+// -emfile names the image of an EM type 4..12 (Radiance .hdr, PFM or raw float) in place of the
+// reference's OpenEXR light probe; it is rotated and scaled as the reference loads that type.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -28,30 +31,6 @@ int Check(int rc, const char *what) {
     return rc;
 }
 
-// Radiance RGBE writer without run-length encoding, as Framebuffer::SaveHDR.
-bool SaveHdr(const char *path, const std::vector<double> &img, uint32_t w, uint32_t h) {
-    FILE *f = std::fopen(path, "wb");
-    if (!f) return false;
-    std::fprintf(f, "#?RADIANCE\n# pg3gpu\nFORMAT=32-bit_rle_rgbe\n\n-Y %u +X %u\n", h, w);
-    for (uint32_t y = 0; y < h; ++y)
-        for (uint32_t x = 0; x < w; ++x) {
-            const double *p = &img[((size_t)y * w + x) * 3];
-            const double v = std::fmax(p[0], std::fmax(p[1], p[2]));
-            unsigned char rgbe[4] = {0, 0, 0, 0};
-            if (v >= 1e-32) {
-                int e;
-                const double m = std::frexp(v, &e) * 256.0 / v;
-                rgbe[0] = (unsigned char)(p[0] * m);
-                rgbe[1] = (unsigned char)(p[1] * m);
-                rgbe[2] = (unsigned char)(p[2] * m);
-                rgbe[3] = (unsigned char)(e + 128);
-            }
-            std::fwrite(rgbe, 1, 4, f);
-        }
-    std::fclose(f);
-    return true;
-}
-
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -61,6 +40,8 @@ int main(int argc, char **argv) {
     double iic = -1, sb = -1;
     unsigned long long seed = 1234;
     std::string out = "pg3gpu.hdr";
+    const char *emFile = nullptr;
+    float gamma = 2.2f;
     const char *acronyms[] = {"el", "nv", "dbs", "dlsa", "dlss", "dmis", "ptn", "pt"};
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -79,6 +60,8 @@ int main(int argc, char **argv) {
         else if (a == "-iic") iic = std::atof(need());
         else if (a == "-sb") sb = std::atof(need());
         else if (a == "-em") emType = (uint32_t)std::atol(need());
+        else if (a == "-emfile") emFile = need();
+        else if (a == "-gamma") gamma = (float)std::atof(need());
         else if (a == "-r") res = (uint32_t)std::atol(need());
         else if (a == "-seed") seed = std::strtoull(need(), nullptr, 10);
         else if (a == "-k") kernel = (std::string(need()) == "wavefront") ? PG3G_KERNEL_WAVEFRONT : PG3G_KERNEL_MEGAKERNEL;
@@ -87,6 +70,9 @@ int main(int argc, char **argv) {
     }
     pg3g_scene_build *sb_ = nullptr;
     if (workload > 0) Check(pg3g_build_workload_scene(workload, seed, 0, res, &sb_), "pg3g_build_workload_scene");
+    else if (emFile)
+        Check(pg3g_build_reference_scene_em(sceneId < 0 ? 2u : (uint32_t)sceneId, emType, res, emFile, &sb_),
+              "pg3g_build_reference_scene_em");
     else Check(pg3g_build_reference_scene(sceneId < 0 ? 2u : (uint32_t)sceneId, emType, res, &sb_), "pg3g_build_reference_scene");
     pg3g_scene_desc *d = pg3g_scene_build_mutable(sb_);
     pg3g_render_params &p = d->params;
@@ -110,7 +96,8 @@ int main(int argc, char **argv) {
     uint64_t it = 0;
     Check(pg3g_read_framebuffer(ctx, img.data(), &it), "pg3g_read_framebuffer");
     for (double &v : img) v /= (double)(it ? it : 1);
-    const bool ok = SaveHdr(out.c_str(), img, w, h);
+    const bool ok = pg3g_save_image(out.c_str(), img.data(), w, h, 1.0, gamma) == PG3G_OK;
+    if (!ok) std::fprintf(stderr, "pg3gpu: %s\n", pg3g_last_error());
     double mean = 0;
     for (double v : img) mean += v;
     mean /= (double)img.size();

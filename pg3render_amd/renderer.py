@@ -6,6 +6,7 @@
 `was_used()` is `WasUsed`.  Scenes come from the C++ builders in libpg3g (include/pg3g_scenes.h).
 """
 import ctypes as C
+import os
 import weakref
 
 import numpy as np
@@ -29,10 +30,15 @@ class SceneBuild:
         return cls(h)
 
     @classmethod
-    def reference(cls, scene_id, em_type=0, resolution=0):
+    def reference(cls, scene_id, em_type=0, resolution=0, em_path=None):
+        """Reference scene ID; em_path names the image of an EM type 2..12 (.hdr, PFM or raw)."""
         lib = _ffi.load()
         h = C.c_void_p()
-        check(lib.pg3g_build_reference_scene(scene_id, em_type, resolution, C.byref(h)), "pg3g_build_reference_scene")
+        if em_path is None:
+            check(lib.pg3g_build_reference_scene(scene_id, em_type, resolution, C.byref(h)), "pg3g_build_reference_scene")
+        else:
+            check(lib.pg3g_build_reference_scene_em(scene_id, em_type, resolution, os.fsencode(em_path), C.byref(h)),
+                  "pg3g_build_reference_scene_em")
         return cls(h)
 
     @property
@@ -83,6 +89,51 @@ class SceneBuild:
             pass
 
 
+def steerable_cache_dir(path):
+    """Sets the directory of the on-disk steerable-table cache (None: no cache)."""
+    check(_ffi.load().pg3g_steerable_cache_dir(None if path is None else os.fsencode(path)), "pg3g_steerable_cache_dir")
+
+
+def load_image(path, em_rotation=None, em_scale=1.0):
+    """RGB float32 image (h, w, 3) from a Radiance .hdr, PFM or raw float file; with em_rotation
+    set, rotated in azimuth and scaled as the reference loads an environment map."""
+    lib = _ffi.load()
+    w, h = C.c_uint32(), C.c_uint32()
+    p = os.fsencode(path)
+    if em_rotation is None:
+        check(lib.pg3g_load_image(p, C.byref(w), C.byref(h), None, 0), "pg3g_load_image")
+        img = np.zeros((h.value, w.value, 3), np.float32)
+        check(lib.pg3g_load_image(p, C.byref(w), C.byref(h), img.ctypes.data_as(C.POINTER(C.c_float)), img.size),
+              "pg3g_load_image")
+    else:
+        check(lib.pg3g_load_em(p, em_rotation, em_scale, C.byref(w), C.byref(h), None, 0), "pg3g_load_em")
+        img = np.zeros((h.value, w.value, 3), np.float32)
+        check(lib.pg3g_load_em(p, em_rotation, em_scale, C.byref(w), C.byref(h),
+                               img.ctypes.data_as(C.POINTER(C.c_float)), img.size), "pg3g_load_em")
+    return img
+
+
+def save_image(path, rgb, scale=1.0, gamma=1.0):
+    """Writes an (h, w, 3) image as .hdr (RGBE) or .bmp (24-bit, value^(1/gamma))."""
+    a = np.ascontiguousarray(rgb, np.float64)
+    check(_ffi.load().pg3g_save_image(os.fsencode(path), a.ctypes.data_as(C.POINTER(C.c_double)), a.shape[1],
+                                      a.shape[0], scale, gamma), "pg3g_save_image")
+
+
+def compare_images(image, reference):
+    """Per-pixel and whole-image relative RMSE of luminance (pg3g_image_compare) and |diff|."""
+    a = np.ascontiguousarray(image, np.float32)
+    b = np.ascontiguousarray(reference, np.float32)
+    if a.shape != b.shape:
+        raise ValueError(f"image shapes differ: {a.shape} vs {b.shape}")
+    d = _ffi.ImageDiff()
+    diff = np.zeros_like(a)
+    fp = C.POINTER(C.c_float)
+    check(_ffi.load().pg3g_image_compare(a.ctypes.data_as(fp), b.ctypes.data_as(fp), a.shape[1], a.shape[0],
+                                         C.byref(d), diff.ctypes.data_as(fp)), "pg3g_image_compare")
+    return {k: float(getattr(d, k)) for k, _ in _ffi.ImageDiff._fields_}, diff
+
+
 def steerable_build(em_desc, threads=0, leaves=True):
     """Host-only build of the steerable sampler tables of `em_desc` (an EmDesc, e.g.
     SceneBuild.desc.em): returns (info dict, leaf directions (n, 3, 3) or None, root weight (9,))."""
@@ -91,6 +142,7 @@ def steerable_build(em_desc, threads=0, leaves=True):
     root = np.zeros(9, np.float32)
     check(lib.pg3g_steerable_build(C.byref(em_desc), threads, C.byref(info), None, 0,
                                    root.ctypes.data_as(C.POINTER(C.c_float))), "pg3g_steerable_build")
+    from_cache, build_ms = bool(info.from_cache), float(info.build_ms)   # of the first call
     out = None
     if leaves:
         out = np.zeros((info.num_leaves, 3, 3), np.float32)
@@ -98,7 +150,8 @@ def steerable_build(em_desc, threads=0, leaves=True):
                                        out.ctypes.data_as(C.POINTER(C.c_float)), info.num_leaves, None),
               "pg3g_steerable_build")
     d = {"num_leaves": int(info.num_leaves), "num_vertices": int(info.num_vertices),
-         "num_levels": int(info.num_levels), "threads": int(info.threads), "build_ms": float(info.build_ms)}
+         "num_levels": int(info.num_levels), "threads": int(info.threads), "from_cache": from_cache,
+         "build_ms": build_ms}
     return d, out, root
 
 
