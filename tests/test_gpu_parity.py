@@ -262,6 +262,27 @@ def test_reference_scene_parity(ctx, scene_id):
     assert_image_parity(g, o, frac=0.03)
 
 
+@pytest.mark.parametrize("scene_id,em_type", [(22, 2), (22, 3), (22, 7), (12, 3)])
+def test_reference_scene_image_em_parity(ctx, scene_id, em_type, tmp_path):
+    """Reference scenes 22 (environment light only) and 12 (four lights) with image EMs: the
+    constant white 8x4 and 1024x512 images built in memory (em types 2, 3), and a 64x32 probe
+    read from a PFM file with the reference's rotation 0.83 and scale 1.5 of em type 7
+    (scene.hxx:891-897); simple spherical sampler, the reference's default."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    path = None
+    if em_type == 7:
+        rng = np.random.default_rng(9)
+        img = (rng.random((32, 64, 3)) ** 4 * 3).astype(np.float32)
+        path = str(tmp_path / "probe.pfm")
+        with open(path, "wb") as f:
+            f.write(b"PF\n64 32\n-1.0\n" + np.ascontiguousarray(img[::-1]).astype("<f4").tobytes())
+    sb = SceneBuild.reference(scene_id, em_type=em_type, resolution=40, em_path=path)
+    assert sb.desc.em.sampler == _ffi.EM_SIMPLE
+    sb.set_params(splitting_budget=1.0, indirect_clipping=0.0, max_path_length=8, seed=7)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+
+
 def test_iterations_accumulate_like_run_iteration(ctx):
     """Rendering iterations [0,4) then [4,8) equals rendering [0,8) (RunIteration is additive)."""
     ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
