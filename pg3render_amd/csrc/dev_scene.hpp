@@ -7,10 +7,16 @@
 
 namespace pg3 {
 
-// Triangle record: 64 B, read with wave-uniform (scalar) loads in the brute-force loop.
+// Triangle record: 96 B, read with wave-uniform (scalar) loads in the brute-force loop.  The
+// edges are stored as Pluecker lines: the reference's edge test (scene_graph.hxx:107-120),
+// dot(cross(pj - o, pi - o), d), equals dot(d, pj x pi) - dot(o x d, pj - pi), so with the ray's
+// moment o x d computed once per ray each edge costs two 3-term dot products instead of three
+// subtractions, a cross product and a dot product.  Edge k: u[k] = pj - pi, v[k] = pj x pi for
+// (j, i) = (2, 1), (1, 0), (0, 2).  np0 = dot(n, p0) gives the plane distance.
 struct alignas(16) DevTri {
-    float p0[3], p1[3], p2[3];
+    float u[3][3], v[3][3];
     float n[3];          // normalize(cross(p1-p0, p2-p0)), scene_graph.hxx:100
+    float np0;
     int32_t mat;
     int32_t pad[3];
 };
@@ -90,17 +96,15 @@ struct Hit {
 // The intersection routines are built with FMA contraction and a fast reciprocal division (the
 // rest of the device code is not): they are the bulk of the arithmetic, and their results enter the
 // parity checks only through whole images, where one-ulp hit distances are invisible.
-PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, float tmin, Hit &h) {       // :103-134
+// m = org x dir, the ray's moment (computed once per ray).
+PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, F3 m, float tmin, Hit &h) {  // :103-134
 #pragma clang fp contract(fast)
-    const F3 ao = mk(t.p0[0], t.p0[1], t.p0[2]) - org;
-    const F3 bo = mk(t.p1[0], t.p1[1], t.p1[2]) - org;
-    const F3 co = mk(t.p2[0], t.p2[1], t.p2[2]) - org;
-    const float v0d = Dot(Cross(co, bo), dir);
-    const float v1d = Dot(Cross(bo, ao), dir);
-    const float v2d = Dot(Cross(ao, co), dir);
+    const float v0d = Dot(dir, mk(t.v[0][0], t.v[0][1], t.v[0][2])) - Dot(m, mk(t.u[0][0], t.u[0][1], t.u[0][2]));
+    const float v1d = Dot(dir, mk(t.v[1][0], t.v[1][1], t.v[1][2])) - Dot(m, mk(t.u[1][0], t.u[1][1], t.u[1][2]));
+    const float v2d = Dot(dir, mk(t.v[2][0], t.v[2][1], t.v[2][2])) - Dot(m, mk(t.u[2][0], t.u[2][1], t.u[2][2]));
     if (((v0d < 0.f) && (v1d < 0.f) && (v2d < 0.f)) || ((v0d >= 0.f) && (v1d >= 0.f) && (v2d >= 0.f))) {
         const F3 n = mk(t.n[0], t.n[1], t.n[2]);
-        const float d = __fdividef(Dot(n, ao), Dot(n, dir));
+        const float d = __fdividef(t.np0 - Dot(n, org), Dot(n, dir));
         if ((d > tmin) & (d < h.dist)) { h.dist = d; h.mat = t.mat; h.n = n; return true; }
     }
     return false;
@@ -152,12 +156,23 @@ template <class T> PG3_DEV T LoadGlobal(const T *p) {
 typedef float F4v __attribute__((ext_vector_type(4)));
 PG3_DEV DevTri LoadTri(const DevTri *base, uint32_t i) {
     const ConstPtr<F4v> q = (ConstPtr<F4v>)AsConst(base + i);
-    const F4v a = q[0], b = q[1], c = q[2], d = q[3];
+    float f[24];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const F4v a = q[k];
+        f[4 * k] = a.x; f[4 * k + 1] = a.y; f[4 * k + 2] = a.z; f[4 * k + 3] = a.w;
+    }
     DevTri t;
-    t.p0[0] = a.x; t.p0[1] = a.y; t.p0[2] = a.z; t.p1[0] = a.w;
-    t.p1[1] = b.x; t.p1[2] = b.y; t.p2[0] = b.z; t.p2[1] = b.w;
-    t.p2[2] = c.x; t.n[0] = c.y; t.n[1] = c.z; t.n[2] = c.w;
-    t.mat = __float_as_int(d.x);
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            t.u[e][c] = f[3 * e + c];
+            t.v[e][c] = f[9 + 3 * e + c];
+        }
+    t.n[0] = f[18]; t.n[1] = f[19]; t.n[2] = f[20];
+    t.np0 = f[21];
+    t.mat = __float_as_int(f[22]);
     return t;
 }
 PG3_DEV DevSphere LoadSphere(const DevSphere *base, uint32_t i) {
@@ -169,11 +184,16 @@ PG3_DEV DevSphere LoadSphere(const DevSphere *base, uint32_t i) {
     return s;
 }
 
+PG3_DEV F3 RayMoment(F3 org, F3 dir) {
+#pragma clang fp contract(fast)
+    return Cross(org, dir);
+}
 PG3_DEV bool Intersect(const SceneView &S, F3 org, F3 dir, float tmin, Hit &h) {  // scene.hxx:42-59
     bool any = false;
+    const F3 m = RayMoment(org, dir);
     for (uint32_t i = 0; i < S.numTris; ++i) {
         const DevTri t = LoadTri(S.tris, i);
-        any |= TriHit(t, org, dir, tmin, h);
+        any |= TriHit(t, org, dir, m, tmin, h);
     }
     for (uint32_t i = 0; i < S.numSphs; ++i) {
         const DevSphere s = LoadSphere(S.sphs, i);
@@ -185,9 +205,10 @@ PG3_DEV bool Occluded(const SceneView &S, F3 p, F3 dir, float tmax) {           
     const F3 org = p + dir * kEpsRay;
     Hit h;
     h.dist = tmax - 2 * kEpsRay;
+    const F3 m = RayMoment(org, dir);
     for (uint32_t i = 0; i < S.numTris; ++i) {
         const DevTri t = LoadTri(S.tris, i);
-        if (TriHit(t, org, dir, 0.f, h)) return true;
+        if (TriHit(t, org, dir, m, 0.f, h)) return true;
     }
     for (uint32_t i = 0; i < S.numSphs; ++i) {
         const DevSphere s = LoadSphere(S.sphs, i);

@@ -325,9 +325,17 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
         DevTri &t = tris[i];
         std::memset(&t, 0, sizeof(t));
         const float *p = d->triangles + i * 9;
-        for (int k = 0; k < 3; ++k) { t.p0[k] = p[k]; t.p1[k] = p[3 + k]; t.p2[k] = p[6 + k]; }
-        const F3 n = Normalize(Cross(mk(p[3] - p[0], p[4] - p[1], p[5] - p[2]), mk(p[6] - p[0], p[7] - p[1], p[8] - p[2])));
+        const F3 P[3] = {mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5]), mk(p[6], p[7], p[8])};
+        const int ends[3][2] = {{2, 1}, {1, 0}, {0, 2}};   // edge k runs from P[i] to P[j]: (j, i)
+        for (int e = 0; e < 3; ++e) {
+            const F3 a = P[ends[e][0]], b = P[ends[e][1]];
+            const F3 u = a - b, v = Cross(a, b);
+            t.u[e][0] = u.x; t.u[e][1] = u.y; t.u[e][2] = u.z;
+            t.v[e][0] = v.x; t.v[e][1] = v.y; t.v[e][2] = v.z;
+        }
+        const F3 n = Normalize(Cross(P[1] - P[0], P[2] - P[0]));
         t.n[0] = n.x; t.n[1] = n.y; t.n[2] = n.z;
+        t.np0 = Dot(n, P[0]);
         t.mat = d->triangle_mat[i];
     }
     std::vector<DevSphere> sphs(d->num_spheres);
