@@ -66,7 +66,7 @@ BatchKernel(const SceneView *__restrict__ views, const BatchScene *__restrict__ 
     __shared__ double accLds[4][2 * 64 * 3];
     uint32_t segments = 0, samples = 0;
     BatchItems<kEmKind, kOneLight> src{views, info, numScenes, numItems, work[1], work};
-    TraceItems(src, accLds[threadIdx.x >> 6], segments, samples);
+    TraceItems<ChainPath<BatchItems<kEmKind, kOneLight>::kMergeEm>>(src, accLds[threadIdx.x >> 6], segments, samples);
     const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
     if (lane == 0 && counters) {
         atomicAdd(counters + 0, (unsigned long long)segSum);

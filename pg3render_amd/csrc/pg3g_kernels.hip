@@ -111,7 +111,7 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     SceneItems<kOneLight && kEmKind == kEmSteerable> src{Sp,        spec,     work,     fb,      firstIter,
                                                          nIter,     itersPerThread, numTiles * numChunks, numTiles,
                                                          tilesX};
-    TraceItems(src, accLds[threadIdx.x >> 6], segments, samples);
+    TraceItems<ChainPath<kOneLight && kEmKind == kEmSteerable>>(src, accLds[threadIdx.x >> 6], segments, samples);
     const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
     if (lane == 0 && counters) {
         atomicAdd(counters + 0, (unsigned long long)segSum);

@@ -890,7 +890,9 @@ struct TileItem {
 // nothing: a sample's path depends only on its (pixel, iteration) RNG stream.
 //   src.Next(TileItem&) -> bool   fetches the next item (false: none left; wave-uniform)
 //   src.View(scene) -> const SceneView *, src.Spec(scene) -> LightSpec
-template <class Src>
+//   Path: a lane's sample; Start(S, pixel, iteration), Step(S, spec, segments) -> true when the
+//   sample is done, Result() -> its radiance
+template <class Path, class Src>
 PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &samples) {
     const uint32_t lane = __lane_id();
 #pragma unroll
@@ -906,8 +908,7 @@ PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &sam
     uint32_t scene = 0;                // scene of the running paths
     bool busy = false;
     uint32_t pix = 0, slot = 0;        // this lane's sample: tile pixel and slot
-    RngKey key{};
-    PathState ps;
+    Path path;
     for (;;) {
         const uint64_t idle = __ballot(!busy);
         if (idle != 0 && more && !(liveA && liveB) && !((cur ? liveB : liveA) && next < (cur ? B.total : A.total))) {
@@ -938,9 +939,7 @@ PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &sam
                 pix = q & 63u;
                 const uint32_t x = T.x0 + (pix & 7u), y = T.y0 + (pix >> 3);
                 if (x < S.resX && y < S.resY) {
-                    F3 org, dir;
-                    CameraSample(S, y * S.resX + x, T.it + (q >> 6), key, org, dir);
-                    InitPath(ps, org, dir);
+                    path.Start(S, y * S.resX + x, T.it + (q >> 6));
                     busy = true;
                     slot = cur;
                     start = true;
@@ -954,17 +953,12 @@ PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &sam
         if (__ballot(busy) != 0) {
             bool done = false;
             if (busy) {
-                Hit h;
-                h.dist = 1e36f;
-                const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
-                ShadowReq sr;
-                const bool cont = ShadeVertex<Src::kMergeEm>(S, spec, key, ps, hit, h, sr, segments);
-                if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
-                if (!cont) {
+                if (path.Step(S, spec, segments)) {
+                    const F3 L = path.Result();
                     double *a = acc + slot * 192u + pix * 3u;
-                    atomicAdd(a + 0, (double)ps.L.x);
-                    atomicAdd(a + 1, (double)ps.L.y);
-                    atomicAdd(a + 2, (double)ps.L.z);
+                    atomicAdd(a + 0, (double)L.x);
+                    atomicAdd(a + 1, (double)L.y);
+                    atomicAdd(a + 2, (double)L.z);
                     ++samples;
                     busy = false;
                     done = true;
@@ -989,6 +983,29 @@ PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &sam
         if (!more && !liveA && !liveB) break;
     }
 }
+
+// A lane's sample of the path tracer with -sb 1 -iic 0: the chain of ShadeVertex, one vertex
+// (extension ray, light sample and its shadow ray, BSDF sample) per step.
+template <bool kMergeEm>
+struct ChainPath {
+    RngKey key;
+    PathState ps;
+    PG3_DEV void Start(const SceneView &S, uint32_t pixel, uint32_t iteration) {
+        F3 org, dir;
+        CameraSample(S, pixel, iteration, key, org, dir);
+        InitPath(ps, org, dir);
+    }
+    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, uint32_t &segments) {
+        Hit h;
+        h.dist = 1e36f;
+        const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
+        ShadowReq sr;
+        const bool cont = ShadeVertex<kMergeEm>(S, spec, key, ps, hit, h, sr, segments);
+        if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
+        return !cont;
+    }
+    PG3_DEV F3 Result() const { return ps.L; }
+};
 
 // Item source of one scene: items chunk * numTiles + tile from the launch's work counter.
 template <bool kMerge>
