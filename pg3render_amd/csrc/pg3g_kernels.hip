@@ -75,7 +75,9 @@ template <int kEmKind>
 constexpr uint32_t PTBlock() { return kEmKind == kEmSteerable ? 64u * kPTWavesPerCU : 256u; }
 static_assert(64u * kPTWavesPerCU <= 1024u, "one steerable workgroup per CU holds every resident wavefront");
 
-template <int kEmKind, bool kOneLight>
+// kSplit: the path tracer with splitting at the first vertex and indirect clipping (SplitPath),
+// else the -sb 1 -iic 0 chain (ChainPath).
+template <int kEmKind, bool kOneLight, bool kSplit>
 __global__ void __launch_bounds__(PTBlock<kEmKind>()) __attribute__((amdgpu_waves_per_eu(PG3_PT_WAVES_PER_EU)))
 RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nIter, uint32_t itersPerThread,
                uint32_t numChunks, double *__restrict__ fb, uint32_t *__restrict__ work,
@@ -111,7 +113,8 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     SceneItems<kOneLight && kEmKind == kEmSteerable> src{Sp,        spec,     work,     fb,      firstIter,
                                                          nIter,     itersPerThread, numTiles * numChunks, numTiles,
                                                          tilesX};
-    TraceItems<ChainPath<kOneLight && kEmKind == kEmSteerable>>(src, accLds[threadIdx.x >> 6], segments, samples);
+    if constexpr (kSplit) TraceItems<SplitPath>(src, accLds[threadIdx.x >> 6], segments, samples);
+    else TraceItems<ChainPath<kOneLight && kEmKind == kEmSteerable>>(src, accLds[threadIdx.x >> 6], segments, samples);
     const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
     if (lane == 0 && counters) {
         atomicAdd(counters + 0, (unsigned long long)segSum);
@@ -334,11 +337,14 @@ __global__ void ResetWorkCounter(uint32_t *work) {
 }
 
 // Specialised path-tracer kernel for a scene (`pt` only).
-static PTKernelFn PickPTKernel(const SceneView &S) {
+static PTKernelFn PickPTKernel(const SceneView &S, bool split) {
     const bool one = (S.numLights == 1);
     const int em = SceneLightSpec(S).em;
 #define PG3_PT_CASE(K)                                                                        \
-    if (em == (K)) return one ? RenderPTKernel<(K), true> : RenderPTKernel<(K), false>;
+    if (em == (K)) {                                                                          \
+        if (split) return one ? RenderPTKernel<(K), true, true> : RenderPTKernel<(K), false, true>; \
+        return one ? RenderPTKernel<(K), true, false> : RenderPTKernel<(K), false, false>;    \
+    }
     PG3_PT_CASE(kEmAbsent)
     PG3_PT_CASE(kEmCosine)
     PG3_PT_CASE(kEmSimple)
@@ -361,7 +367,11 @@ hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t 
     const uint64_t threads = (uint64_t)tiles * 64u * numChunks;
     const uint32_t blocks = (uint32_t)((threads + 255) / 256);
     const bool chain = (S.algorithm == 7) && S.splitBudget == 1.f && !(S.iic > 0.f);   // -sb 1 -iic 0
-    const PTKernelFn pt = chain ? PickPTKernel(S) : nullptr;
+    // ComputeSplittingCounts at the first vertex: when it leaves budget 1 to the next vertex,
+    // every later vertex has one BSDF sample (SplitPath); other budgets take the general tree
+    const uint32_t nB0 = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * S.splitBudget);
+    const bool rootSplit = (S.algorithm == 7) && !chain && nB0 >= 1u && (1 + ((S.splitBudget - nB0) / (float)nB0)) == 1.f;
+    const PTKernelFn pt = (chain || rootSplit) ? PickPTKernel(S, rootSplit) : nullptr;
     const uint32_t ptBlock = (SceneLightSpec(S).em == kEmSteerable) ? PTBlock<kEmSteerable>() : PTBlock<kEmSimple>();
     if (pt) {
         if (!work) return hipErrorInvalidValue;

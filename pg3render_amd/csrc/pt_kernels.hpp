@@ -1007,6 +1007,267 @@ struct ChainPath {
     PG3_DEV F3 Result() const { return ps.L; }
 };
 
+// A lane's sample of EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with splitting at the
+// first vertex only and indirect clipping: the case of every integer splitting budget at the
+// reference's splitting level 1 (its defaults -sb 4 -iic 16), where ComputeSplittingCounts
+// (:432-454) gives the first vertex nB BSDF and nL light samples and budget 1 to every later
+// vertex.  The first vertex's estimate is its light samples plus, per BSDF sample, the MIS-
+// weighted emission found and the clipped indirect estimate of that sample's subtree; the terms
+// of different BSDF samples add, so the nB subtrees are traced one after another, each a chain.
+// Clipping makes a chain vertex's estimate R_v = A_v + clip(w_v R_{v+1}) depend on the whole
+// suffix (A_v: its light samples and the emission its BSDF sample finds), so each vertex pushes
+// (A_v, the factors of w_v) on a per-lane stack and the chain's end folds it back to front.  One
+// vertex per step, the same RNG streams, operations and order as RadiancePTTree, which handles
+// every other splitting configuration.
+constexpr uint32_t kChainStack = kTreeDepth - 1;   // vertices 2 .. kTreeDepth
+struct SplitPath {
+    // ray in flight: it reaches vertex `len` on RNG branch `branch`
+    F3 org, dir;
+    float tmin;
+    uint32_t len, branch;
+    bool refl;                   // the vertex it reaches is shaded (not cut by Russian roulette)
+    bool nextRoot;               // the next step starts the first vertex's next BSDF sample
+    RngKey key;                  // branch 0 (c3 low half = 0)
+    // first vertex
+    F3 rSp, rN, rWol, em1, reflRoot;
+    int32_t rMat;
+    float rRR, rLcSum;
+    uint32_t nB, nL, s, rSides;
+    F3 rAttCos;                  // current first-vertex BSDF sample: clip((rAttCos * R) / rDen)
+    float rDen;
+    bool rCut;
+    // parent of the vertex the ray reaches (MIS of the emission found there)
+    F3 pAttCos, pSp, pN;
+    float pPdfW, pComp, pLcSum;
+    uint32_t pSides, pLen, pBranch, pnB, pnL;
+    bool pFinite;
+    // chain stack: vertices 2 .. ; the top entry's A is kept in topA
+    int32_t top;
+    uint64_t cutMask;
+    F3 topA;
+    float stA[kChainStack][3], stW[kChainStack][3], stDen[kChainStack];
+    F3 L;
+
+    PG3_DEV void Start(const SceneView &S, uint32_t pixel, uint32_t iteration) {
+        CameraSample(S, pixel, iteration, key, org, dir);
+        tmin = 0.f;
+        len = 1;
+        branch = 0;
+        refl = true;
+        nextRoot = false;
+        em1 = mk(0.f);
+        reflRoot = mk(0.f);
+        top = -1;
+    }
+    PG3_DEV RngKey Branch(uint32_t b) const {
+        RngKey k = key;
+        k.c3 = (key.c3 & 0xFFFF0000u) | b;
+        return k;
+    }
+    // Light-sample estimate of a shaded vertex (:252-265, :350-425) with nLv light and nBv BSDF
+    // samples (MIS counts).
+    PG3_DEV F3 LightSamples(const SceneView &S, const LightSpec &spec, const RngKey &k, uint32_t vlen, F3 sp,
+                            const Frame &fr, F3 wol, const DevMaterial &mat, uint32_t nLv, uint32_t nBv,
+                            const LightCache &lc) const {
+        F3 acc = mk(0.f);
+        if ((vlen + 1) < S.minPL) return acc;
+        for (uint32_t l = 0; l < nLv; ++l) {
+            float prob = 1.f;
+            const int32_t li = PickLight(spec, lc, RngGet(k, vlen, l, kDimLightPick), prob);
+            LightSample ls;
+            if (li >= 0 && SampleIllumination(S, spec, (uint32_t)li, sp, fr, mat.sides, RngGet(k, vlen, l, kDimLight0),
+                                              RngGet(k, vlen, l, kDimLight1), RngGet(k, vlen, l, kDimLight2), ls)) {
+                ls.prob = prob;
+                if (MaxC(ls.sample) > 0.f && !Occluded(S, sp, ls.wig, ls.dist)) {
+                    BsdfRec r;
+                    EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
+                    if (ls.pdfW != kInf) {
+                        const float bsdfPdf = r.pdfW * r.compProb;
+                        const float lightPdf = ls.pdfW * ls.prob;
+                        acc += (ls.sample * r.att * MisPowerN(lightPdf, nLv, bsdfPdf, nBv)) / lightPdf;
+                    } else {
+                        acc += (ls.sample * r.att) / (ls.prob * nLv);
+                    }
+                }
+            }
+        }
+        return acc;
+    }
+    // BSDF sample `sub` of a vertex; false for a blocker.  Sets the ray to the next vertex, the
+    // parent record for that vertex's emission, and the indirect-term factors (attCos, den, cut).
+    PG3_DEV bool SampleAt(const SceneView &S, const RngKey &k, uint32_t vlen, uint32_t vbranch, uint32_t sub, F3 sp,
+                          const Frame &fr, F3 wol, const DevMaterial &mat, uint32_t nBv, uint32_t nLv, float rr,
+                          float lcSum, F3 &attCos, float &den, bool &cut) {
+        cut = (S.maxPL == 0) && (RngGet(k, vlen, sub, kDimRR) > rr);
+        BsdfRec r;
+        SampleBsdf(mat, wol, RngGet(k, vlen, sub, kDimBsdf0), RngGet(k, vlen, sub, kDimBsdf1),
+                   RngGet(k, vlen, sub, kDimBsdf2), r);
+        if (IsBlocker(r)) return false;
+        const bool finite = (r.pdfW != kInf);
+        const float cosAbs = fabsf(r.wil.z);
+        attCos = finite ? r.att * cosAbs : r.att;
+        den = finite ? (r.pdfW * nBv * rr * r.compProb) : (nBv * rr * r.compProb);
+        pAttCos = attCos;
+        pPdfW = r.pdfW;
+        pComp = r.compProb;
+        pFinite = finite;
+        pSp = sp;
+        pN = fr.z;
+        pSides = mat.sides;
+        pLen = vlen;
+        pBranch = vbranch;
+        pnB = nBv;
+        pnL = nLv;
+        pLcSum = lcSum;
+        org = sp;
+        dir = ToWorld(fr, r.wil);
+        tmin = EpsRayCos(cosAbs);
+        len = vlen + 1;
+        branch = (nBv > 1) ? ((vbranch * 7u + sub + 1u) & 0xFFFFu) : vbranch;
+        refl = !cut;
+        return true;
+    }
+    // MIS-weighted emission delivered to the parent (path_tracer.hxx:327-369)
+    PG3_DEV F3 ParentEmission(const SceneView &S, const LightSpec &spec, F3 em, float lpdf, int32_t lid) const {
+        if (IsZero(em) || lid < 0) return mk(0.f);
+        if (!pFinite) return (pAttCos * em) / (static_cast<float>(pnB) * pComp);
+        float pick = 1.f;
+        if (spec.numLights == 0) {
+            pick = 0.f;
+        } else if (spec.numLights > 1) {
+            pick = (pLcSum > 0.f) ? EstimateContribution(S, spec, (uint32_t)lid, pSp, FrameFromZ(pN), pSides,
+                                                         Branch(pBranch), pLen) / pLcSum
+                                  : 1.f / spec.numLights;
+        }
+        const float lp = lpdf * pick;
+        const float bp = pPdfW * pComp;
+        return (pAttCos * em * MisPowerN(bp, pnB, lp, pnL)) / bp;
+    }
+    PG3_DEV static F3 Indirect(F3 attCos, F3 R, float den, float iic) {
+        F3 ind = (attCos * R) / den;
+        if (iic > 0.f) ind = ClipProportionally(ind, iic);
+        return ind;
+    }
+    // End of a first-vertex subtree: fold the chain stack and add the clipped indirect term.
+    PG3_DEV void FoldChain(const SceneView &S) {
+        F3 R = mk(0.f);
+        for (int32_t k = top; k >= 0; --k) {
+            const F3 A = (k == top) ? topA : mk(stA[k][0], stA[k][1], stA[k][2]);
+            F3 Rk = A;
+            if (!((cutMask >> k) & 1ull) && !IsZero(R))
+                Rk += Indirect(mk(stW[k][0], stW[k][1], stW[k][2]), R, stDen[k], S.iic);
+            R = Rk;
+        }
+        if (!rCut && !IsZero(R)) reflRoot += Indirect(rAttCos, R, rDen, S.iic);
+        top = -1;
+        ++s;
+        nextRoot = true;
+    }
+    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, uint32_t &segments) {
+        if (nextRoot) {                                     // next BSDF sample of the first vertex
+            nextRoot = false;
+            const DevMaterial mat = LoadGlobal(S.mats + rMat);
+            const Frame fr = FrameFromZ(rN);
+            bool started = false;
+            while (s < nB && !started) {
+                started = SampleAt(S, key, 1u, 0u, s, rSp, fr, rWol, mat, nB, nL, rRR, rLcSum, rAttCos, rDen, rCut);
+                if (!started) ++s;
+            }
+            if (!started) {
+                L = em1 + reflRoot;
+                return true;
+            }
+            cutMask = 0;
+        }
+        const RngKey k = Branch(branch);
+        Hit h;
+        h.dist = 1e36f;
+        const bool hit = Intersect(S, org, dir, tmin, h);
+        F3 em = mk(0.f);
+        float lpdf = 0.f;
+        int32_t lid = -1;
+        F3 sp = mk(0.f), wol = mk(0.f);
+        Frame fr{};
+        DevMaterial mat{};
+        if (hit) {                                                                    // :176-238
+            sp = org + dir * h.dist;
+            fr = FrameFromZ(h.n);
+            wol = ToLocal(fr, -dir);
+            mat = LoadGlobal(S.mats + h.mat);
+            if (mat.lightId >= 0 && len >= S.minPL) {
+                const DevLight Lt = LoadGlobal(S.lights + mat.lightId);
+                if (Lt.type == kLightArea) {
+                    if (len > 1) lpdf = AreaEmissionPdf(Lt, sp, org);
+                    if (wol.z > 0.f) em = Lt.radiance;
+                } else if (Lt.type == kLightPoint) {
+                    lpdf = kInf;
+                }
+                lid = mat.lightId;
+            }
+        } else if (len >= S.minPL && S.bgLight >= 0) {                               // :408-422
+            em = BgRadiance(S, spec, dir);
+            if (len > 1) lpdf = BgPdf(S, spec, dir, FrameFromZ(pN), pSides);
+            lid = S.bgLight;
+        }
+        if (len == 1) em1 = em;
+        else if (len == 2) reflRoot += ParentEmission(S, spec, em, lpdf, lid);
+        else topA += ParentEmission(S, spec, em, lpdf, lid);
+        const bool stop = !hit || !refl || ReflectanceZero(mat) || (S.maxPL > 0 && len >= S.maxPL) || len >= 1200u ||
+                          len > kTreeDepth;
+        if (stop) {
+            if (len == 1) {
+                L = em1 + reflRoot;
+                return true;
+            }
+            FoldChain(S);
+            return false;
+        }
+        ++segments;
+        LightCache lc;
+        lc.sum = 0.f;
+        if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
+        if (len == 1) {                                                               // the first vertex
+            nB = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * S.splitBudget);
+            nL = (uint32_t)lroundf(fmaxf(S.lightRatio * nB, 1.f));
+            reflRoot = LightSamples(S, spec, k, len, sp, fr, wol, mat, nL, nB, lc);
+            rSp = sp;
+            rN = fr.z;
+            rWol = wol;
+            rMat = h.mat;
+            rSides = mat.sides;
+            rLcSum = lc.sum;
+            rRR = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
+            s = 0;
+            nextRoot = true;
+            return false;
+        }
+        // a chain vertex: budget 1, one BSDF sample
+        const uint32_t nLc = (uint32_t)lroundf(fmaxf(S.lightRatio * 1u, 1.f));
+        if (top >= 0) {
+            stA[top][0] = topA.x;
+            stA[top][1] = topA.y;
+            stA[top][2] = topA.z;
+        }
+        ++top;
+        topA = LightSamples(S, spec, k, len, sp, fr, wol, mat, nLc, 1u, lc);
+        const float rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
+        F3 attCos;
+        float den;
+        bool cut;
+        if (!SampleAt(S, k, len, branch, 0u, sp, fr, wol, mat, 1u, nLc, rr, lc.sum, attCos, den, cut)) {
+            FoldChain(S);                                   // a blocker: this vertex has no subtree
+            return false;
+        }
+        stW[top][0] = attCos.x;
+        stW[top][1] = attCos.y;
+        stW[top][2] = attCos.z;
+        stDen[top] = den;
+        if (cut) cutMask |= 1ull << top;
+        return false;
+    }
+    PG3_DEV F3 Result() const { return L; }
+};
+
 // Item source of one scene: items chunk * numTiles + tile from the launch's work counter.
 template <bool kMerge>
 struct SceneItems {

@@ -331,6 +331,31 @@ def test_splitting_and_clipping_parity(ctx, params):
     assert_image_parity(g, o, frac=0.03)
 
 
+@pytest.mark.parametrize("scene_id", [3, 11, 18, 32, 34, 37])
+def test_reference_defaults_on_reference_scenes(ctx, scene_id):
+    """The reference's command-line defaults (-sb 4 -iic 16, Russian roulette: first-vertex
+    splitting, SplitPath) on its scene IDs: Phong, GGX and smooth (Dirac) materials, area, point
+    and background lights."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.reference(scene_id, resolution=24)
+    sb.set_params(algorithm="pt", splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0, seed=3)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+
+
+@pytest.mark.parametrize("params", [dict(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0),
+                                    dict(splitting_budget=3.0, indirect_clipping=2.0, max_path_length=6,
+                                         light_to_bsdf_ratio=2.0)])
+def test_splitting_with_several_lights_and_steerable_em(ctx, params):
+    """Three lights (two area triangles and a steerable EM): light picking by contribution at the
+    split vertex and along the chains, EM pdfs in the MIS of escaped subtrees."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(3, seed=2024, variant=11, res=24)
+    sb.set_params(algorithm="pt", seed=5, **params)
+    g, o = render_pair(ctx, sb, 2)
+    assert_image_parity(g, o, frac=0.03)
+
+
 def test_splitting_on_reference_scene_with_em(ctx):
     """Reference scene 2 (box, spheres, EM background) with the reference's defaults -sb 4 -iic 16
     and Russian roulette."""
