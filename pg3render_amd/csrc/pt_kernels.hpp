@@ -1026,7 +1026,6 @@ struct SplitPath {
     float tmin;
     uint32_t len, branch;
     bool refl;                   // the vertex it reaches is shaded (not cut by Russian roulette)
-    bool nextRoot;               // the next step starts the first vertex's next BSDF sample
     RngKey key;                  // branch 0 (c3 low half = 0)
     // first vertex
     F3 rSp, rN, rWol, em1, reflRoot;
@@ -1054,7 +1053,6 @@ struct SplitPath {
         len = 1;
         branch = 0;
         refl = true;
-        nextRoot = false;
         em1 = mk(0.f);
         reflRoot = mk(0.f);
         top = -1;
@@ -1161,24 +1159,8 @@ struct SplitPath {
         if (!rCut && !IsZero(R)) reflRoot += Indirect(rAttCos, R, rDen, S.iic);
         top = -1;
         ++s;
-        nextRoot = true;
     }
     PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, uint32_t &segments) {
-        if (nextRoot) {                                     // next BSDF sample of the first vertex
-            nextRoot = false;
-            const DevMaterial mat = LoadGlobal(S.mats + rMat);
-            const Frame fr = FrameFromZ(rN);
-            bool started = false;
-            while (s < nB && !started) {
-                started = SampleAt(S, key, 1u, 0u, s, rSp, fr, rWol, mat, nB, nL, rRR, rLcSum, rAttCos, rDen, rCut);
-                if (!started) ++s;
-            }
-            if (!started) {
-                L = em1 + reflRoot;
-                return true;
-            }
-            cutMask = 0;
-        }
         const RngKey k = Branch(branch);
         Hit h;
         h.dist = 1e36f;
@@ -1214,55 +1196,92 @@ struct SplitPath {
         else topA += ParentEmission(S, spec, em, lpdf, lid);
         const bool stop = !hit || !refl || ReflectanceZero(mat) || (S.maxPL > 0 && len >= S.maxPL) || len >= 1200u ||
                           len > kTreeDepth;
+        // what the step samples next: the first vertex's next BSDF sample or the chain vertex's one
+        bool wantRoot = false, wantChain = false;
+        float rr = 1.0f;
+        LightCache lc;
+        lc.sum = 0.f;
         if (stop) {
             if (len == 1) {
                 L = em1 + reflRoot;
                 return true;
             }
             FoldChain(S);
-            return false;
+            wantRoot = true;
+        } else {
+            ++segments;
+            if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
+            const bool first = (len == 1);
+            if (first) {                                                              // :240-250
+                nB = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * S.splitBudget);
+                nL = (uint32_t)lroundf(fmaxf(S.lightRatio * nB, 1.f));
+            }
+            const uint32_t nLv = first ? nL : (uint32_t)lroundf(fmaxf(S.lightRatio * 1u, 1.f));
+            const F3 A = LightSamples(S, spec, k, len, sp, fr, wol, mat, nLv, first ? nB : 1u, lc);
+            rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
+            if (first) {
+                reflRoot = A;
+                rSp = sp;
+                rN = fr.z;
+                rWol = wol;
+                rMat = h.mat;
+                rSides = mat.sides;
+                rLcSum = lc.sum;
+                rRR = rr;
+                s = 0;
+                wantRoot = true;
+            } else {
+                if (top >= 0) {
+                    stA[top][0] = topA.x;
+                    stA[top][1] = topA.y;
+                    stA[top][2] = topA.z;
+                }
+                ++top;
+                topA = A;
+                wantChain = true;
+            }
         }
-        ++segments;
-        LightCache lc;
-        lc.sum = 0.f;
-        if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
-        if (len == 1) {                                                               // the first vertex
-            nB = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * S.splitBudget);
-            nL = (uint32_t)lroundf(fmaxf(S.lightRatio * nB, 1.f));
-            reflRoot = LightSamples(S, spec, k, len, sp, fr, wol, mat, nL, nB, lc);
-            rSp = sp;
-            rN = fr.z;
-            rWol = wol;
-            rMat = h.mat;
-            rSides = mat.sides;
-            rLcSum = lc.sum;
-            rRR = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
-            s = 0;
-            nextRoot = true;
-            return false;
-        }
-        // a chain vertex: budget 1, one BSDF sample
+        // One call site of the BSDF sampler for both kinds of vertex.  A chain vertex whose sample
+        // is a blocker ends its chain, and the first vertex's next sample follows in the same pass.
         const uint32_t nLc = (uint32_t)lroundf(fmaxf(S.lightRatio * 1u, 1.f));
-        if (top >= 0) {
-            stA[top][0] = topA.x;
-            stA[top][1] = topA.y;
-            stA[top][2] = topA.z;
-        }
-        ++top;
-        topA = LightSamples(S, spec, k, len, sp, fr, wol, mat, nLc, 1u, lc);
-        const float rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
-        F3 attCos;
-        float den;
-        bool cut;
-        if (!SampleAt(S, k, len, branch, 0u, sp, fr, wol, mat, 1u, nLc, rr, lc.sum, attCos, den, cut)) {
+        for (int pass = 0; pass < 2 && (wantRoot || wantChain); ++pass) {
+            if (wantRoot && s >= nB) {
+                L = em1 + reflRoot;
+                return true;
+            }
+            F3 attCos;
+            float den;
+            bool cut;
+            const DevMaterial vm = wantRoot ? LoadGlobal(S.mats + rMat) : mat;
+            const Frame vfr = wantRoot ? FrameFromZ(rN) : fr;
+            const bool ok = SampleAt(S, wantRoot ? key : k, wantRoot ? 1u : len, wantRoot ? 0u : branch, wantRoot ? s : 0u,
+                                     wantRoot ? rSp : sp, vfr, wantRoot ? rWol : wol, vm, wantRoot ? nB : 1u,
+                                     wantRoot ? nL : nLc, wantRoot ? rRR : rr, wantRoot ? rLcSum : lc.sum, attCos, den,
+                                     cut);
+            if (wantRoot) {
+                if (ok) {
+                    rAttCos = attCos;
+                    rDen = den;
+                    rCut = cut;
+                    cutMask = 0;
+                    return false;
+                }
+                ++s;                                        // a blocker: try the next sample
+                --pass;
+                continue;
+            }
+            if (ok) {
+                stW[top][0] = attCos.x;
+                stW[top][1] = attCos.y;
+                stW[top][2] = attCos.z;
+                stDen[top] = den;
+                if (cut) cutMask |= 1ull << top;
+                return false;
+            }
             FoldChain(S);                                   // a blocker: this vertex has no subtree
-            return false;
+            wantChain = false;
+            wantRoot = true;
         }
-        stW[top][0] = attCos.x;
-        stW[top][1] = attCos.y;
-        stW[top][2] = attCos.z;
-        stDen[top] = den;
-        if (cut) cutMask |= 1ull << top;
         return false;
     }
     PG3_DEV F3 Result() const { return L; }
