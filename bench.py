@@ -42,6 +42,27 @@ def parse():
     return ap.parse_args()
 
 
+def defaults_measure(ctx):
+    """The reference's command-line defaults (-sb 4 -iic 16, Russian roulette) on the C1 and C2
+    scenes: first-vertex splitting with indirect clipping (the split path of the fused kernel)."""
+    from pg3render_amd import SceneBuild
+    out = {}
+    for name, wl in (("box_area_light_256px_16spp", 1), ("box_steerable_em_512px_64spp", 2)):
+        sb = SceneBuild.workload(wl, seed=1)
+        sb.set_params(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0)
+        ctx.upload(sb, 1)
+        ctx.render(0, 2, 1)
+        ctx.synchronize()
+        ctx.reset_counters()
+        n = sb.spp
+        ms = [ctx.render_timed(i * n, n, 1) for i in range(3)]
+        seg, smp = ctx.counters()
+        t = sum(ms) / len(ms)
+        out[name] = {"ms_per_frame": round(t, 3), "msamples_per_s": round(smp / 3 / t / 1e3, 1),
+                     "msegments_per_s": round(seg / 3 / t / 1e3, 1)}
+    return out
+
+
 def host_threads():
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():       # the host share given to this job
@@ -251,6 +272,7 @@ def main():
            "rays_per_s": round(rays_per_s, 1), "roofline": roofline}
     if not args.no_batch:
         out["batched"] = batch_measure(ctx, args.batch_tune)
+    out["reference_defaults"] = defaults_measure(ctx)
     if not args.no_diff:
         out["differentiation"] = diff_measure(ctx)
         if args.inverse_steps > 0:
