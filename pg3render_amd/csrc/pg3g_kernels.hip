@@ -63,22 +63,30 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
 #ifndef PG3_PT_WAVES_PER_EU
 #define PG3_PT_WAVES_PER_EU 4
 #endif
-// Persistent: each CU holds kPTWavesPerCU (16) wavefronts for the whole launch, and every wavefront takes work
+// Persistent: each CU holds 4 x PTWavesPerEU wavefronts for the whole launch, and every wavefront takes work
 // items (8x8 tile x iteration chunk) from a device counter until none is left, so a wavefront
 // that finishes early starts the next item instead of waiting for its workgroup.  With the
 // steerable sampler the workgroup is 1024 lanes (one per CU) and stages the inner levels of the
 // EM's triangle tree in LDS once (kTreeLdsNodes nodes, 60 KiB): the tree descent of every light
 // sample then gathers from LDS instead of from the L2.
 constexpr uint32_t kTreeLdsNodes = 1280;
-constexpr uint32_t kPTWavesPerCU = 4u * PG3_PT_WAVES_PER_EU;
-template <int kEmKind>
-constexpr uint32_t PTBlock() { return kEmKind == kEmSteerable ? 64u * kPTWavesPerCU : 256u; }
-static_assert(64u * kPTWavesPerCU <= 1024u, "one steerable workgroup per CU holds every resident wavefront");
+#ifndef PG3_SPLIT_WAVES_PER_EU
+#define PG3_SPLIT_WAVES_PER_EU 4
+#endif
+// Waves per SIMD of the chain path and of the split path (the split path keeps more state)
+template <bool kSplit>
+constexpr uint32_t PTWavesPerEU() { return kSplit ? PG3_SPLIT_WAVES_PER_EU : PG3_PT_WAVES_PER_EU; }
+template <bool kSplit>
+constexpr uint32_t PTWavesPerCU() { return 4u * PTWavesPerEU<kSplit>(); }
+template <int kEmKind, bool kSplit>
+constexpr uint32_t PTBlock() { return kEmKind == kEmSteerable ? 64u * PTWavesPerCU<kSplit>() : 256u; }
+static_assert(64u * PTWavesPerCU<false>() <= 1024u && 64u * PTWavesPerCU<true>() <= 1024u,
+              "one steerable workgroup per CU holds every resident wavefront");
 
 // kSplit: the path tracer with splitting at the first vertex and indirect clipping (SplitPath),
 // else the -sb 1 -iic 0 chain (ChainPath).
 template <int kEmKind, bool kOneLight, bool kSplit>
-__global__ void __launch_bounds__(PTBlock<kEmKind>()) __attribute__((amdgpu_waves_per_eu(PG3_PT_WAVES_PER_EU)))
+__global__ void __launch_bounds__((PTBlock<kEmKind, kSplit>())) __attribute__((amdgpu_waves_per_eu(PTWavesPerEU<kSplit>())))
 RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nIter, uint32_t itersPerThread,
                uint32_t numChunks, double *__restrict__ fb, uint32_t *__restrict__ work,
                unsigned long long *__restrict__ counters) {
@@ -86,7 +94,7 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     // its ~450 bytes of fields spill into VGPR lanes of this register-heavy kernel.
     const SceneView &S = *Sp;
     LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
-    __shared__ double accLds[PTBlock<kEmKind>() / 64u][2 * 64 * 3];
+    __shared__ double accLds[PTBlock<kEmKind, kSplit>() / 64u][2 * 64 * 3];
     if constexpr (kEmKind == kEmSteerable) {
         __shared__ F4v treeLds[kTreeLdsNodes * 3];
         // the levels from the root down that fit (contiguous at the end of nodeW)
@@ -372,18 +380,21 @@ hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t 
     const uint32_t nB0 = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * S.splitBudget);
     const bool rootSplit = (S.algorithm == 7) && !chain && nB0 >= 1u && (1 + ((S.splitBudget - nB0) / (float)nB0)) == 1.f;
     const PTKernelFn pt = (chain || rootSplit) ? PickPTKernel(S, rootSplit) : nullptr;
-    const uint32_t ptBlock = (SceneLightSpec(S).em == kEmSteerable) ? PTBlock<kEmSteerable>() : PTBlock<kEmSimple>();
+    const bool steer = SceneLightSpec(S).em == kEmSteerable;
+    const uint32_t ptBlock = rootSplit ? (steer ? PTBlock<kEmSteerable, true>() : PTBlock<kEmSimple, true>())
+                                       : (steer ? PTBlock<kEmSteerable, false>() : PTBlock<kEmSimple, false>());
+    const uint32_t wavesPerCU = rootSplit ? PTWavesPerCU<true>() : PTWavesPerCU<false>();
     if (pt) {
         if (!work) return hipErrorInvalidValue;
         // items of a few iterations: the wave overlaps items, so small items cost no tail, and
         // they even out the waves' shares at the end of the launch
-        const uint32_t waves = (uint32_t)max(1, numCUs) * kPTWavesPerCU;
+        const uint32_t waves = (uint32_t)max(1, numCUs) * wavesPerCU;
         const uint64_t perWave = (uint64_t)nIter * tiles / (16u * waves);
         uint32_t ipi = perWave >= 2u ? 2u : 1u;
         if (const char *e = getenv("PG3G_PT_ITEM_ITERS")) ipi = (uint32_t)max(1, atoi(e));
         const uint32_t ptIpi = min(ipi, nIter), ptChunks = (nIter + ptIpi - 1) / ptIpi;
-        // kPTWavesPerCU resident wavefronts per CU (the kernel's register budget per SIMD x 4 SIMDs)
-        const uint32_t grid = (uint32_t)max(1, numCUs) * (64u * kPTWavesPerCU / ptBlock);
+        // resident wavefronts per CU: the kernel's register budget per SIMD x 4 SIMDs
+        const uint32_t grid = (uint32_t)max(1, numCUs) * (64u * wavesPerCU / ptBlock);
         hipLaunchKernelGGL(ResetWorkCounter, dim3(1), dim3(64), 0, stream, work);
         hipLaunchKernelGGL(pt, dim3(grid), dim3(ptBlock), 0, stream, Sdev, firstIter, nIter, ptIpi, ptChunks, fb, work,
                            counters);
