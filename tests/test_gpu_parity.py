@@ -343,6 +343,18 @@ def test_reference_defaults_on_reference_scenes(ctx, scene_id):
     assert_image_parity(g, o, frac=0.03)
 
 
+@pytest.mark.parametrize("variant", [11, 40])
+def test_several_lights_and_steerable_em_chain_parity(ctx, variant):
+    """-sb 1 -iic 0 on a scene with three lights (two area triangles and a steerable EM): light
+    picking by contribution, the EM as one of several lights in the shared EM lookups of escaped
+    paths and EM light samples."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(3, seed=2024, variant=variant, res=32)
+    sb.set_params(seed=9)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+
+
 @pytest.mark.parametrize("params", [dict(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0),
                                     dict(splitting_budget=3.0, indirect_clipping=2.0, max_path_length=6,
                                          light_to_bsdf_ratio=2.0)])
