@@ -69,7 +69,6 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
 // steerable sampler the workgroup is 1024 lanes (one per CU) and stages the inner levels of the
 // EM's triangle tree in LDS once (kTreeLdsNodes nodes, 60 KiB): the tree descent of every light
 // sample then gathers from LDS instead of from the L2.
-constexpr uint32_t kTreeLdsNodes = 1280;
 #ifndef PG3_SPLIT_WAVES_PER_EU
 #define PG3_SPLIT_WAVES_PER_EU 4
 #endif
@@ -97,22 +96,7 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     __shared__ double accLds[PTBlock<kEmKind, kSplit>() / 64u][2 * 64 * 3];
     if constexpr (kEmKind == kEmSteerable) {
         __shared__ F4v treeLds[kTreeLdsNodes * 3];
-        // the levels from the root down that fit (contiguous at the end of nodeW)
-        const DevEm &E = S.em;
-        uint32_t minLevel = E.numLevels, count = 0;
-        for (int32_t l = (int32_t)E.numLevels - 1; l >= 1; --l) {
-            if (count + E.levelCount[l] > kTreeLdsNodes) break;
-            count += E.levelCount[l];
-            minLevel = (uint32_t)l;
-        }
-        if (minLevel < E.numLevels) {
-            const uint32_t base = E.levelBase[minLevel];
-            const GlobalPtr<F4v> src = (GlobalPtr<F4v>)AsGlobal(E.nodeW + (size_t)base * 12u);
-            for (uint32_t i = threadIdx.x; i < count * 3u; i += blockDim.x) treeLds[i] = src[i];
-            spec.tree.nodes = reinterpret_cast<const float *>(treeLds);
-            spec.tree.minLevel = minLevel;
-            spec.tree.baseNode = base;
-        }
+        spec.tree = StageTree(S.em, treeLds);
         __syncthreads();
     }
     const uint32_t tilesX = (S.resX + 7u) / 8u, numTiles = tilesX * ((S.resY + 7u) / 8u);

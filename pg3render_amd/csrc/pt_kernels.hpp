@@ -850,6 +850,30 @@ PG3_DEV void CameraSample(const SceneView &S, uint32_t pixel, uint32_t iteration
     GenerateRay(S, float(x) + jx, float(y) + jy, org, dir);
 }
 
+// Stages the inner levels of a steerable EM's triangle tree in `lds` (12 floats per node as F4v
+// triples, at most kTreeLdsNodes nodes: the levels from the root down that fit, which are
+// contiguous at the end of nodeW), cooperatively by the workgroup; the caller synchronises the
+// workgroup before the returned cache is read.
+constexpr uint32_t kTreeLdsNodes = 1280;
+PG3_DEV TreeCache StageTree(const DevEm &E, F4v *lds) {
+    TreeCache tc;
+    uint32_t minLevel = E.numLevels, count = 0;
+    for (int32_t l = (int32_t)E.numLevels - 1; l >= 1; --l) {
+        if (count + E.levelCount[l] > kTreeLdsNodes) break;
+        count += E.levelCount[l];
+        minLevel = (uint32_t)l;
+    }
+    if (minLevel < E.numLevels) {
+        const uint32_t base = E.levelBase[minLevel];
+        const GlobalPtr<F4v> src = (GlobalPtr<F4v>)AsGlobal(E.nodeW + (size_t)base * 12u);
+        for (uint32_t i = threadIdx.x; i < count * 3u; i += blockDim.x) lds[i] = src[i];
+        tc.nodes = reinterpret_cast<const float *>(lds);
+        tc.minLevel = minLevel;
+        tc.baseNode = base;
+    }
+    return tc;
+}
+
 // Adds LDS slot `slot` (64 pixels x 3 sums of one tile) to the framebuffer and clears it.
 PG3_DEV void FlushSlot(double *slot, uint32_t x0, uint32_t y0, uint32_t resX, uint32_t resY, double *fb) {
     const uint32_t lane = __lane_id();
