@@ -256,13 +256,19 @@ def main():
     tflops = flops / kernel_s / 1e12
     gbs = bytes_alg / kernel_s / 1e9
     rays_per_s = (samples + 2 * segments) / wall
+    traffic = MEASURED_TRAFFIC.get((args.workload, args.kernel))
+    kernel_ms_avg = sum(kernel_ms) / len(kernel_ms)
+    roofline_hbm = ({"hbm_measured_gbs": round(traffic / (kernel_ms_avg / 1000.0) / 1e9, 1),
+                     "hbm_measured_frac": round(traffic / (kernel_ms_avg / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)}
+                    if traffic else {})
     roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
                 "traffic": MEASURED_TRAFFIC.get((args.workload, args.kernel)),
                 "compute": "fp32 VALU (no matrix work on this path); peak = MI355X FP32 157.3 TFLOP/s",
                 "hbm_model_gbs": round(gbs, 1), "hbm_model_frac": round(gbs / HBM_PEAK_GBS, 4),
                 "kernel_ms_per_step": round(sum(kernel_ms) / len(kernel_ms), 3),
-                "flop_per_segment": 2 * ray_flop + SHADE_FLOP, "segments_per_step": segments // args.steps}
+                "flop_per_segment": 2 * ray_flop + SHADE_FLOP, "segments_per_step": segments // args.steps,
+                **roofline_hbm}
     out = {"metric": "Msamples/s", "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
