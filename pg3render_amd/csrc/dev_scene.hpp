@@ -339,8 +339,8 @@ PG3_DEV F3 EmEval(const DevEm &E, F3 dir) {
 // Simple spherical sampler (em_simple_spherical_sampler.hxx) over the 2-D CDF
 // (distribution.hxx:391-455, 510-541): upper_bound over the full CDF selects the same segment as
 // the reference's 16-float hierarchical blocks.
-PG3_DEV uint32_t UpperBound(const float *cdfp, uint32_t n1, float x) {   // first index with cdf > x
-    const GlobalPtr<float> cdf = AsGlobal(cdfp);
+template <class P>
+PG3_DEV uint32_t UpperBound(P cdf, uint32_t n1, float x) {   // first index with cdf > x
     uint32_t lo = 0, len = n1;
     while (len > 0) {
         const uint32_t half = len >> 1;
@@ -349,10 +349,10 @@ PG3_DEV uint32_t UpperBound(const float *cdfp, uint32_t n1, float x) {   // firs
     }
     return lo;
 }
-PG3_DEV void Sample1D(const float *cdfp, uint32_t count, float u, float &x, uint32_t &seg, float &pdf) {
-    const GlobalPtr<float> cdf = AsGlobal(cdfp);
+template <class P>
+PG3_DEV void Sample1D(P cdf, uint32_t count, float u, float &x, uint32_t &seg, float &pdf) {
     const float ut = u * 0.999999f;
-    int32_t s = (int32_t)UpperBound(cdfp, count + 1, ut) - 1;
+    int32_t s = (int32_t)UpperBound(cdf, count + 1, ut) - 1;
     s = max(0, min(s, (int32_t)count - 1));
     seg = (uint32_t)s;
     const float c0 = cdf[seg], c1 = cdf[seg + 1];
@@ -362,12 +362,14 @@ PG3_DEV void Sample1D(const float *cdfp, uint32_t count, float u, float &x, uint
     x = (seg + off) / count;
 }
 PG3_DEV float SinMidTheta(uint32_t h, uint32_t sy) { return sinf(kPiF * (sy + 0.5f) / h); }  // :177-189
+// margLds (may be null): an LDS copy of the marginal CDF staged by the kernel.
 PG3_DEV void SimpleSample(const DevEm &E, const Frame &fr, uint32_t sides, float u0, float u1, F3 &dir,
-                          float &pdf, F3 &radCos) {                                      // :27-81
+                          float &pdf, F3 &radCos, const float *margLds = nullptr) {      // :27-81
     float u, v, mp, cp;
     uint32_t sx, sy;
-    Sample1D(E.margCdf, E.h, u0, v, sy, mp);
-    Sample1D(E.condCdf + (size_t)sy * (E.w + 1), E.w, u1, u, sx, cp);
+    if (margLds) Sample1D(margLds, E.h, u0, v, sy, mp);
+    else Sample1D(AsGlobal(E.margCdf), E.h, u0, v, sy, mp);
+    Sample1D(AsGlobal(E.condCdf + (size_t)sy * (E.w + 1)), E.w, u1, u, sx, cp);
     dir = LatLong2Dir(u, v);
     pdf = (mp * cp) * E.plan2ang / SinMidTheta(E.h, sy);
     const F3 rad = !E.bilinear ? Texel(E, sx, sy) : EmEvalUV(E, u, v);
@@ -375,15 +377,17 @@ PG3_DEV void SimpleSample(const DevEm &E, const Frame &fr, uint32_t sides, float
     if (((sides & kSideFront) && c > 0.0f) || ((sides & kSideBack) && c < 0.0f)) radCos = rad * fabsf(c);
     else radCos = mk(0.f);
 }
-PG3_DEV float SimplePdf(const DevEm &E, F3 dir) {                                    // :84-98, distribution.hxx:530-540
+PG3_DEV float SimplePdf(const DevEm &E, F3 dir, const float *margLds = nullptr) {    // :84-98, distribution.hxx:530-540
     float u, v;
     Dir2LatLong(dir, u, v);
     const uint32_t iu = min((uint32_t)fmaxf(0.f, u * E.w), E.w - 1);
     const uint32_t iv = min((uint32_t)fmaxf(0.f, v * E.h), E.h - 1);
     const GlobalPtr<float> cc = AsGlobal(E.condCdf + (size_t)iv * (E.w + 1));
-    const GlobalPtr<float> mc = AsGlobal(E.margCdf);
     const float pc = (cc[iu + 1] - cc[iu]) * E.w;
-    const float pm = (mc[iv + 1] - mc[iv]) * E.h;
+    float m0, m1;
+    if (margLds) { m0 = margLds[iv]; m1 = margLds[iv + 1]; }
+    else { const GlobalPtr<float> mc = AsGlobal(E.margCdf); m0 = mc[iv]; m1 = mc[iv + 1]; }
+    const float pm = (m1 - m0) * E.h;
     const uint32_t seg = min((uint32_t)(v * E.h), E.h - 1u);
     return (pc * pm) * E.plan2ang / SinMidTheta(E.h, seg);
 }
@@ -447,6 +451,7 @@ struct TreeCache {
     const float *nodes = nullptr;
     uint32_t minLevel = 0xFFFFFFFFu;
     uint32_t baseNode = 0;
+    const float *marg = nullptr;   // LDS copy of the simple sampler's marginal CDF (or null)
 };
 // One level of PickTriangle: children [first, first + cc) with weights at w (12 floats per node).
 template <class P>
@@ -630,15 +635,16 @@ PG3_DEV float SteerIrradiance(const DevEm &E, F3 n, uint32_t sides) {           
 
 // EnvironmentMap::PdfW of the active sampler (em.hxx:113-120)
 // `kind` is the sampler (E.sampler at run time, or a constant in specialised kernels).
-PG3_DEV float EmPdf(const DevEm &E, int kind, F3 dir, const Frame &fr, uint32_t sides, const EmPair *pre = nullptr) {
-    if (kind == kEmSimple) return SimplePdf(E, dir);
+PG3_DEV float EmPdf(const DevEm &E, int kind, F3 dir, const Frame &fr, uint32_t sides, const EmPair *pre = nullptr,
+                     const float *margLds = nullptr) {
+    if (kind == kEmSimple) return SimplePdf(E, dir, margLds);
     if (kind == kEmSteerable) return SteerPdf(E, dir, fr.z, sides, pre);
     return CosSpherePdfW(sides & kSideFront, sides & kSideBack, ToLocal(fr, dir));
 }
 // EnvironmentMapSampler::Sample of the active sampler (em_sampler.hxx:49-64)
 PG3_DEV bool EmSample(const DevEm &E, int kind, const Frame &fr, uint32_t sides, float u0, float u1, float u2, F3 &dir,
                       float &pdf, F3 &radCos, const TreeCache &tc = TreeCache{}) {
-    if (kind == kEmSimple) { SimpleSample(E, fr, sides, u0, u1, dir, pdf, radCos); return true; }
+    if (kind == kEmSimple) { SimpleSample(E, fr, sides, u0, u1, dir, pdf, radCos, tc.marg); return true; }
     if (kind == kEmSteerable) return SteerSample(E, fr.z, sides, u2, u0, u1, dir, pdf, radCos, tc);
     const F3 w = SampleCosSphereParamPdfW(u0, u1, u2, sides & kSideFront, sides & kSideBack, pdf);
     dir = ToWorld(fr, w);
@@ -699,7 +705,7 @@ PG3_DEV F3 BgRadiance(const SceneView &S, const LightSpec &ls, F3 dir) {
 // ... and its PDF w.r.t. a shading frame/material
 PG3_DEV float BgPdf(const SceneView &S, const LightSpec &ls, F3 dir, const Frame &fr, uint32_t sides,
                     const EmPair *pre = nullptr) {
-    if (ls.em != kEmAbsent) return EmPdf(S.em, ls.em, dir, fr, sides, pre);
+    if (ls.em != kEmAbsent) return EmPdf(S.em, ls.em, dir, fr, sides, pre, ls.tree.marg);
     return CosSpherePdfW(sides & kSideFront, sides & kSideBack, ToLocal(fr, dir));
 }
 // AreaLight::GetEmmision PDF part, lights.hxx:88-99
