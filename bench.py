@@ -63,6 +63,29 @@ def defaults_measure(ctx):
     return out
 
 
+def wavefront_measure(ctx, steps=2):
+    """The wavefront pipeline (pg3g_set_kernel_mode(WAVEFRONT): ray generation, intersection,
+    shading and shadow kernels with path state and ray queues in HBM) on the default workload,
+    beside the fused kernel that bench.py times; queue bytes from the fixed SoA record model."""
+    from pg3render_amd import SceneBuild, _ffi
+    sb = SceneBuild.workload(2, seed=1)
+    ctx.set_kernel_mode(_ffi.KERNEL_WAVEFRONT)
+    try:
+        ctx.upload(sb, 1)
+        n = sb.spp
+        ctx.render(0, n, 1)
+        ctx.synchronize()
+        ctx.reset_counters()
+        ms = [ctx.render_timed((i + 1) * n, n, 1) for i in range(steps)]
+        seg, smp = ctx.counters()
+    finally:
+        ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    t = sum(ms) / len(ms) / 1000.0
+    qbytes = (BYTES_PER_SAMPLE * smp + BYTES_PER_SEGMENT * seg) / steps
+    return {"ms_per_frame": round(t * 1000.0, 3), "msamples_per_s": round(smp / steps / t / 1e6, 1),
+            "queue_model_gbs": round(qbytes / t / 1e9, 1), "queue_model_frac": round(qbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def host_threads():
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():       # the host share given to this job
@@ -279,6 +302,7 @@ def main():
     if not args.no_batch:
         out["batched"] = batch_measure(ctx, args.batch_tune)
     out["reference_defaults"] = defaults_measure(ctx)
+    out["wavefront_pipeline"] = wavefront_measure(ctx)
     if not args.no_diff:
         out["differentiation"] = diff_measure(ctx)
         if args.inverse_steps > 0:
