@@ -136,10 +136,13 @@ __global__ void __launch_bounds__(256) WfIntersect(SceneView S, WfQueues Q, uint
 }
 
 // Shading stage, specialised like the fused kernel (kEmKind: EM sampler or kEmAbsent; kOneLight)
-// and kept at 4 waves per SIMD.
+// and kept at 4 waves per SIMD.  The scene is read through a device copy (held by value in
+// kernel arguments its fields would occupy SGPRs and spill), and a scene whose only light is a
+// steerable EM takes the vertex that shares the EM lookups of escaped paths and light samples.
 template <int kEmKind, bool kOneLight>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-WfShade(SceneView S, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce) {
+WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce) {
+    const SceneView &S = *Sp;
     const LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
     const uint32_t count = Q.counts[qi];
     const uint32_t *q = Q.rayQ[qi];
@@ -163,7 +166,7 @@ WfShade(SceneView S, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce) {
             h.n = mk(ha.x, ha.y, ha.z);
             h.mat = Q.hMat[i];
             RngKey key{S.k0, S.k1, pixel, it, S.sceneId << 16};
-            cont = ShadeVertex(S, spec, key, ps, h.mat >= 0, h, sr, segments);
+            cont = ShadeVertex<kOneLight && kEmKind == kEmSteerable>(S, spec, key, ps, h.mat >= 0, h, sr, segments);
             shadow = sr.valid;
             StorePath(Q, i, ps, pixel, it);
         }
@@ -268,7 +271,7 @@ struct WfStorage {
 
 }  // namespace
 
-using ShadeFn = void (*)(SceneView, WfQueues, uint32_t, uint32_t, uint32_t);
+using ShadeFn = void (*)(const SceneView *, WfQueues, uint32_t, uint32_t, uint32_t);
 static ShadeFn PickShadeKernel(const SceneView &S) {
     const bool one = (S.numLights == 1);
     const int em = SceneLightSpec(S).em;
@@ -322,7 +325,7 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
             hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, qi);
             hipLaunchKernelGGL(WfSetCounts, dim3(1), dim3(64), 0, stream, Q.counts, 0u, 0u, 0u, 0u,
                                (qo == 0 ? 1u : 2u) | 4u);
-            hipLaunchKernelGGL(shade, grid, block, 0, stream, S, Q, qi, qo, b);
+            hipLaunchKernelGGL(shade, grid, block, 0, stream, Sdev, Q, qi, qo, b);
             hipLaunchKernelGGL(WfShadow, grid, block, 0, stream, S, Q);
         }
         hipLaunchKernelGGL(WfAccumulate, grid, block, 0, stream, Q, count, fb, counters);
