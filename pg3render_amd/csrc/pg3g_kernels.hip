@@ -70,7 +70,6 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
 // EM's triangle tree in LDS once (kTreeLdsNodes nodes, 60 KiB): the tree descent of every light
 // sample then gathers from LDS instead of from the L2.  With the simple sampler the marginal CDF
 // is staged instead (up to kMargLdsMax entries, an EM of up to 4095 rows).
-constexpr uint32_t kMargLdsMax = 4096;
 #ifndef PG3_SPLIT_WAVES_PER_EU
 #define PG3_SPLIT_WAVES_PER_EU 4
 #endif
@@ -102,13 +101,8 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
         __syncthreads();
     }
     if constexpr (kEmKind == kEmSimple) {
-        // the simple sampler's marginal CDF (h + 1 floats), searched by every light sample
         __shared__ float margLds[kMargLdsMax];
-        if (S.em.h + 1u <= kMargLdsMax) {
-            const GlobalPtr<float> src = AsGlobal(S.em.margCdf);
-            for (uint32_t i = threadIdx.x; i <= S.em.h; i += blockDim.x) margLds[i] = src[i];
-            spec.tree.marg = margLds;
-        }
+        StageMarginal(S.em, margLds, spec.tree);
         __syncthreads();
     }
     const uint32_t tilesX = (S.resX + 7u) / 8u, numTiles = tilesX * ((S.resY + 7u) / 8u);

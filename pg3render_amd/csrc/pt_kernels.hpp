@@ -874,6 +874,17 @@ PG3_DEV TreeCache StageTree(const DevEm &E, F4v *lds) {
     return tc;
 }
 
+// Stages the simple sampler's marginal CDF (h + 1 floats, searched by every light sample) in
+// `lds` when it fits in kMargLdsMax entries (an EM of up to 4095 rows), cooperatively by the
+// workgroup; the caller synchronises the workgroup before the staged table is read.
+constexpr uint32_t kMargLdsMax = 4096;
+PG3_DEV void StageMarginal(const DevEm &E, float *lds, TreeCache &tc) {
+    if (E.h + 1u > kMargLdsMax) return;
+    const GlobalPtr<float> src = AsGlobal(E.margCdf);
+    for (uint32_t i = threadIdx.x; i <= E.h; i += blockDim.x) lds[i] = src[i];
+    tc.marg = lds;
+}
+
 // Adds LDS slot `slot` (64 pixels x 3 sums of one tile) to the framebuffer and clears it.
 PG3_DEV void FlushSlot(double *slot, uint32_t x0, uint32_t y0, uint32_t resX, uint32_t resY, double *fb) {
     const uint32_t lane = __lane_id();

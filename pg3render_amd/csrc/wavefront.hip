@@ -1,14 +1,20 @@
 // Wavefront organisation of the path tracer for gfx950.
 //
 // A batch of up to `capacity` camera samples is traced vertex by vertex with four kernels per
-// bounce: Intersect (closest hit of every queued ray), Shade (emission + MIS, light sample, BSDF
+// bounce: Intersect (closest hit of every live path), Shade (emission + MIS, light sample, BSDF
 // sample), Shadow (occlusion of the light samples) and, after the last bounce, Accumulate (FP64
-// framebuffer).  Path state, hit records and both queues live in HBM as 16-byte-per-lane
-// records, so every kernel reads and writes fully coalesced cache lines.  Queue appends are
-// compacted per wavefront: a 64-lane ballot, one atomic per wave and a popcount prefix give each
-// lane its slot.  Every launch is a grid-stride loop over a device-side count, so the host never
-// waits for the GPU between bounces and the whole batch is a fixed launch sequence (capturable
-// in a HIP graph).  Per-vertex shading is the same ShadeVertex the fused kernel uses.
+// framebuffer).  The queues hold the path state itself, not indices into it: the live paths of a
+// bounce are six SoA arrays of 16-byte records in queue order, ping-ponged between two sets, so
+// Intersect and Shade read lane j's path at position j and every load is a fully coalesced cache
+// line.  Shade writes each continuing path to its slot in the other set, a finished path's radiance
+// to the finished array (indexed by the sample's position in the batch) and a light sample to the
+// shadow queue; Shadow adds an unoccluded sample's contribution to the path's radiance wherever
+// Shade put it, and Accumulate sums each pixel's samples of the batch without atomics.  Queue appends
+// are compacted per workgroup: a 64-lane ballot per wave, an LDS scan of the wave counts, one
+// atomic per queue per workgroup and a popcount prefix give each lane its slot, so the lanes of a
+// wave write consecutive records.  Every launch is a grid-stride loop over a device-side count, so
+// the host never waits for the GPU between bounces and the whole batch is a fixed launch sequence
+// (capturable in a HIP graph).  Per-vertex shading is the same ShadeVertex the fused kernel uses.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,55 +25,85 @@
 
 namespace pg3 {
 
+// Path state of the live paths of one bounce, one record per path and field, in queue order.
+struct WfPaths {
+    float4 *A;   // org.xyz, tmin
+    float4 *B;   // dir.xyz, prevBsdfPdf
+    float4 *C;   // beta.xyz, flags
+    float4 *D;   // eThr.xyz, iteration (the path length is the bounce index)
+    float4 *E;   // L.xyz, pixel
+    float4 *F;   // prevN.xyz, light-cache sum
+};
+
 struct WfQueues {
     uint32_t capacity;
-    // path state (one entry per sample of the batch)
-    float4 *pA;    // org.xyz, tmin
-    float4 *pB;    // dir.xyz, prevBsdfPdf
-    float4 *pC;    // beta.xyz, flags
-    float4 *pD;    // eThr.xyz, iteration (the path length is the bounce index)
-    float4 *pE;    // L.xyz, pixel
-    float4 *pF;    // prevN.xyz, light-cache sum
-    // hit records
-    float4 *hA;    // n.xyz, dist (dist < 0: miss)
+    WfPaths P[2];    // live paths entering the current bounce and the next one (ping-pong)
+    float4 *hA;      // hit record of live path j: n.xyz, dist (dist < 0: miss)
     int32_t *hMat;
-    // ray queues (ping-pong) and the shadow queue
-    uint32_t *rayQ[2];
-    float4 *sA;    // p.xyz, tmax
-    float4 *sB;    // dir.xyz, path index
-    float4 *sC;    // contrib.xyz
-    uint32_t *counts;   // [0], [1]: ray queues, [2]: shadow queue, [3]: segments (u32 per batch)
+    float4 *sA;      // shadow queue: p.xyz, tmax
+    float4 *sB;      //   dir.xyz, destination of the contribution (below)
+    float4 *sC;      //   contrib.xyz
+    float4 *done;    // finished paths by sample position in the batch: L.xyz
+    uint32_t *counts;   // [0], [1]: live paths in P[0], P[1]; [2]: shadow queue; [3]: segments
 };
+
+// Destination of a shadow contribution: slot of the continuing path in the next live set, or
+// the sample's position in the finished array with this bit set.
+constexpr uint32_t kDoneBit = 0x80000000u;
+constexpr int kWfBlock = 256;
 
 __device__ __forceinline__ float AsF(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t AsU(float f) { return __float_as_uint(f); }
 
-// Appends the calling lane to a queue if `pred`; one atomic per wavefront.
-__device__ __forceinline__ uint32_t WaveAppend(bool pred, uint32_t *counter) {
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0xFFFFFFFFu;
-    const uint32_t lane = __lane_id();
-    const uint64_t active = __ballot(1);
-    const uint32_t leader = __ffsll((unsigned long long)active) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    const uint64_t below = mask & ((1ull << lane) - 1ull);
-    return pred ? base + (uint32_t)__popcll(below) : 0xFFFFFFFFu;
+// Appends the calling lane to queue q if p[q], for kN queues at once: a ballot per wavefront, the
+// wave counts scanned in LDS and one atomic per queue per workgroup; s[q] receives the lane's slot
+// (0xFFFFFFFF where p[q] is false).  Lanes of one wave get consecutive slots.  A wave-level atomic
+// would make every wavefront of the grid contend for the same few counters.  Every lane of the
+// workgroup calls it (the caller's loop is workgroup-uniform) and kWaves is the workgroup's size
+// in wavefronts.
+template <int kWaves, int kN>
+__device__ __forceinline__ void BlockAppend(const bool (&p)[kN], uint32_t *const (&c)[kN], uint32_t (&s)[kN]) {
+    __shared__ uint32_t cnt[kN][kWaves + 1];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint64_t m[kN];
+#pragma unroll
+    for (int q = 0; q < kN; ++q) m[q] = __ballot(p[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < kN; ++q) cnt[q][wave] = (uint32_t)__popcll(m[q]);
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kN) {
+        uint32_t *k = cnt[threadIdx.x];
+        uint32_t total = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t n = k[w];
+            k[w] = total;
+            total += n;
+        }
+        k[kWaves] = total ? atomicAdd(c[threadIdx.x], total) : 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int q = 0; q < kN; ++q)
+        s[q] = p[q] ? cnt[q][kWaves] + cnt[q][wave] + (uint32_t)__popcll(m[q] & below) : 0xFFFFFFFFu;
+    __syncthreads();   // the next call rewrites the counts
 }
 
-__device__ __forceinline__ void StorePath(const WfQueues &Q, uint32_t i, const PathState &ps, uint32_t pixel,
+__device__ __forceinline__ void StorePath(const WfPaths &P, uint32_t i, const PathState &ps, uint32_t pixel,
                                           uint32_t iteration) {
-    Q.pA[i] = make_float4(ps.org.x, ps.org.y, ps.org.z, ps.tmin);
-    Q.pB[i] = make_float4(ps.dir.x, ps.dir.y, ps.dir.z, ps.prevBsdfPdf);
-    Q.pC[i] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, AsF(ps.flags));
-    Q.pD[i] = make_float4(ps.eThr.x, ps.eThr.y, ps.eThr.z, AsF(iteration));
-    Q.pE[i] = make_float4(ps.L.x, ps.L.y, ps.L.z, AsF(pixel));
-    Q.pF[i] = make_float4(ps.prevN.x, ps.prevN.y, ps.prevN.z, ps.lcSum);
+    P.A[i] = make_float4(ps.org.x, ps.org.y, ps.org.z, ps.tmin);
+    P.B[i] = make_float4(ps.dir.x, ps.dir.y, ps.dir.z, ps.prevBsdfPdf);
+    P.C[i] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, AsF(ps.flags));
+    P.D[i] = make_float4(ps.eThr.x, ps.eThr.y, ps.eThr.z, AsF(iteration));
+    P.E[i] = make_float4(ps.L.x, ps.L.y, ps.L.z, AsF(pixel));
+    P.F[i] = make_float4(ps.prevN.x, ps.prevN.y, ps.prevN.z, ps.lcSum);
 }
-__device__ __forceinline__ void LoadPath(const WfQueues &Q, uint32_t i, PathState &ps, uint32_t &pixel,
+__device__ __forceinline__ void LoadPath(const WfPaths &P, uint32_t i, PathState &ps, uint32_t &pixel,
                                          uint32_t &iteration) {
-    const float4 a = Q.pA[i], b = Q.pB[i], c = Q.pC[i], d = Q.pD[i], e = Q.pE[i], f = Q.pF[i];
+    const float4 a = P.A[i], b = P.B[i], c = P.C[i], d = P.D[i], e = P.E[i], f = P.F[i];
     ps.org = mk(a.x, a.y, a.z);
     ps.tmin = a.w;
     ps.dir = mk(b.x, b.y, b.z);
@@ -82,7 +118,7 @@ __device__ __forceinline__ void LoadPath(const WfQueues &Q, uint32_t i, PathStat
     ps.lcSum = f.w;
 }
 
-// Sample s of the batch -> (pixel, iteration): iterations are the slow index, and pixels of one
+// Sample s of the render call -> (pixel, iteration): iterations are the slow index, and pixels of one
 // iteration are enumerated in 8x8 tiles so a wavefront starts with 64 neighbouring camera rays.
 __device__ __forceinline__ void SampleToPixel(const SceneView &S, uint64_t s, uint32_t firstIter, uint32_t &pixel,
                                               uint32_t &iteration) {
@@ -95,43 +131,51 @@ __device__ __forceinline__ void SampleToPixel(const SceneView &S, uint64_t s, ui
     const uint32_t y = (tile / tilesX) * 8u + (lane >> 3);
     pixel = (x < S.resX && y < S.resY) ? y * S.resX + x : 0xFFFFFFFFu;
 }
-
-__global__ void __launch_bounds__(256) WfGenerate(SceneView S, WfQueues Q, uint64_t batchBase, uint32_t count,
+// The inverse: sample of the render call of an (in-image) pixel and iteration.
+__device__ __forceinline__ uint64_t PixelToSample(const SceneView &S, uint32_t pixel, uint32_t iteration,
                                                   uint32_t firstIter) {
-    uint32_t n = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        uint32_t pixel, it;
-        SampleToPixel(S, batchBase + i, firstIter, pixel, it);
+    const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
+    const uint64_t slots = (uint64_t)tilesX * tilesY * 64u;
+    const uint32_t x = pixel % S.resX, y = pixel / S.resX;
+    const uint32_t r = ((y >> 3) * tilesX + (x >> 3)) * 64u + (y & 7u) * 8u + (x & 7u);
+    return (uint64_t)(iteration - firstIter) * slots + r;
+}
+
+// Camera rays of samples [batchBase, batchBase + count) into live set 0; samples of the tile
+// padding outside the image are not enqueued.
+__global__ void __launch_bounds__(kWfBlock) WfGenerate(SceneView S, WfQueues Q, uint64_t batchBase, uint32_t count,
+                                                       uint32_t firstIter) {
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < count; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t pixel = 0xFFFFFFFFu, it = 0;
+        if (i < count) SampleToPixel(S, batchBase + i, firstIter, pixel, it);
         const bool valid = pixel != 0xFFFFFFFFu;
-        PathState ps;
+        const bool p[1] = {valid};
+        uint32_t *const c[1] = {&Q.counts[0]};
+        uint32_t s[1];
+        BlockAppend<kWfBlock / 64, 1>(p, c, s);
         if (valid) {
             RngKey key{S.k0, S.k1, pixel, it, S.sceneId << 16};
             const float jx = RngGet(key, 0, 0, 0), jy = RngGet(key, 0, 0, 1);   // path_tracer_base.hxx:74-77
             F3 org, dir;
             GenerateRay(S, float(pixel % S.resX) + jx, float(pixel / S.resX) + jy, org, dir);
+            PathState ps;
             InitPath(ps, org, dir);
-        } else {
-            InitPath(ps, mk(0.f), mk(0.f, 0.f, 1.f));
+            StorePath(Q.P[0], s[0], ps, pixel, it);
         }
-        StorePath(Q, i, ps, valid ? pixel : 0xFFFFFFFFu, it);
-        const uint32_t slot = WaveAppend(valid, &Q.counts[0]);
-        if (valid) Q.rayQ[0][slot] = i;
-        n += valid;
     }
-    (void)n;
 }
 
-__global__ void __launch_bounds__(256) WfIntersect(SceneView S, WfQueues Q, uint32_t qi) {
+__global__ void __launch_bounds__(kWfBlock) WfIntersect(SceneView S, WfQueues Q, uint32_t qi) {
     const uint32_t count = Q.counts[qi];
-    const uint32_t *q = Q.rayQ[qi];
+    const WfPaths P = Q.P[qi];
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
-        const uint32_t i = q[j];
-        const float4 a = Q.pA[i], b = Q.pB[i];
+        const float4 a = P.A[j], b = P.B[j];
         Hit h;
         h.dist = 1e36f;
         const bool hit = Intersect(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, h);
-        Q.hA[i] = make_float4(h.n.x, h.n.y, h.n.z, hit ? h.dist : -1.f);
-        Q.hMat[i] = hit ? h.mat : -1;
+        Q.hA[j] = make_float4(h.n.x, h.n.y, h.n.z, hit ? h.dist : -1.f);
+        Q.hMat[j] = hit ? h.mat : -1;
     }
 }
 
@@ -139,45 +183,62 @@ __global__ void __launch_bounds__(256) WfIntersect(SceneView S, WfQueues Q, uint
 // and kept at 4 waves per SIMD.  The scene is read through a device copy (held by value in
 // kernel arguments its fields would occupy SGPRs and spill), and a scene whose only light is a
 // steerable EM takes the vertex that shares the EM lookups of escaped paths and light samples.
+// Unlike the fused kernel it does not stage the steerable tree in LDS: that needs one 1024-lane
+// workgroup per CU, whose queue-append barriers then hold all 16 wavefronts of the CU in step,
+// and measured no faster.  The simple sampler's marginal CDF is staged.
+template <int kEmKind>
+constexpr uint32_t WfShadeBlock() { return (uint32_t)kWfBlock; }
+
 template <int kEmKind, bool kOneLight>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce) {
+__global__ void __launch_bounds__((WfShadeBlock<kEmKind>())) __attribute__((amdgpu_waves_per_eu(4)))
+WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce, uint64_t batchBase,
+        uint32_t firstIter) {
     const SceneView &S = *Sp;
-    const LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
+    LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
+    if constexpr (kEmKind == kEmSimple) {
+        __shared__ float margLds[kMargLdsMax];
+        StageMarginal(S.em, margLds, spec.tree);
+        __syncthreads();
+    }
     const uint32_t count = Q.counts[qi];
-    const uint32_t *q = Q.rayQ[qi];
+    const WfPaths Pi = Q.P[qi], Po = Q.P[qo];
     uint32_t segments = 0;
     for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < count; j0 += gridDim.x * blockDim.x) {
         const uint32_t j = j0 + threadIdx.x;
         const bool active = j < count;
-        bool cont = false, shadow = false;
-        uint32_t i = 0;
+        bool cont = false;
+        uint32_t pixel = 0xFFFFFFFFu, it = 0;
+        PathState ps;
         ShadowReq sr;
         sr.valid = false;
         if (active) {
-            i = q[j];
-            PathState ps;
-            uint32_t pixel, it;
-            LoadPath(Q, i, ps, pixel, it);
+            LoadPath(Pi, j, ps, pixel, it);
             ps.len = bounce;
-            const float4 ha = Q.hA[i];
+            const float4 ha = Q.hA[j];
             Hit h;
             h.dist = ha.w;
             h.n = mk(ha.x, ha.y, ha.z);
-            h.mat = Q.hMat[i];
+            h.mat = Q.hMat[j];
             RngKey key{S.k0, S.k1, pixel, it, S.sceneId << 16};
             cont = ShadeVertex<kOneLight && kEmKind == kEmSteerable>(S, spec, key, ps, h.mat >= 0, h, sr, segments);
-            shadow = sr.valid;
-            StorePath(Q, i, ps, pixel, it);
         }
-        // queue appends are wave-uniform calls (inactive tail lanes pass false)
-        const uint32_t rs = WaveAppend(cont, &Q.counts[qo]);
-        if (cont) Q.rayQ[qo][rs] = i;
-        const uint32_t ss = WaveAppend(shadow, &Q.counts[2]);
-        if (shadow) {
-            Q.sA[ss] = make_float4(sr.p.x, sr.p.y, sr.p.z, sr.tmax);
-            Q.sB[ss] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, AsF(i));
-            Q.sC[ss] = make_float4(sr.contrib.x, sr.contrib.y, sr.contrib.z, 0.f);
+        // workgroup-uniform appends (inactive tail lanes pass false)
+        const bool p[2] = {cont, sr.valid};
+        uint32_t *const c[2] = {&Q.counts[qo], &Q.counts[2]};
+        uint32_t s[2];
+        BlockAppend<WfShadeBlock<kEmKind>() / 64, 2>(p, c, s);
+        uint32_t fin = 0;
+        if (cont) {
+            StorePath(Po, s[0], ps, pixel, it);
+        } else if (active) {
+            fin = (uint32_t)(PixelToSample(S, pixel, it, firstIter) - batchBase);
+            Q.done[fin] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.f);
+        }
+        if (sr.valid) {
+            const uint32_t dst = cont ? s[0] : (fin | kDoneBit);
+            Q.sA[s[1]] = make_float4(sr.p.x, sr.p.y, sr.p.z, sr.tmax);
+            Q.sB[s[1]] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, AsF(dst));
+            Q.sC[s[1]] = make_float4(sr.contrib.x, sr.contrib.y, sr.contrib.z, 0.f);
         }
     }
     uint32_t w = segments;
@@ -186,33 +247,53 @@ WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, 
     if (__lane_id() == 0 && w) atomicAdd(&Q.counts[3], w);
 }
 
-__global__ void __launch_bounds__(256) WfShadow(SceneView S, WfQueues Q) {
+// Occlusion of the light samples of one bounce; qo is the live set the continuing paths were
+// written to.
+__global__ void __launch_bounds__(kWfBlock) WfShadow(SceneView S, WfQueues Q, uint32_t qo) {
     const uint32_t count = Q.counts[2];
+    float4 *const live = Q.P[qo].E;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
         const float4 a = Q.sA[j], b = Q.sB[j];
         if (!Occluded(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w)) {
-            const uint32_t i = AsU(b.w);
+            const uint32_t dst = AsU(b.w);
+            float4 *const L = (dst & kDoneBit) ? Q.done + (dst & ~kDoneBit) : live + dst;
             const float4 c = Q.sC[j];
-            float4 L = Q.pE[i];
-            L.x += c.x;
-            L.y += c.y;
-            L.z += c.z;
-            Q.pE[i] = L;
+            float4 v = *L;
+            v.x += c.x;
+            v.y += c.y;
+            v.z += c.z;
+            *L = v;
         }
     }
 }
 
-__global__ void __launch_bounds__(256) WfAccumulate(WfQueues Q, uint32_t count, double *__restrict__ fb,
-                                                    unsigned long long *__restrict__ counters) {
+// Adds the finished paths of samples [batchBase, batchBase + count) to the FP64 framebuffer.  Lane
+// r' owns the pixel of sample batchBase + r' and sums, in iteration order, every sample of the
+// batch on that pixel (positions r', r' + slots, ...), so each pixel is read and written once, by
+// one lane.  Every path has finished: the last bounce stops at the path-length limit.
+__global__ void __launch_bounds__(kWfBlock) WfAccumulate(SceneView S, WfQueues Q, uint64_t batchBase, uint32_t count,
+                                                         uint32_t firstIter, double *__restrict__ fb,
+                                                         unsigned long long *__restrict__ counters) {
+    const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
+    const uint32_t slots = tilesX * tilesY * 64u;
+    const uint32_t n = std::min(slots, count);
     uint32_t samples = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const float4 L = Q.pE[i];
-        const uint32_t pixel = AsU(L.w);
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        uint32_t pixel, it;
+        SampleToPixel(S, batchBase + r, firstIter, pixel, it);
         if (pixel == 0xFFFFFFFFu) continue;
-        atomicAdd(fb + (size_t)pixel * 3 + 0, (double)L.x);
-        atomicAdd(fb + (size_t)pixel * 3 + 1, (double)L.y);
-        atomicAdd(fb + (size_t)pixel * 3 + 2, (double)L.z);
-        ++samples;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (uint32_t i = r; i < count; i += slots) {
+            const float4 L = Q.done[i];
+            sx += (double)L.x;
+            sy += (double)L.y;
+            sz += (double)L.z;
+            ++samples;
+        }
+        double *const f = fb + (size_t)pixel * 3;
+        f[0] += sx;
+        f[1] += sy;
+        f[2] += sz;
     }
     uint32_t w = samples;
 #pragma unroll
@@ -223,13 +304,9 @@ __global__ void __launch_bounds__(256) WfAccumulate(WfQueues Q, uint32_t count, 
     }
 }
 
-__global__ void WfSetCounts(uint32_t *counts, uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t mask) {
-    if (threadIdx.x == 0) {
-        if (mask & 1) counts[0] = a;
-        if (mask & 2) counts[1] = b;
-        if (mask & 4) counts[2] = c;
-        if (mask & 8) counts[3] = d;
-    }
+// Zeroes the counters selected by `mask` (bit k: counts[k]).
+__global__ void WfClearCounts(uint32_t *counts, uint32_t mask) {
+    if (threadIdx.x < 4 && (mask >> threadIdx.x & 1u)) counts[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -249,21 +326,22 @@ struct WfStorage {
         if (mem) (void)hipFree(mem);
         mem = nullptr;
         const size_t f4 = sizeof(float4) * (size_t)cap;
-        const size_t bytes = 6 * f4 + f4 + sizeof(int32_t) * (size_t)cap +
-                             2 * sizeof(uint32_t) * (size_t)cap + 3 * f4 + 64 + 17 * 256;  // + alignment
+        // two live sets of 6 records, the hit record (16 + 4 B), 3 shadow records, 1 finished record
+        const size_t bytes = 12 * f4 + f4 + sizeof(int32_t) * (size_t)cap + 3 * f4 + f4 + 64 + 20 * 256;
         hipError_t e = hipMalloc(&mem, bytes);
         if (e != hipSuccess) return e;
         capacity = cap;
         char *p = static_cast<char *>(mem);
         auto take = [&](size_t n) { char *r = p; p += (n + 255) & ~size_t(255); return r; };
         q.capacity = cap;
-        q.pA = (float4 *)take(f4); q.pB = (float4 *)take(f4); q.pC = (float4 *)take(f4);
-        q.pD = (float4 *)take(f4); q.pE = (float4 *)take(f4); q.pF = (float4 *)take(f4);
+        for (WfPaths &P : q.P) {
+            P.A = (float4 *)take(f4); P.B = (float4 *)take(f4); P.C = (float4 *)take(f4);
+            P.D = (float4 *)take(f4); P.E = (float4 *)take(f4); P.F = (float4 *)take(f4);
+        }
         q.hA = (float4 *)take(f4);
         q.hMat = (int32_t *)take(sizeof(int32_t) * (size_t)cap);
-        q.rayQ[0] = (uint32_t *)take(sizeof(uint32_t) * (size_t)cap);
-        q.rayQ[1] = (uint32_t *)take(sizeof(uint32_t) * (size_t)cap);
         q.sA = (float4 *)take(f4); q.sB = (float4 *)take(f4); q.sC = (float4 *)take(f4);
+        q.done = (float4 *)take(f4);
         q.counts = (uint32_t *)take(64);
         return hipSuccess;
     }
@@ -271,23 +349,27 @@ struct WfStorage {
 
 }  // namespace
 
-using ShadeFn = void (*)(const SceneView *, WfQueues, uint32_t, uint32_t, uint32_t);
-static ShadeFn PickShadeKernel(const SceneView &S) {
+using ShadeFn = void (*)(const SceneView *, WfQueues, uint32_t, uint32_t, uint32_t, uint64_t, uint32_t);
+struct ShadeLaunch {
+    ShadeFn fn;
+    uint32_t block;
+};
+static ShadeLaunch PickShadeKernel(const SceneView &S) {
     const bool one = (S.numLights == 1);
     const int em = SceneLightSpec(S).em;
 #define PG3_WF_CASE(K) \
-    if (em == (K)) return one ? WfShade<(K), true> : WfShade<(K), false>;
+    if (em == (K)) return {one ? WfShade<(K), true> : WfShade<(K), false>, WfShadeBlock<(K)>()};
     PG3_WF_CASE(kEmAbsent)
     PG3_WF_CASE(kEmCosine)
     PG3_WF_CASE(kEmSimple)
     PG3_WF_CASE(kEmSteerable)
 #undef PG3_WF_CASE
-    return nullptr;
+    return {nullptr, 0u};
 }
 
 struct WavefrontContext {
     WfStorage st;
-    uint32_t capacity = 1u << 21;   // samples per batch (~2M paths, ~300 MB of queues)
+    uint32_t capacity = 1u << 21;   // samples per batch (~2M paths, ~580 MB of queues)
     uint32_t gridBlocks = 0;
 };
 
@@ -313,22 +395,22 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
     hipError_t e = W->st.Ensure(cap);
     if (e != hipSuccess) return e;
     const WfQueues &Q = W->st.q;
-    const ShadeFn shade = PickShadeKernel(S);
-    if (!shade) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, work, numCUs, stream);
-    const dim3 grid(W->gridBlocks), block(256);
+    const ShadeLaunch shade = PickShadeKernel(S);
+    if (!shade.fn) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, work, numCUs, stream);
+    const dim3 grid(W->gridBlocks), block(kWfBlock);
     for (uint64_t base = 0; base < total; base += cap) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(cap, total - base);
-        hipLaunchKernelGGL(WfSetCounts, dim3(1), dim3(64), 0, stream, Q.counts, 0u, 0u, 0u, 0u, 15u);
+        hipLaunchKernelGGL(WfClearCounts, dim3(1), dim3(64), 0, stream, Q.counts, 0xFu);
         hipLaunchKernelGGL(WfGenerate, grid, block, 0, stream, S, Q, base, count, firstIter);
         for (uint32_t b = 1; b <= S.maxPL; ++b) {
             const uint32_t qi = (b - 1) & 1u, qo = qi ^ 1u;
             hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, qi);
-            hipLaunchKernelGGL(WfSetCounts, dim3(1), dim3(64), 0, stream, Q.counts, 0u, 0u, 0u, 0u,
-                               (qo == 0 ? 1u : 2u) | 4u);
-            hipLaunchKernelGGL(shade, grid, block, 0, stream, Sdev, Q, qi, qo, b);
-            hipLaunchKernelGGL(WfShadow, grid, block, 0, stream, S, Q);
+            hipLaunchKernelGGL(WfClearCounts, dim3(1), dim3(64), 0, stream, Q.counts, (1u << qo) | 4u);
+            hipLaunchKernelGGL(shade.fn, dim3(W->gridBlocks * (uint32_t)kWfBlock / shade.block), dim3(shade.block), 0,
+                               stream, Sdev, Q, qi, qo, b, base, firstIter);
+            hipLaunchKernelGGL(WfShadow, grid, block, 0, stream, S, Q, qo);
         }
-        hipLaunchKernelGGL(WfAccumulate, grid, block, 0, stream, Q, count, fb, counters);
+        hipLaunchKernelGGL(WfAccumulate, grid, block, 0, stream, S, Q, base, count, firstIter, fb, counters);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

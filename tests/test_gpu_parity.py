@@ -220,9 +220,15 @@ def test_workload2_image_parity(ctx, kernel):
     assert_image_parity(g, o, frac=0.03)
 
 
-def test_wavefront_equals_megakernel(ctx):
-    """Both kernel organisations run the same per-vertex code on the same RNG streams."""
-    sb = SceneBuild.workload(2, seed=25, res=64)
+@pytest.mark.parametrize("workload,sampler", [(2, None), (2, _ffi.EM_SIMPLE), (2, _ffi.EM_COSINE), (1, None), (3, None)],
+                         ids=["steerable_em", "simple_em", "cosine_em", "area_lights", "lights_and_em"])
+def test_wavefront_equals_megakernel(ctx, workload, sampler):
+    """Both kernel organisations run the same per-vertex code on the same RNG streams: the
+    one-light steerable vertex, the simple sampler with its LDS-staged marginal, the cosine
+    sampler, area lights only, and area lights with an EM (light picking)."""
+    sb = SceneBuild.workload(workload, seed=25, res=64)
+    if sampler is not None:
+        sb.desc.em.sampler = sampler
     out = []
     for k in (_ffi.KERNEL_MEGAKERNEL, _ffi.KERNEL_WAVEFRONT):
         ctx.set_kernel_mode(k)
