@@ -186,11 +186,8 @@ __global__ void __launch_bounds__(kWfBlock) WfIntersect(SceneView S, WfQueues Q,
 // Unlike the fused kernel it does not stage the steerable tree in LDS: that needs one 1024-lane
 // workgroup per CU, whose queue-append barriers then hold all 16 wavefronts of the CU in step,
 // and measured no faster.  The simple sampler's marginal CDF is staged.
-template <int kEmKind>
-constexpr uint32_t WfShadeBlock() { return (uint32_t)kWfBlock; }
-
 template <int kEmKind, bool kOneLight>
-__global__ void __launch_bounds__((WfShadeBlock<kEmKind>())) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(4)))
 WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce, uint64_t batchBase,
         uint32_t firstIter) {
     const SceneView &S = *Sp;
@@ -226,7 +223,7 @@ WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, 
         const bool p[2] = {cont, sr.valid};
         uint32_t *const c[2] = {&Q.counts[qo], &Q.counts[2]};
         uint32_t s[2];
-        BlockAppend<WfShadeBlock<kEmKind>() / 64, 2>(p, c, s);
+        BlockAppend<kWfBlock / 64, 2>(p, c, s);
         uint32_t fin = 0;
         if (cont) {
             StorePath(Po, s[0], ps, pixel, it);
@@ -350,27 +347,27 @@ struct WfStorage {
 }  // namespace
 
 using ShadeFn = void (*)(const SceneView *, WfQueues, uint32_t, uint32_t, uint32_t, uint64_t, uint32_t);
-struct ShadeLaunch {
-    ShadeFn fn;
-    uint32_t block;
-};
-static ShadeLaunch PickShadeKernel(const SceneView &S) {
+static ShadeFn PickShadeKernel(const SceneView &S) {
     const bool one = (S.numLights == 1);
     const int em = SceneLightSpec(S).em;
 #define PG3_WF_CASE(K) \
-    if (em == (K)) return {one ? WfShade<(K), true> : WfShade<(K), false>, WfShadeBlock<(K)>()};
+    if (em == (K)) return one ? WfShade<(K), true> : WfShade<(K), false>;
     PG3_WF_CASE(kEmAbsent)
     PG3_WF_CASE(kEmCosine)
     PG3_WF_CASE(kEmSimple)
     PG3_WF_CASE(kEmSteerable)
 #undef PG3_WF_CASE
-    return {nullptr, 0u};
+    return nullptr;
 }
 
 struct WavefrontContext {
     WfStorage st;
     uint32_t capacity = 1u << 21;   // samples per batch (~2M paths, ~580 MB of queues)
-    uint32_t gridBlocks = 0;
+    uint32_t gridBlocks = 0;    // grid-stride stages: 8 workgroups per CU
+    // shading stage: exactly the resident workgroups (16 wavefronts per CU at its 4 waves per
+    // SIMD), each looping over the live paths; more workgroups measured slower (launch and
+    // set-up per workgroup), e.g. 25.0 ms per C2 frame at 8 per CU against 23.2 at 4
+    uint32_t shadeBlocks = 0;
 };
 
 WavefrontContext *CreateWavefrontContext() { return new (std::nothrow) WavefrontContext(); }
@@ -387,6 +384,7 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         W->gridBlocks = (uint32_t)cus * 8u;
+        W->shadeBlocks = (uint32_t)cus * 16u / (kWfBlock / 64u);
     }
     const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
     const uint64_t slotsPerIter = (uint64_t)tilesX * tilesY * 64u;
@@ -395,8 +393,8 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
     hipError_t e = W->st.Ensure(cap);
     if (e != hipSuccess) return e;
     const WfQueues &Q = W->st.q;
-    const ShadeLaunch shade = PickShadeKernel(S);
-    if (!shade.fn) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, work, numCUs, stream);
+    const ShadeFn shade = PickShadeKernel(S);
+    if (!shade) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, work, numCUs, stream);
     const dim3 grid(W->gridBlocks), block(kWfBlock);
     for (uint64_t base = 0; base < total; base += cap) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(cap, total - base);
@@ -406,8 +404,8 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
             const uint32_t qi = (b - 1) & 1u, qo = qi ^ 1u;
             hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, qi);
             hipLaunchKernelGGL(WfClearCounts, dim3(1), dim3(64), 0, stream, Q.counts, (1u << qo) | 4u);
-            hipLaunchKernelGGL(shade.fn, dim3(W->gridBlocks * (uint32_t)kWfBlock / shade.block), dim3(shade.block), 0,
-                               stream, Sdev, Q, qi, qo, b, base, firstIter);
+            hipLaunchKernelGGL(shade, dim3(W->shadeBlocks), block, 0, stream, Sdev, Q, qi, qo, b, base,
+                               firstIter);
             hipLaunchKernelGGL(WfShadow, grid, block, 0, stream, S, Q, qo);
         }
         hipLaunchKernelGGL(WfAccumulate, grid, block, 0, stream, S, Q, base, count, firstIter, fb, counters);
