@@ -27,7 +27,8 @@ typedef struct pg3g_scene_build pg3g_scene_build;
  *   4: workload-1 geometry lit only by a 128x64 EM, two GGX-conductor spheres with RGB tints;
  *      256x256, maxpl 8 (differentiation).
  *   5: a box holding 4 GGX-conductor spheres, 128x64 EM of 3 Gaussian lobes on a sky gradient,
- *      view `variant` of 8; 256x256, maxpl 6 (inverse rendering).
+ *      view `variant` of 8 (the same scene for every variant: spheres and EM depend on the seed
+ *      only); 256x256, maxpl 6 (inverse rendering).
  * res_override > 0 replaces both resolution axes (tests render smaller images).
  * Render parameters are set to the workload's (pt, -sb 1, -iic 0, minpl 1). */
 int pg3g_build_workload_scene(int workload, uint64_t seed, uint32_t variant, uint32_t res_override,

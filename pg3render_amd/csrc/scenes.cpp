@@ -318,6 +318,9 @@ int BuildWorkload(pg3g_scene_build &b, int workload, uint64_t seed, uint32_t var
     }
     case 5: {
         if (variant >= 8) return PG3G_ERR_INVALID_ARG;
+        // one scene seen from 8 views: materials and EM depend on the seed only, the variant
+        // picks the camera
+        r = Rand{seed * 0x2545F4914F6CDD1Dull + 0x1234567ull};
         UnitBox(b, false);   // floor only: the 8 views circle the spheres
         const int a = (int)b.mats.size();
         const float xs[4] = {0.3f, 0.7f, 0.3f, 0.7f}, ys[4] = {0.3f, 0.3f, 0.7f, 0.7f};

@@ -82,20 +82,25 @@ class DiffScene:
         return out + (float(ms.value),) if timed else out
 
 
-def render_function(diff_scene, n_iter):
+def render_function(diff_scene, n_iter, decorrelate=False):
     """A torch.autograd.Function for `diff_scene`: apply(scalars, em) -> image (H, W, 3) float64.
 
     Each call renders the next `n_iter` iterations (fresh samples every call); the parameters of
-    the call become the sampling parameters, and backward replays exactly those samples."""
+    the call become the sampling parameters.  Backward replays exactly those samples, or, with
+    `decorrelate`, the next `n_iter` iterations at the same parameters: the loss gradient then
+    pairs the residual of one set of samples with the image derivative of an independent set, so
+    for a loss like L2 (image - target)^2 its expectation is the gradient of the loss of the
+    expected image, without the covariance of image and derivative that shared samples add."""
     import torch
 
     state = {"next": 0}
+    span = 2 * n_iter if decorrelate else n_iter
 
     class RenderFn(torch.autograd.Function):
         @staticmethod
         def forward(ctx, scalars, em):
             first = state["next"]
-            state["next"] += n_iter
+            state["next"] += span
             diff_scene.set_params(scalars.detach().cpu().numpy(),
                                   None if em is None else em.detach().cpu().numpy(), refresh_sampling=True)
             img = diff_scene.render(first, n_iter)
@@ -106,7 +111,7 @@ def render_function(diff_scene, n_iter):
 
         @staticmethod
         def backward(ctx, grad_img):
-            gs, ge = diff_scene.adjoint(ctx.first, n_iter, grad_img.detach().cpu().numpy())
+            gs, ge = diff_scene.adjoint(ctx.first + span - n_iter, n_iter, grad_img.detach().cpu().numpy())
             g_em = torch.from_numpy(ge).to(ctx.dtypes[1]) if (ctx.has_em and ge is not None) else None
             return torch.from_numpy(gs).to(ctx.dtypes[0]), g_em
 

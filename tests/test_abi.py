@@ -48,6 +48,29 @@ def test_workload_scene_builders(workload, variant):
     assert sb.spp in (4, 16, 32, 64, 256)
 
 
+def test_workload5_views_share_one_scene():
+    """The 8 views of the inverse-rendering workload see one scene: the sphere materials and the
+    EM depend on the seed only; the cameras differ."""
+    def content(v):
+        sb = SceneBuild.workload(5, seed=2024, variant=v, res=32)
+        d = sb.desc
+        mats = [(m.type, tuple(m.glossy), m.alpha, tuple(m.diffuse)) for m in (d.materials[i] for i in range(d.num_materials))]
+        sph = np.ctypeslib.as_array(d.spheres, shape=(d.num_spheres, 4)).copy()
+        return mats, sb.em_texels().copy(), sph, tuple(d.camera.position)
+
+    m0, e0, s0, c0 = content(0)
+    alphas = [m[2] for m in m0 if m[0] == _ffi.MAT_GGX_CONDUCTOR]
+    assert len(alphas) == 4 and len(set(alphas)) == 4
+    for v in range(1, 8):
+        m, e, s, c = content(v)
+        assert m == m0 and np.array_equal(e, e0) and np.array_equal(s, s0)
+        assert c != c0
+    # another seed is another scene
+    a = SceneBuild.workload(5, seed=2024, variant=0, res=32).em_texels().copy()
+    b = SceneBuild.workload(5, seed=2025, variant=0, res=32).em_texels().copy()
+    assert not np.array_equal(a, b)
+
+
 def test_workload1_light_is_radiance_25():
     sb = SceneBuild.workload(1)
     d = sb.desc

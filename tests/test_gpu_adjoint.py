@@ -114,3 +114,26 @@ def test_autograd_function_and_adjoint_cost(setup):
     _, fwd_ms = ds.render(0, SPP, timed=True)
     *_, adj_ms = ds.adjoint(0, SPP, dl, timed=True)
     print(f"forward {fwd_ms:.3f} ms, adjoint {adj_ms:.3f} ms, ratio {adj_ms / fwd_ms:.2f}")
+
+
+def test_decorrelated_autograd_function(setup):
+    """With decorrelate=True the image of call k is iterations [2k SPP, (2k+1) SPP) and its
+    gradient is the adjoint of the next SPP iterations, weighted by that image's residual."""
+    import torch
+    _, _, ds, theta0, em0, target = setup
+    ds.set_params(theta0, em0, refresh_sampling=True)
+    fn = render_function(ds, SPP, decorrelate=True)
+    for call in range(2):
+        th = torch.tensor(theta0, requires_grad=True)
+        em = torch.tensor(em0, requires_grad=True)
+        img = fn.apply(th, em)
+        np.testing.assert_array_equal(img.detach().numpy(), ds.render(2 * call * SPP, SPP))
+        l = torch.mean((img - torch.from_numpy(target)) ** 2)
+        l.backward()
+        dl = 2.0 * (img.detach().numpy() - target) / img.numel()
+        gs, ge = ds.adjoint((2 * call + 1) * SPP, SPP, dl)
+        np.testing.assert_allclose(th.grad.numpy(), gs, rtol=1e-4, atol=1e-9 * np.abs(gs).max())
+        np.testing.assert_allclose(em.grad.numpy(), ge, rtol=1e-4, atol=1e-6 * np.abs(ge).max())
+        # an independent block: not the gradient of the image's own samples
+        gs_own, _ = ds.adjoint(2 * call * SPP, SPP, dl)
+        assert not np.allclose(gs_own, gs, rtol=1e-3)

@@ -241,6 +241,27 @@ def test_wavefront_equals_megakernel(ctx, workload, sampler):
     assert abs(a.sum() - b.sum()) / a.sum() < 1e-6
 
 
+@pytest.mark.parametrize("res", [37, 9])
+def test_wavefront_ragged_resolution(ctx, res):
+    """Resolutions that leave 8x8 tiles partly outside the image: padding samples are never
+    enqueued and every pixel is accumulated once per sample."""
+    sb = SceneBuild.workload(2, seed=27, res=res)
+    out = []
+    for k in (_ffi.KERNEL_MEGAKERNEL, _ffi.KERNEL_WAVEFRONT):
+        ctx.set_kernel_mode(k)
+        ctx.upload(sb)
+        ctx.synchronize()
+        ctx.reset_counters()
+        ctx.render(0, 6)
+        out.append(ctx.framebuffer(0, sb.resolution)[0])
+        assert ctx.counters()[1] == res * res * 6
+    a, b = out
+    assert a.shape[:2] == (res, res)
+    rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-6)
+    assert np.mean(rel > 1e-6) < 1e-3
+    assert abs(a.sum() - b.sum()) / a.sum() < 1e-6
+
+
 def test_wavefront_batch_larger_than_queue_capacity(ctx):
     """A render of more samples than one wavefront batch holds (2^21): two batches, partial tail."""
     ctx.set_kernel_mode(_ffi.KERNEL_WAVEFRONT)
