@@ -420,3 +420,48 @@ def test_splitting_with_iic0_sb1_equals_chain_kernel(ctx):
     rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-6)
     assert np.mean(rel > 1e-5) < 0.02
     assert abs(a.sum() - b.sum()) / a.sum() < 1e-4
+
+
+@pytest.mark.parametrize("kernel,params", [
+    (_ffi.KERNEL_MEGAKERNEL, {}), (_ffi.KERNEL_WAVEFRONT, {}),
+    (_ffi.KERNEL_MEGAKERNEL, dict(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0)),
+    (_ffi.KERNEL_MEGAKERNEL, dict(algorithm="ptn"))], ids=["chain", "wavefront", "split_rr", "ptn"])
+def test_single_pixel_image(ctx, kernel, params):
+    """A 1x1 image (one partly filled 8x8 tile): every kernel organisation against the oracle,
+    over enough iterations that the pixel's sum is a stable comparison."""
+    ctx.set_kernel_mode(kernel)
+    sb = SceneBuild.workload(1, seed=31, res=1)
+    sb.set_params(**params)
+    g, o = render_pair(ctx, sb, 256)
+    assert g.shape == (1, 1, 3) and np.isfinite(g).all() and g.sum() > 0
+    assert abs(g.sum() - o.sum()) / o.sum() < 1e-3
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+
+
+def test_empty_iteration_range_is_a_no_op(ctx):
+    for k in (_ffi.KERNEL_MEGAKERNEL, _ffi.KERNEL_WAVEFRONT):
+        ctx.set_kernel_mode(k)
+        sb = SceneBuild.workload(1, seed=32, res=16)
+        ctx.upload(sb)
+        ctx.render(0, 0)
+        g, it = ctx.framebuffer(0, sb.resolution)
+        assert it == 0 and not g.any()
+        ctx.render(0, 2)
+        a, _ = ctx.framebuffer(0, sb.resolution)
+        ctx.render(2, 0)
+        b, it = ctx.framebuffer(0, sb.resolution)
+        assert it == 2 and np.array_equal(a, b)
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+
+
+@pytest.mark.parametrize("min_pl,max_pl", [(1, 1), (2, 2), (2, 8), (3, 5)])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_path_length_bounds(ctx, kernel, min_pl, max_pl):
+    """-minpl / -maxpl (path_tracer.hxx:198-210, 252-265): direct light only, one bounce only,
+    emission from the camera ray skipped."""
+    ctx.set_kernel_mode(kernel)
+    sb = SceneBuild.workload(2, seed=33, res=24)
+    sb.set_params(min_path_length=min_pl, max_path_length=max_pl)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
