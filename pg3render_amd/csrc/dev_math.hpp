@@ -42,11 +42,16 @@ PG3_HD F3 Cross(F3 a, F3 b) { return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x *
 PG3_HD float LenSqr(F3 a) { return Dot(a, a); }
 PG3_HD float Length(F3 a) { return sqrtf(Dot(a, a)); }
 PG3_HD F3 Normalize(F3 a) { return a / sqrtf(Dot(a, a)); }            // math.hxx:653-658
-// Division by the hardware reciprocal (<= 2 ulp; `__fdividef`).  Used where a one-ulp change only
-// moves a sample within its pdf (the steerable sampler's arithmetic, path throughput updates);
-// BSDF values and pdfs keep correctly rounded division, which their 1e-5 parity needs.
+// Division by the hardware reciprocal: a * v_rcp_f32(b), within 2 ulp of the correctly rounded
+// quotient (1 ulp reciprocal, then a rounded product); one transcendental and one multiply in
+// place of the ~10-instruction correctly rounded sequence.  (`__fdividef` would not do: this HIP
+// defines it as plain `/`.)  Used where a one-ulp change only moves a value by ~1e-7 relative:
+// pdf ratios, direction normalisation, MIS weights, path throughput and contributions.  BSDF
+// values and pdfs, and the sampling warps whose output a one-ulp change would move by more (the
+// steerable tree descent rescales its random number at every level), keep correctly rounded
+// division.
 #if defined(__HIP_DEVICE_COMPILE__)
-PG3_HD float FastDiv(float a, float b) { return __fdividef(a, b); }
+PG3_HD float FastDiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 #else
 PG3_HD float FastDiv(float a, float b) { return a / b; }
 #endif

@@ -467,7 +467,7 @@ PG3_DEV uint32_t PickChild(P w, uint32_t cc, const SH9 &c, float &u) {
 #pragma unroll
     for (uint32_t i = 1; i <= 6; ++i) if (i == cc) whole = cs[i];
     if (whole > 1E-10f) {
-        const float inv = FastDiv(1.f, whole);
+        const float inv = 1.f / whole;
 #pragma unroll
         for (uint32_t i = 1; i <= 6; ++i) cs[i] *= inv;
     }
@@ -483,7 +483,7 @@ PG3_DEV uint32_t PickChild(P w, uint32_t cc, const SH9 &c, float &u) {
         if (i == ub - 1u) low = cs[i];
         if (i == ub) high = cs[i];
     }
-    const float nu = FastDiv(ut - low, fmaxf(high - low, 0.00001f));
+    const float nu = (ut - low) / fmaxf(high - low, 0.00001f);
     u = fminf(nu, 1.f);
     return ub - 1u;
 }
@@ -505,9 +505,9 @@ PG3_DEV uint32_t PickLeaf(const DevEm &E, const SH9 &c, float &u, const TreeCach
 }
 // SampleTriangleFFunction / GFunction / Bilinear, :3709-3747, :3882-3905 (math.hxx:284-342)
 PG3_DEV void SampleTriBilinear(float ux, float uy, float va, float vb, float vc, float &b0, float &b1) {
-    const float x = FastDiv(vb - va, 3.f) + FastDiv(vc - vb, 6.f);
+    const float x = (vb - va) / 3.f + (vc - vb) / 6.f;
     const float y = va * 0.5f;
-    const float a = FastDiv(x, x + y), b = FastDiv(y, x + y);
+    const float a = x / (x + y), b = y / (x + y);
     float s = 0.5f;
     bool bail = false;
 #pragma unroll
@@ -516,14 +516,14 @@ PG3_DEV void SampleTriBilinear(float ux, float uy, float va, float vb, float vc,
         const float d = s * (s * 3.f * a + 2.f * b) + 0.f;
         if (!bail) {
             if (fabsf(d) < 0.0001f) { bail = true; }
-            else s = Clampf(s - FastDiv(f, d), 0.f, 1.f);
+            else s = Clampf(s - f / d, 0.f, 1.f);
         }
     }
     if (bail) s = 0.5f;
     const float t0 = (s * (vc - vb)) + (2 * (1 - s) * va) + (s * vb);
-    const float gamma = FastDiv(s * (vc - vb), t0);
-    const float rho = FastDiv(2 * ((1 - s) * va + s * vb), t0);
-    const float t = FastDiv(2 * uy, rho + SafeSqrt(rho * rho + 4 * gamma * uy));
+    const float gamma = (s * (vc - vb)) / t0;
+    const float rho = (2 * ((1 - s) * va + s * vb)) / t0;
+    const float t = (2 * uy) / (rho + SafeSqrt(rho * rho + 4 * gamma * uy));
     b0 = 1 - s;
     b1 = s * (1 - t);
 }

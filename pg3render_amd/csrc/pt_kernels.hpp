@@ -25,7 +25,7 @@ __device__ __forceinline__ uint32_t WaveSum(uint32_t v) {
 // MISWeight2Power, path_tracer_base.hxx:522-541 (sample counts 1)
 PG3_DEV float MisPower(float p1, float p2) {
     const float q1 = p1 * p1, q2 = p2 * p2;
-    return q1 / (q1 + q2) / 1u;
+    return FastDiv(q1, q1 + q2);
 }
 
 struct LightCache {            // LightSamplingContext, path_tracer_base.hxx:22-32
@@ -174,9 +174,9 @@ PG3_DEV void LightSampleRequest(const PathState &ps, const DevMaterial &mat, con
         if (ls.pdfW != kInf) {
             const float lightPdf = ls.pdfW * ls.prob;
             const float bsdfPdf = r.pdfW * r.compProb;
-            sr.contrib = ps.beta * ((ls.sample * r.att * MisPower(lightPdf, bsdfPdf)) / lightPdf);
+            sr.contrib = ps.beta * FastDiv(ls.sample * r.att * MisPower(lightPdf, bsdfPdf), lightPdf);
         } else {
-            sr.contrib = ps.beta * ((ls.sample * r.att) / (ls.prob * 1u));
+            sr.contrib = ps.beta * FastDiv(ls.sample * r.att, ls.prob);
         }
         sr.p = sp;
         sr.dir = ls.wig;
@@ -424,9 +424,9 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey
                 if (ls.pdfW != kInf) {
                     const float lightPdf = ls.pdfW * ls.prob;
                     const float bsdfPdf = r.pdfW * r.compProb;
-                    sr.contrib = ps.beta * ((ls.sample * r.att * MisPower(lightPdf, bsdfPdf)) / lightPdf);
+                    sr.contrib = ps.beta * FastDiv(ls.sample * r.att * MisPower(lightPdf, bsdfPdf), lightPdf);
                 } else {
-                    sr.contrib = ps.beta * ((ls.sample * r.att) / (ls.prob * 1u));
+                    sr.contrib = ps.beta * FastDiv(ls.sample * r.att, ls.prob);
                 }
                 sr.p = sp;
                 sr.dir = ls.wig;
