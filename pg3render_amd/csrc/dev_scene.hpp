@@ -665,15 +665,20 @@ PG3_DEV float SideCos(uint32_t sides, float c) {                                
     if (sides & kSideBack) return fmaxf(-c, 0.0f);
     return 0.0f;
 }
+// Direction and distance to a light point by the hardware reciprocal square root (v_rsq_f32,
+// 1 ulp): light sampling and the light-picking estimates (four of these per area light per
+// vertex) need no correctly rounded sqrt and divisions; a one-ulp change moves the shadow ray's
+// direction and length by ~1e-7 relative.
 PG3_DEV void AreaComputeSample(const DevLight &L, F3 sp, F3 pt, F3 n, uint32_t sides, LightSample &o) {  // :171-219
     o.wig = pt - sp;
     const float d2 = LenSqr(o.wig);
-    o.dist = sqrtf(d2);
-    o.wig = o.wig / o.dist;
+    const float rinv = __builtin_amdgcn_rsqf(d2);
+    o.dist = d2 * rinv;
+    o.wig = o.wig * rinv;
     const float cosOut = -Dot(L.n, o.wig);
     const float cosIn = SideCos(sides, Dot(n, o.wig));
     o.sample = (cosOut > 0.f) ? L.radiance * cosIn : mk(0.f);
-    o.pdfW = fmaxf(L.invArea * (d2 / fabsf(cosOut)), kEpsDist);
+    o.pdfW = fmaxf(L.invArea * FastDiv(d2, fabsf(cosOut)), kEpsDist);
     o.prob = 1.0f;
 }
 PG3_DEV void PointCompute(const DevLight &L, F3 sp, F3 n, uint32_t sides, LightSample &o) {  // :296-331
@@ -712,8 +717,8 @@ PG3_DEV float BgPdf(const SceneView &S, const LightSpec &ls, F3 dir, const Frame
 PG3_DEV float AreaEmissionPdf(const DevLight &L, F3 lightPt, F3 surfPt) {
     F3 wig = lightPt - surfPt;
     const float d2 = LenSqr(wig);
-    wig = wig / sqrtf(d2);
-    return fmaxf(L.invArea * (d2 / fabsf(Dot(L.n, wig))), kEpsDist);
+    wig = wig * __builtin_amdgcn_rsqf(d2);
+    return fmaxf(L.invArea * FastDiv(d2, fabsf(Dot(L.n, wig))), kEpsDist);
 }
 
 // EnvironmentMap::EstimateIrradiance MC fallback (em.hxx:123-170), simple + cosine samplers
@@ -748,8 +753,8 @@ PG3_DEV float EstimateContribution(const SceneView &S, const LightSpec &spec, ui
         AreaComputeSample(L, sp, P1, fr.z, sides, b);
         AreaComputeSample(L, sp, P2, fr.z, sides, c);
         AreaComputeSample(L, sp, P3, fr.z, sides, d);
-        return (Luminance(a.sample) / a.pdfW + Luminance(b.sample) / b.pdfW + Luminance(c.sample) / c.pdfW +
-                Luminance(d.sample) / d.pdfW) / 4.f;
+        return (FastDiv(Luminance(a.sample), a.pdfW) + FastDiv(Luminance(b.sample), b.pdfW) +
+                FastDiv(Luminance(c.sample), c.pdfW) + FastDiv(Luminance(d.sample), d.pdfW)) * 0.25f;
     }
     if (L.type == kLightPoint) {                                                       // :277-290
         LightSample s;
