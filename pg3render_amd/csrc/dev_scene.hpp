@@ -104,10 +104,30 @@ PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, F3 m, float tmin, Hit &h) {
     const float v2d = Dot(dir, mk(t.v[2][0], t.v[2][1], t.v[2][2])) - Dot(m, mk(t.u[2][0], t.u[2][1], t.u[2][2]));
     if (((v0d < 0.f) && (v1d < 0.f) && (v2d < 0.f)) || ((v0d >= 0.f) && (v1d >= 0.f) && (v2d >= 0.f))) {
         const F3 n = mk(t.n[0], t.n[1], t.n[2]);
-        const float d = __fdividef(t.np0 - Dot(n, org), Dot(n, dir));
+        const float d = FastDiv(t.np0 - Dot(n, org), Dot(n, dir));
         if ((d > tmin) & (d < h.dist)) { h.dist = d; h.mat = t.mat; h.n = n; return true; }
     }
     return false;
+}
+// FP64 quotient and square root from the hardware estimates (v_rcp_f64, v_rsq_f64), each refined
+// by two Newton steps and one residual correction: within a few ulp of the correctly rounded FP64
+// result, so a root rounded to float agrees with the correctly rounded one unless it lies within
+// ~1e-15 relative of a float rounding boundary.  About half the FP64 instructions of the
+// correctly rounded sequences (scale, two Newton steps, fused correction, fix-up).
+PG3_DEV double DivD(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    const double q = a * r;
+    return fma(r, fma(-b, q, a), q);
+}
+PG3_DEV double SqrtD(double x) {   // x >= 0
+    if (!(x > 0.0)) return 0.0;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    const double s = x * y;
+    return fma(0.5 * y, fma(-s, s, x), s);
 }
 // :176-245.  B*B - 4*A*C is evaluated in FP32 (all float operands) and widened; the root, q and
 // both roots are FP64 as in the reference.
@@ -119,10 +139,10 @@ PG3_DEV bool SphHit(const DevSphere &s, F3 org, F3 dir, float tmin, Hit &h) {
     const float C = Dot(o, o) - (s.r * s.r);
     const float discF = B * B - 4 * A * C;
     if (discF < 0.f) return false;
-    const double ds = sqrt((double)discF);
-    const double q = (B >= 0) ? ((-(double)B - ds) / 2.0) : ((-(double)B + ds) / 2.0);
-    double t0 = q / (double)A;
-    double t1 = (fabs(q) > 0.1) ? ((double)C / q) : t0;
+    const double ds = SqrtD((double)discF);
+    const double q = (B >= 0) ? ((-(double)B - ds) * 0.5) : ((-(double)B + ds) * 0.5);
+    double t0 = DivD(q, (double)A);
+    double t1 = (fabs(q) > 0.1) ? DivD((double)C, q) : t0;
     if (t0 > t1) { const double t = t0; t0 = t1; t1 = t; }
     float resT;
     if (t0 > tmin && t0 < h.dist) resT = (float)t0;
