@@ -638,7 +638,7 @@ constexpr uint32_t kTreeDepth = 48;
 PG3_DEV float MisPowerN(float p1, uint32_t n1, float p2, uint32_t n2) {
     const float s1 = n1 * p1, s2 = n2 * p2;
     const float q1 = s1 * s1, q2 = s2 * s2;
-    return q1 / (q1 + q2) / n1;
+    return FastDiv(q1, (q1 + q2) * n1);
 }
 // Vec3Base::ClipProportionally, math.hxx:538-549
 PG3_DEV F3 ClipProportionally(F3 v, float maxAllowed) {
@@ -746,9 +746,9 @@ PG3_DEV F3 RadiancePTTree(const SceneView &S, const LightSpec &spec, const RngKe
                                     if (ls.pdfW != kInf) {
                                         const float bsdfPdf = r.pdfW * r.compProb;
                                         const float lightPdf = ls.pdfW * ls.prob;
-                                        F.refl += (ls.sample * r.att * MisPowerN(lightPdf, nL, bsdfPdf, nB)) / lightPdf;
+                                        F.refl += FastDiv(ls.sample * r.att * MisPowerN(lightPdf, nL, bsdfPdf, nB), lightPdf);
                                     } else {
-                                        F.refl += (ls.sample * r.att) / (ls.prob * nL);
+                                        F.refl += FastDiv(ls.sample * r.att, ls.prob * nL);
                                     }
                                 }
                             }
@@ -823,15 +823,15 @@ PG3_DEV F3 RadiancePTTree(const SceneView &S, const LightSpec &spec, const RngKe
                     }
                     const float lp = rPdf * pick;
                     const float bp = P.pdfW * P.comp;
-                    P.refl += (P.att * P.cosAbs * rEm * MisPowerN(bp, P.nB, lp, P.nL)) / bp;
+                    P.refl += FastDiv(P.att * P.cosAbs * rEm * MisPowerN(bp, P.nB, lp, P.nL), bp);
                 } else {
-                    P.refl += (P.att * rEm) / (static_cast<float>(P.nB) * P.comp);
+                    P.refl += FastDiv(P.att * rEm, static_cast<float>(P.nB) * P.comp);
                 }
             }
             if (!cut && !IsZero(rRefl)) {
                 F3 ind;
-                if (finite) ind = (P.att * P.cosAbs * rRefl) / (P.pdfW * P.nB * P.rr * P.comp);
-                else ind = (P.att * rRefl) / (P.nB * P.rr * P.comp);
+                if (finite) ind = FastDiv(P.att * P.cosAbs * rRefl, P.pdfW * P.nB * P.rr * P.comp);
+                else ind = FastDiv(P.att * rRefl, P.nB * P.rr * P.comp);
                 if (S.iic > 0.f) ind = ClipProportionally(ind, S.iic);
                 P.refl += ind;
             }
@@ -1117,9 +1117,9 @@ struct SplitPath {
                     if (ls.pdfW != kInf) {
                         const float bsdfPdf = r.pdfW * r.compProb;
                         const float lightPdf = ls.pdfW * ls.prob;
-                        acc += (ls.sample * r.att * MisPowerN(lightPdf, nLv, bsdfPdf, nBv)) / lightPdf;
+                        acc += FastDiv(ls.sample * r.att * MisPowerN(lightPdf, nLv, bsdfPdf, nBv), lightPdf);
                     } else {
-                        acc += (ls.sample * r.att) / (ls.prob * nLv);
+                        acc += FastDiv(ls.sample * r.att, ls.prob * nLv);
                     }
                 }
             }
@@ -1163,7 +1163,7 @@ struct SplitPath {
     // MIS-weighted emission delivered to the parent (path_tracer.hxx:327-369)
     PG3_DEV F3 ParentEmission(const SceneView &S, const LightSpec &spec, F3 em, float lpdf, int32_t lid) const {
         if (IsZero(em) || lid < 0) return mk(0.f);
-        if (!pFinite) return (pAttCos * em) / (static_cast<float>(pnB) * pComp);
+        if (!pFinite) return FastDiv(pAttCos * em, static_cast<float>(pnB) * pComp);
         float pick = 1.f;
         if (spec.numLights == 0) {
             pick = 0.f;
@@ -1174,10 +1174,10 @@ struct SplitPath {
         }
         const float lp = lpdf * pick;
         const float bp = pPdfW * pComp;
-        return (pAttCos * em * MisPowerN(bp, pnB, lp, pnL)) / bp;
+        return FastDiv(pAttCos * em * MisPowerN(bp, pnB, lp, pnL), bp);
     }
     PG3_DEV static F3 Indirect(F3 attCos, F3 R, float den, float iic) {
-        F3 ind = (attCos * R) / den;
+        F3 ind = FastDiv(attCos * R, den);
         if (iic > 0.f) ind = ClipProportionally(ind, iic);
         return ind;
     }
