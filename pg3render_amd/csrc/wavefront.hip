@@ -186,8 +186,14 @@ __global__ void __launch_bounds__(kWfBlock) WfIntersect(SceneView S, WfQueues Q,
 // Unlike the fused kernel it does not stage the steerable tree in LDS: that needs one 1024-lane
 // workgroup per CU, whose queue-append barriers then hold all 16 wavefronts of the CU in step,
 // and measured no faster.  The simple sampler's marginal CDF is staged.
+// Register budget of the shading stage in waves per SIMD (its grid is the resident workgroups).
+// C2 frame: 2, 3, 4, 5, 6, 8 waves: 22.5, 21.3, 20.9, 24.9, 29.9, 40.7 ms (fewer waves hide less
+// load latency; more spill).
+#ifndef PG3_WF_SHADE_WAVES
+#define PG3_WF_SHADE_WAVES 4
+#endif
 template <int kEmKind, bool kOneLight>
-__global__ void __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(PG3_WF_SHADE_WAVES)))
 WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, uint32_t bounce, uint64_t batchBase,
         uint32_t firstIter) {
     const SceneView &S = *Sp;
@@ -384,7 +390,7 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         W->gridBlocks = (uint32_t)cus * 8u;
-        W->shadeBlocks = (uint32_t)cus * 16u / (kWfBlock / 64u);
+        W->shadeBlocks = (uint32_t)cus * 4u * PG3_WF_SHADE_WAVES / (kWfBlock / 64u);
     }
     const uint32_t tilesX = (S.resX + 7u) / 8u, tilesY = (S.resY + 7u) / 8u;
     const uint64_t slotsPerIter = (uint64_t)tilesX * tilesY * 64u;
