@@ -110,7 +110,8 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     uint32_t segments = 0, samples = 0;
     SceneItems<kOneLight && kEmKind == kEmSteerable> src{Sp,        spec,     work,     fb,      firstIter,
                                                          nIter,     itersPerThread, numTiles * numChunks, numTiles,
-                                                         tilesX};
+                                                         tilesX,    0u,       0u};
+    src.SetTail(numChunks);
     if constexpr (kSplit) TraceItems<SplitPath>(src, accLds[threadIdx.x >> 6], segments, samples);
     else TraceItems<ChainPath<kOneLight && kEmKind == kEmSteerable>>(src, accLds[threadIdx.x >> 6], segments, samples);
     const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);

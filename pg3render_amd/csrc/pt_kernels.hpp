@@ -1331,19 +1331,39 @@ struct SceneItems {
     uint32_t *work;
     double *fb;
     uint32_t firstIter, nIter, itersPerItem, numItems, numTiles, tilesX;
+    // items [0, mainItems) cover iterations [0, mainIters) in chunks of itersPerItem; the
+    // remaining iterations follow as single-iteration items, so the items handed out last are
+    // the shortest and the waves' finishing times even out at the end of the launch
+    uint32_t mainItems, mainIters;
     PG3_DEV bool Next(TileItem &t) {
         uint32_t item = 0;
         if (__lane_id() == 0) item = atomicAdd(work, 1u);
         item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
         if (item >= numItems) return false;   // every wave leaves once the counter passes the last item
-        const uint32_t tile = item % numTiles, chunk = item / numTiles;
+        uint32_t tile, n;
+        if (item < mainItems) {
+            tile = item % numTiles;
+            t.it = firstIter + (item / numTiles) * itersPerItem;
+            n = min(firstIter + nIter, t.it + itersPerItem) - t.it;
+        } else {
+            const uint32_t j = item - mainItems;
+            tile = j % numTiles;
+            t.it = firstIter + mainIters + j / numTiles;
+            n = 1u;
+        }
         t.scene = 0;
         t.x0 = (tile % tilesX) * 8u;
         t.y0 = (tile / tilesX) * 8u;
-        t.it = firstIter + chunk * itersPerItem;
-        t.total = 64u * (min(firstIter + nIter, t.it + itersPerItem) - t.it);
+        t.total = 64u * n;
         t.fb = fb;
         return true;
+    }
+    // Splits the last chunk (when items hold more than one iteration) into single iterations.
+    PG3_DEV void SetTail(uint32_t numChunks) {
+        const uint32_t mainChunks = (itersPerItem > 1u && numChunks > 1u) ? numChunks - 1u : numChunks;
+        mainIters = min(nIter, mainChunks * itersPerItem);
+        mainItems = numTiles * mainChunks;
+        numItems = mainItems + numTiles * (nIter - mainIters);
     }
     PG3_DEV const SceneView *View(uint32_t) const { return S; }
     PG3_DEV LightSpec Spec(uint32_t) const { return spec; }
