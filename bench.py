@@ -23,6 +23,9 @@ BYTES_PER_SAMPLE, BYTES_PER_SEGMENT = 112, 304
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 KiB, factors from a 512 MiB copy): profiles/render_pt_box_steerable_em_512px_64spp.md
 MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 8375358 + 6484874) * 1024)}
+# HBM bytes per frame of the wavefront pipeline on the default workload (all Wf* launches of a
+# frame, same calibration): profiles/wavefront_box_steerable_em_512px_64spp.md
+MEASURED_WF_TRAFFIC = int((2.0 * 17476832 + 18135104) * 1024)
 
 
 def parse():
@@ -83,7 +86,9 @@ def wavefront_measure(ctx, steps=2):
     t = sum(ms) / len(ms) / 1000.0
     qbytes = (BYTES_PER_SAMPLE * smp + BYTES_PER_SEGMENT * seg) / steps
     return {"ms_per_frame": round(t * 1000.0, 3), "msamples_per_s": round(smp / steps / t / 1e6, 1),
-            "queue_model_gbs": round(qbytes / t / 1e9, 1), "queue_model_frac": round(qbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+            "queue_model_gbs": round(qbytes / t / 1e9, 1), "queue_model_frac": round(qbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+            "hbm_measured_gbs": round(MEASURED_WF_TRAFFIC / t / 1e9, 1),
+            "hbm_measured_frac": round(MEASURED_WF_TRAFFIC / t / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def host_threads():
