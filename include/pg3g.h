@@ -209,6 +209,16 @@ int pg3g_em_sample_batch(pg3g_context *ctx, uint32_t n, const float *normals,
 int pg3g_em_pdf_batch(pg3g_context *ctx, uint32_t n, const float *dirs, const float *normals,
                       const int32_t *side_flags, float *pdf);
 int pg3g_em_eval_batch(pg3g_context *ctx, uint32_t n, const float *dirs, float *rgb);
+/* Scene::Intersect (src/scene.hxx:42-59; GeometryList / Triangle / Sphere::Intersect,
+ * src/scene_graph.hxx:33-245) of slot 0's scene: the closest hit beyond tmin of n rays
+ * (org, dir: n*3; tmin: n).  Outputs dist (n; -1 for a miss), the geometric normal (n*3) and the
+ * material index (n; -1 for a miss). */
+int pg3g_intersect_batch(pg3g_context *ctx, uint32_t n, const float *org, const float *dir,
+                         const float *tmin, float *dist, float *normal, int32_t *mat);
+/* Scene::Occluded (src/scene.hxx:61-74): whether anything lies on the segment from
+ * p + dir * 1e-3 to distance tmax - 2e-3 along dir (p, dir: n*3; tmax: n) -> occluded (n, 0/1). */
+int pg3g_occluded_batch(pg3g_context *ctx, uint32_t n, const float *p, const float *dir,
+                        const float *tmax, int32_t *occluded);
 
 /* Introspection of the steerable tables of slot 0 (for parity with the reference build). */
 int pg3g_steerable_info(pg3g_context *ctx, uint32_t *num_leaves, uint32_t *num_vertices,

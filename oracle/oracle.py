@@ -41,7 +41,8 @@ def load():
         "oracle_em_eval_uv": (None, [C.c_void_p, C.c_uint32, _FP, _FP]),
         "oracle_em_avg_pixel_lum": (C.c_float, [C.c_void_p, C.c_uint32, C.c_uint32]),
         "oracle_camera_ray": (None, [C.c_void_p, C.c_float, C.c_float, _FP, _FP]),
-        "oracle_intersect": (C.c_int, [C.c_void_p, _FP, _FP, C.c_float, _FP, _IP, _FP]),
+        "oracle_intersect_batch": (None, [C.c_void_p, C.c_uint32, _FP, _FP, _FP, _FP, _FP, _IP]),
+        "oracle_occluded_batch": (None, [C.c_void_p, C.c_uint32, _FP, _FP, _FP, _IP]),
         "oracle_sh_basis": (None, [_FP, _FP]),
         "oracle_clamped_cos": (None, [_FP, C.c_int, _FP]),
         "oracle_fresnel_dielectric": (C.c_float, [C.c_float, C.c_float]),
@@ -139,6 +140,25 @@ class OracleScene:
         o = np.zeros(3, np.float32); d = np.zeros(3, np.float32)
         self.lib.oracle_camera_ray(self.ptr, x, y, fp(o), fp(d))
         return o, d
+
+    def intersect(self, org, dir, tmin):
+        """Scene::Intersect (scene.hxx:42-59): (dist (-1: miss), normal (n, 3), material (-1))."""
+        org, dir = f32(org), f32(dir)
+        n = len(org)
+        tmin = f32(np.broadcast_to(np.asarray(tmin, np.float32), (n,)))
+        dist = np.zeros(n, np.float32); nrm = np.zeros((n, 3), np.float32); mat = np.zeros(n, np.int32)
+        self.lib.oracle_intersect_batch(self.ptr, n, fp(org), fp(dir), fp(tmin), fp(dist), fp(nrm),
+                                        mat.ctypes.data_as(_IP))
+        return dist, nrm, mat
+
+    def occluded(self, p, dir, tmax):
+        """Scene::Occluded (scene.hxx:61-74): bool per segment."""
+        p, dir = f32(p), f32(dir)
+        n = len(p)
+        tmax = f32(np.broadcast_to(np.asarray(tmax, np.float32), (n,)))
+        occ = np.zeros(n, np.int32)
+        self.lib.oracle_occluded_batch(self.ptr, n, fp(p), fp(dir), fp(tmax), occ.ctypes.data_as(_IP))
+        return occ != 0
 
     def steerable(self):
         nl, nv = C.c_uint32(), C.c_uint32()

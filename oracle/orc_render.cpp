@@ -775,12 +775,26 @@ void oracle_camera_ray(void *sp, float x, float y, float *org, float *dir) {
     V3 o, d; static_cast<Scene *>(sp)->GenerateRay(x, y, o, d);
     org[0] = o.x; org[1] = o.y; org[2] = o.z; dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
 }
-int oracle_intersect(void *sp, const float *org, const float *dir, float tmin, float *dist, int32_t *mat, float *normal) {
+// Scene::Intersect / Occluded of n rays (scene.hxx:42-74): dist -1 and material -1 for a miss.
+void oracle_intersect_batch(void *sp, uint32_t n, const float *org, const float *dir, const float *tmin, float *dist,
+                            float *normal, int32_t *mat) {
     const Scene &S = *static_cast<Scene *>(sp);
-    Isect r; r.dist = 1e36f;
-    const bool hit = S.Intersect(V3(org[0], org[1], org[2]), V3(dir[0], dir[1], dir[2]), tmin, r);
-    if (hit) { *dist = r.dist; *mat = r.matID; normal[0] = r.normal.x; normal[1] = r.normal.y; normal[2] = r.normal.z; }
-    return hit ? 1 : 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        Isect r; r.dist = 1e36f;
+        const bool hit = S.Intersect(V3(org[3 * i], org[3 * i + 1], org[3 * i + 2]),
+                                     V3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]), tmin[i], r);
+        dist[i] = hit ? r.dist : -1.f;
+        mat[i] = hit ? r.matID : -1;
+        normal[3 * i] = hit ? r.normal.x : 0.f;
+        normal[3 * i + 1] = hit ? r.normal.y : 0.f;
+        normal[3 * i + 2] = hit ? r.normal.z : 0.f;
+    }
+}
+void oracle_occluded_batch(void *sp, uint32_t n, const float *p, const float *dir, const float *tmax, int32_t *occ) {
+    const Scene &S = *static_cast<Scene *>(sp);
+    for (uint32_t i = 0; i < n; ++i)
+        occ[i] = S.Occluded(V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), V3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
+                            tmax[i]) ? 1 : 0;
 }
 
 // ---- primitives for known-answer tests ----------------------------------------------------------

@@ -114,6 +114,8 @@ _SIGNATURES = {
     "pg3g_em_sample_batch": (C.c_int, [_P, C.c_uint32, _FP, _IP, _FP, _FP, _FP, _FP, _IP]),
     "pg3g_em_pdf_batch": (C.c_int, [_P, C.c_uint32, _FP, _FP, _IP, _FP]),
     "pg3g_em_eval_batch": (C.c_int, [_P, C.c_uint32, _FP, _FP]),
+    "pg3g_intersect_batch": (C.c_int, [_P, C.c_uint32, _FP, _FP, _FP, _FP, _FP, _IP]),
+    "pg3g_occluded_batch": (C.c_int, [_P, C.c_uint32, _FP, _FP, _FP, _IP]),
     "pg3g_steerable_info": (C.c_int, [_P, _U32P, _U32P, _U32P]),
     "pg3g_steerable_leaves": (C.c_int, [_P, _FP]),
     "pg3g_steerable_root_weight": (C.c_int, [_P, _FP]),

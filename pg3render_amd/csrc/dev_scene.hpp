@@ -94,8 +94,9 @@ struct Hit {
 };
 
 // The intersection routines are built with FMA contraction and a fast reciprocal division (the
-// rest of the device code is not): they are the bulk of the arithmetic, and their results enter the
-// parity checks only through whole images, where one-ulp hit distances are invisible.
+// rest of the device code is not): they are the bulk of the arithmetic, and a few-ulp hit distance
+// is invisible in images; the function-level checks (pg3g_intersect_batch / pg3g_occluded_batch
+// against the oracle) hold hit/miss and material exactly and distances to 1e-5 relative.
 // m = org x dir, the ray's moment (computed once per ray).
 PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, F3 m, float tmin, Hit &h) {  // :103-134
 #pragma clang fp contract(fast)
@@ -115,14 +116,15 @@ PG3_DEV bool TriHit(const DevTri &t, F3 org, F3 dir, F3 m, float tmin, Hit &h) {
 // ~1e-15 relative of a float rounding boundary.  About half the FP64 instructions of the
 // correctly rounded sequences (scale, two Newton steps, fused correction, fix-up).
 PG3_DEV double DivD(double a, double b) {
+    if (!(fabs(b) > 0.0 && fabs(b) < 1e300)) return a / b;   // 0, inf, NaN, huge: exact
     double r = __builtin_amdgcn_rcp(b);
     r = fma(r, fma(-b, r, 1.0), r);
     r = fma(r, fma(-b, r, 1.0), r);
     const double q = a * r;
     return fma(r, fma(-b, q, a), q);
 }
-PG3_DEV double SqrtD(double x) {   // x >= 0
-    if (!(x > 0.0)) return 0.0;
+PG3_DEV double SqrtD(double x) {
+    if (!(x > 0.0 && x < 1e300)) return sqrt(x);   // 0, negative, inf, NaN: exact
     double y = __builtin_amdgcn_rsq(x);
     y = y * fma(-0.5 * x * y, y, 1.5);
     y = y * fma(-0.5 * x * y, y, 1.5);

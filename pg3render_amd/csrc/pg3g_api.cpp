@@ -909,6 +909,53 @@ int pg3g_em_eval_batch(pg3g_context *ctx, uint32_t n, const float *dirs, float *
     return PG3G_OK;
 }
 
+int pg3g_intersect_batch(pg3g_context *ctx, uint32_t n, const float *org, const float *dir, const float *tmin,
+                         float *dist, float *normal, int32_t *mat) {
+    SceneSlot *S = GetSlot(ctx, 0);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot 0");
+    if (n && (!org || !dir || !tmin || !dist || !normal || !mat)) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return PG3G_OK;
+    PG3_HIP(hipSetDevice(ctx->device));
+    Staging sg{ctx};
+    hipError_t e[6];
+    float *dO = (float *)sg.Get(12ull * n, e[0]), *dD = (float *)sg.Get(12ull * n, e[1]);
+    float *dT = (float *)sg.Get(4ull * n, e[2]), *dR = (float *)sg.Get(4ull * n, e[3]);
+    float *dN = (float *)sg.Get(12ull * n, e[4]);
+    int32_t *dM = (int32_t *)sg.Get(4ull * n, e[5]);
+    for (hipError_t x : e) PG3_HIP(x);
+    PG3_HIP(hipMemcpyAsync(dO, org, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dD, dir, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dT, tmin, 4ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(LaunchIntersect(S->view, n, dO, dD, dT, dR, dN, dM, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dist, dR, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(normal, dN, 12ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(mat, dM, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    return PG3G_OK;
+}
+
+int pg3g_occluded_batch(pg3g_context *ctx, uint32_t n, const float *p, const float *dir, const float *tmax,
+                        int32_t *occluded) {
+    SceneSlot *S = GetSlot(ctx, 0);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot 0");
+    if (n && (!p || !dir || !tmax || !occluded)) return Fail(PG3G_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return PG3G_OK;
+    PG3_HIP(hipSetDevice(ctx->device));
+    Staging sg{ctx};
+    hipError_t e[4];
+    float *dP = (float *)sg.Get(12ull * n, e[0]), *dD = (float *)sg.Get(12ull * n, e[1]);
+    float *dT = (float *)sg.Get(4ull * n, e[2]);
+    int32_t *dO = (int32_t *)sg.Get(4ull * n, e[3]);
+    for (hipError_t x : e) PG3_HIP(x);
+    PG3_HIP(hipMemcpyAsync(dP, p, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dD, dir, 12ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(dT, tmax, 4ull * n, hipMemcpyHostToDevice, ctx->stream));
+    PG3_HIP(LaunchOccluded(S->view, n, dP, dD, dT, dO, ctx->stream));
+    PG3_HIP(hipMemcpyAsync(occluded, dO, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    return PG3G_OK;
+}
+
 int pg3g_steerable_info(pg3g_context *ctx, uint32_t *num_leaves, uint32_t *num_vertices, uint32_t *num_levels) {
     SceneSlot *S = GetSlot(ctx, 0);
     if (!S || !S->hasSteer) return Fail(PG3G_ERR_NO_SCENE, "slot 0 has no steerable sampler");

@@ -67,6 +67,10 @@ hipError_t LaunchEmSample(const SceneView &S, uint32_t n, const float *nrm, cons
 hipError_t LaunchEmPdf(const SceneView &S, uint32_t n, const float *dirs, const float *nrm, const int32_t *sides,
                        float *pdf, hipStream_t s);
 hipError_t LaunchEmEval(const SceneView &S, uint32_t n, const float *dirs, float *rgb, hipStream_t s);
+hipError_t LaunchIntersect(const SceneView &S, uint32_t n, const float *org, const float *dir, const float *tmin,
+                           float *dist, float *nrm, int32_t *mat, hipStream_t s);
+hipError_t LaunchOccluded(const SceneView &S, uint32_t n, const float *p, const float *dir, const float *tmax,
+                          int32_t *occ, hipStream_t s);
 // Texel luminances of an EM (the steerable pdf's luminance map).
 hipError_t LaunchEmLum(const float4 *tex, float *lum, uint32_t n, hipStream_t s);
 hipError_t LaunchCdfBuild(const DevEm &E, float *condCdf, float *margCdf, float *rowIntegral, hipStream_t s);

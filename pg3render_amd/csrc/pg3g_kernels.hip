@@ -209,6 +209,31 @@ __global__ void EmPdfKernel(SceneView S, uint32_t n, const float *__restrict__ d
     const Frame fr = FrameFromZ(mk(nrm[i * 3], nrm[i * 3 + 1], nrm[i * 3 + 2]));
     pdf[i] = EmPdf(S.em, S.em.sampler, mk(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]), fr, (uint32_t)sides[i]);
 }
+// Scene::Intersect / Occluded of n rays, one thread per ray (function-level parity).
+__global__ void IntersectKernel(SceneView S, uint32_t n, const float *__restrict__ org, const float *__restrict__ dir,
+                                const float *__restrict__ tmin, float *__restrict__ dist, float *__restrict__ nrm,
+                                int32_t *__restrict__ mat) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Hit h;
+    h.dist = 1e36f;
+    h.mat = -1;
+    h.n = mk(0.f);
+    const bool hit = Intersect(S, mk(org[i * 3], org[i * 3 + 1], org[i * 3 + 2]),
+                               mk(dir[i * 3], dir[i * 3 + 1], dir[i * 3 + 2]), tmin[i], h);
+    dist[i] = hit ? h.dist : -1.f;
+    mat[i] = hit ? h.mat : -1;
+    nrm[i * 3] = hit ? h.n.x : 0.f;
+    nrm[i * 3 + 1] = hit ? h.n.y : 0.f;
+    nrm[i * 3 + 2] = hit ? h.n.z : 0.f;
+}
+__global__ void OccludedKernel(SceneView S, uint32_t n, const float *__restrict__ p, const float *__restrict__ dir,
+                               const float *__restrict__ tmax, int32_t *__restrict__ occ) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    occ[i] = Occluded(S, mk(p[i * 3], p[i * 3 + 1], p[i * 3 + 2]), mk(dir[i * 3], dir[i * 3 + 1], dir[i * 3 + 2]),
+                      tmax[i]) ? 1 : 0;
+}
 __global__ void EmEvalKernel(SceneView S, uint32_t n, const float *__restrict__ dirs, float *__restrict__ rgb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -416,6 +441,16 @@ hipError_t LaunchEmSample(const SceneView &S, uint32_t n, const float *nrm, cons
 hipError_t LaunchEmPdf(const SceneView &S, uint32_t n, const float *dirs, const float *nrm, const int32_t *sides,
                        float *pdf, hipStream_t s) {
     hipLaunchKernelGGL(EmPdfKernel, dim3((n + 255) / 256), dim3(256), 0, s, S, n, dirs, nrm, sides, pdf);
+    return hipGetLastError();
+}
+hipError_t LaunchIntersect(const SceneView &S, uint32_t n, const float *org, const float *dir, const float *tmin,
+                           float *dist, float *nrm, int32_t *mat, hipStream_t s) {
+    hipLaunchKernelGGL(IntersectKernel, dim3((n + 255) / 256), dim3(256), 0, s, S, n, org, dir, tmin, dist, nrm, mat);
+    return hipGetLastError();
+}
+hipError_t LaunchOccluded(const SceneView &S, uint32_t n, const float *p, const float *dir, const float *tmax,
+                          int32_t *occ, hipStream_t s) {
+    hipLaunchKernelGGL(OccludedKernel, dim3((n + 255) / 256), dim3(256), 0, s, S, n, p, dir, tmax, occ);
     return hipGetLastError();
 }
 hipError_t LaunchEmEval(const SceneView &S, uint32_t n, const float *dirs, float *rgb, hipStream_t s) {

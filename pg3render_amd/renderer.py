@@ -277,6 +277,32 @@ class Context:
         check(self._lib.pg3g_em_eval_batch(self.ptr, n, fp(dirs), fp(rgb)), "pg3g_em_eval_batch")
         return rgb
 
+    def intersect(self, org, dir, tmin):
+        """Scene::Intersect of slot 0 for n rays: (dist (-1: miss), normal (n, 3), material (-1))."""
+        n = len(org)
+        org = np.ascontiguousarray(org, np.float32)
+        dir = np.ascontiguousarray(dir, np.float32)
+        tmin = np.ascontiguousarray(np.broadcast_to(np.asarray(tmin, np.float32), (n,)))
+        dist = np.zeros(n, np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        mat = np.zeros(n, np.int32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        check(self._lib.pg3g_intersect_batch(self.ptr, n, fp(org), fp(dir), fp(tmin), fp(dist), fp(nrm),
+                                             mat.ctypes.data_as(C.POINTER(C.c_int32))), "pg3g_intersect_batch")
+        return dist, nrm, mat
+
+    def occluded(self, p, dir, tmax):
+        """Scene::Occluded of slot 0 for n segments: bool array."""
+        n = len(p)
+        p = np.ascontiguousarray(p, np.float32)
+        dir = np.ascontiguousarray(dir, np.float32)
+        tmax = np.ascontiguousarray(np.broadcast_to(np.asarray(tmax, np.float32), (n,)))
+        occ = np.zeros(n, np.int32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        check(self._lib.pg3g_occluded_batch(self.ptr, n, fp(p), fp(dir), fp(tmax),
+                                            occ.ctypes.data_as(C.POINTER(C.c_int32))), "pg3g_occluded_batch")
+        return occ != 0
+
     def steerable_leaves(self):
         nl, nv, lv = C.c_uint32(), C.c_uint32(), C.c_uint32()
         check(self._lib.pg3g_steerable_info(self.ptr, C.byref(nl), C.byref(nv), C.byref(lv)), "pg3g_steerable_info")

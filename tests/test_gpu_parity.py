@@ -465,3 +465,66 @@ def test_path_length_bounds(ctx, kernel, min_pl, max_pl):
     g, o = render_pair(ctx, sb, 4)
     assert_image_parity(g, o, frac=0.03)
     ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+
+
+# ---- scene intersection (scene.hxx:42-74), function level ---------------------------------------
+from geom64 import cast as cast64, random_rays  # noqa: E402
+
+ISECT_SCENES = [pytest.param(lambda: SceneBuild.workload(1, seed=51, res=8), id="workload1"),
+                pytest.param(lambda: SceneBuild.workload(2, seed=52, res=8), id="workload2"),
+                pytest.param(lambda: SceneBuild.reference(2, em_type=0, resolution=8), id="reference_scene2"),
+                pytest.param(lambda: SceneBuild.reference(11, em_type=0, resolution=8), id="reference_scene11")]
+
+
+def _isect_rays(seed, n=100000):
+    rng = np.random.default_rng(seed)
+    org_in, dir_in = random_rays(rng, n // 2)
+    org_out, dir_out = random_rays(rng, n - n // 2, -1.0, 2.0)
+    org, dir = np.concatenate([org_in, org_out]), np.concatenate([dir_in, dir_out])
+    tmin = rng.choice([0.0, 1e-3, 0.05], len(org)).astype(np.float32)
+    return rng, org, dir, tmin
+
+
+@pytest.mark.parametrize("build", ISECT_SCENES)
+def test_intersect_parity(ctx, build):
+    """pg3g_intersect_batch against the oracle on 100k rays from inside and outside the scene:
+    the same hit/miss and material, distances and normals to float32 rounding (the GPU tests
+    triangle edges as Pluecker lines and contracts FMAs), except on rays within 1e-5 of an edge,
+    a tangent, a bound or a second surface (classified by an independent FP64 ray caster)."""
+    sb = build()
+    ctx.upload(sb)
+    _, org, dir, tmin = _isect_rays(61)
+    dg, ng, mg = ctx.intersect(org, dir, tmin)
+    do, no, mo = O.OracleScene(sb).intersect(org, dir, tmin)
+    _, _, _, margin = cast64(sb, org, dir, tmin)
+    clear = margin > 1e-5
+    assert clear.mean() > 0.99
+    assert np.array_equal((dg >= 0)[clear], (do >= 0)[clear])
+    assert np.array_equal(mg[clear], mo[clear])
+    both = clear & (do >= 0)
+    assert both.mean() > 0.3
+    rel = np.abs(dg[both] - do[both]) / np.maximum(do[both], 1e-2)
+    assert rel.max() < 1e-5, rel.max()
+    # normals: near-tangent sphere hits amplify an ulp of the quadratic's coefficients by
+    # 1/sqrt(discriminant); the bulk agrees to float32 rounding
+    en = np.abs(ng[both] - no[both]).max(axis=1)
+    assert en.max() < 1e-4 and np.quantile(en, 0.999) < 2e-6, (en.max(), np.quantile(en, 0.999))
+    # all rays, ambiguous ones included: disagreements stay rare
+    assert np.mean((dg >= 0) != (do >= 0)) < 1e-3
+
+
+@pytest.mark.parametrize("build", ISECT_SCENES)
+def test_occluded_parity(ctx, build):
+    sb = build()
+    ctx.upload(sb)
+    rng, p, dir, _ = _isect_rays(62)
+    tmax = rng.uniform(0.05, 1.5, len(p)).astype(np.float32)
+    og = ctx.occluded(p, dir, tmax)
+    oo = O.OracleScene(sb).occluded(p, dir, tmax)
+    org = p.astype(np.float64) + 1e-3 * dir.astype(np.float64)
+    _, _, _, margin = cast64(sb, org, dir, 0.0, tmax.astype(np.float64) - 2e-3)
+    clear = margin > 1e-5
+    assert clear.mean() > 0.99
+    assert np.array_equal(og[clear], oo[clear])
+    assert np.mean(og != oo) < 1e-3
+    assert 0.05 < oo.mean() < 0.95

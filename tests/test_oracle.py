@@ -499,3 +499,55 @@ def test_direct_illumination_estimators_agree():
         means[alg] = img.mean() / (2048 if alg == "dbs" else 512)
     for alg, m in means.items():
         assert abs(m - means["dmis"]) / means["dmis"] < 0.02, (alg, means)
+
+
+# ---- scene intersection (scene.hxx:42-74, scene_graph.hxx:33-245) -----------------------------
+from geom64 import cast as cast64, random_rays  # noqa: E402
+
+
+@pytest.mark.parametrize("workload", [1, 2])
+def test_oracle_intersect_against_fp64_ray_casting(workload):
+    """The oracle's closest hit (float32, the reference's edge-sign test and FP64 sphere roots)
+    against an independent float64 ray caster: identical hit/miss and material, distance and
+    normal to float32 rounding, except on rays within 1e-5 of an edge, a tangent, a bound or a
+    second surface."""
+    sb = SceneBuild.workload(workload, seed=41, res=8)
+    orc = O.OracleScene(sb)
+    rng = np.random.default_rng(41 + workload)
+    # half the rays start inside the box, half outside it (walls hit from both sides)
+    org_in, dir_in = random_rays(rng, 10000)
+    org_out, dir_out = random_rays(rng, 10000, -1.0, 2.0)
+    org, dir = np.concatenate([org_in, org_out]), np.concatenate([dir_in, dir_out])
+    tmin = rng.choice([0.0, 1e-3, 0.05], len(org)).astype(np.float32)
+    dist, nrm, mat = orc.intersect(org, dir, tmin)
+    t64, m64, n64, margin = cast64(sb, org, dir, tmin)
+    clear = margin > 1e-5
+    assert clear.mean() > 0.99
+    hit64 = np.isfinite(t64)
+    assert np.array_equal((dist >= 0)[clear], hit64[clear])
+    assert np.array_equal(mat[clear], m64[clear])
+    both = clear & hit64
+    assert both.mean() > 0.3
+    rel = np.abs(dist[both] - t64[both]) / np.maximum(t64[both], 1e-2)
+    assert rel.max() < 2e-5, rel.max()
+    assert np.abs(nrm[both] - n64[both]).max() < 2e-5
+    # a plausible slip (a dropped sphere root or one-sided triangles) would show here
+    assert (mat[clear & hit64] >= 0).all() and len(np.unique(m64[both])) >= 5
+
+
+def test_oracle_occluded_against_fp64_ray_casting():
+    """Occluded(p, dir, tmax) = a hit between p + 1e-3 dir and tmax - 2e-3 (scene.hxx:61-74)."""
+    sb = SceneBuild.workload(2, seed=42, res=8)
+    orc = O.OracleScene(sb)
+    rng = np.random.default_rng(43)
+    p_in, dir_in = random_rays(rng, 10000)
+    p_out, dir_out = random_rays(rng, 10000, -1.0, 2.0)
+    p, dir = np.concatenate([p_in, p_out]), np.concatenate([dir_in, dir_out])
+    tmax = rng.uniform(0.05, 1.5, len(p)).astype(np.float32)
+    occ = orc.occluded(p, dir, tmax)
+    org = p.astype(np.float64) + 1e-3 * dir.astype(np.float64)
+    t64, _, _, margin = cast64(sb, org, dir, 0.0, tmax.astype(np.float64) - 2e-3)
+    clear = margin > 1e-5
+    assert clear.mean() > 0.99
+    assert np.array_equal(occ[clear], np.isfinite(t64)[clear])
+    assert 0.1 < occ.mean() < 0.8
