@@ -631,7 +631,8 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
 // recursion of path_tracer.hxx:167-430 run as a post-order walk over an explicit stack of
 // vertices, since clipping needs each child's reflected estimate before it is weighted.  Paths
 // are cut at kTreeDepth vertices (the reference cuts at 1200, kMaxPathLengthHard); with
-// -maxpl <= kTreeDepth the two agree exactly.
+// -maxpl <= kTreeDepth the two agree exactly.  (SplitPath, the first-vertex splitting of the
+// reference's defaults, re-traces deep chains instead and keeps the reference's 1200.)
 constexpr uint32_t kTreeDepth = 48;
 
 // MISWeight2Power with sample counts, path_tracer_base.hxx:522-541
@@ -1051,10 +1052,15 @@ struct ChainPath {
 // of different BSDF samples add, so the nB subtrees are traced one after another, each a chain.
 // Clipping makes a chain vertex's estimate R_v = A_v + clip(w_v R_{v+1}) depend on the whole
 // suffix (A_v: its light samples and the emission its BSDF sample finds), so each vertex pushes
-// (A_v, the factors of w_v) on a per-lane stack and the chain's end folds it back to front.  One
-// vertex per step, the same RNG streams, operations and order as RadiancePTTree, which handles
-// every other splitting configuration.
-constexpr uint32_t kChainStack = kTreeDepth - 1;   // vertices 2 .. kTreeDepth
+// (A_v, the factors of w_v) on a per-lane ring of kChainStack records and the chain's end folds
+// them back to front.  A chain deeper than the ring (Russian roulette lets a path bounce inside
+// a dielectric for many vertices; the reference stops at 1200) folds its deepest kChainStack
+// records, then re-traces the chain from its start up to the folded depth (the counter-based RNG
+// replays it exactly) and folds the next segment onto the folded suffix, and so on to the root:
+// the same operations in the same order as one fold of the whole chain.  One vertex per step,
+// the same RNG streams, operations and order as RadiancePTTree, which handles every other
+// splitting configuration.
+constexpr uint32_t kChainStack = kTreeDepth - 1;
 struct SplitPath {
     // ray in flight: it reaches vertex `len` on RNG branch `branch`
     F3 org, dir;
@@ -1075,10 +1081,13 @@ struct SplitPath {
     float pPdfW, pComp, pLcSum;
     uint32_t pSides, pLen, pBranch, pnB, pnL;
     bool pFinite;
-    // chain stack: vertices 2 .. ; the top entry's A is kept in topA
-    int32_t top;
-    uint64_t cutMask;
-    F3 topA;
+    // chain ring: record d (chain depth d = vertex d + 2) in slot d % kChainStack; `top` is the
+    // newest record's depth (-1: none) and its A is kept in topA.  Re-tracing a deep chain:
+    // `target` is the depth at which the pass stops (-1: first pass) and rSuffix the folded
+    // estimate of the chain beyond it.
+    int32_t top, target;
+    uint64_t cutMask;            // per slot: the record's BSDF sample was cut by Russian roulette
+    F3 topA, rSuffix;
     float stA[kChainStack][3], stW[kChainStack][3], stDen[kChainStack];
     F3 L;
 
@@ -1091,7 +1100,9 @@ struct SplitPath {
         em1 = mk(0.f);
         reflRoot = mk(0.f);
         top = -1;
+        target = -1;
     }
+    PG3_DEV static uint32_t Slot(int32_t d) { return (uint32_t)d % kChainStack; }
     PG3_DEV RngKey Branch(uint32_t b) const {
         RngKey k = key;
         k.c3 = (key.c3 & 0xFFFF0000u) | b;
@@ -1181,19 +1192,31 @@ struct SplitPath {
         if (iic > 0.f) ind = ClipProportionally(ind, iic);
         return ind;
     }
-    // End of a first-vertex subtree: fold the chain stack and add the clipped indirect term.
-    PG3_DEV void FoldChain(const SceneView &S) {
-        F3 R = mk(0.f);
-        for (int32_t k = top; k >= 0; --k) {
-            const F3 A = (k == top) ? topA : mk(stA[k][0], stA[k][1], stA[k][2]);
+    // End of a chain pass (records [0, top] pushed): fold the records the ring still holds onto
+    // the suffix beyond them.  If older records were overwritten, set up the re-trace up to the
+    // oldest folded depth and return false (the caller restarts the chain from the same
+    // first-vertex sample); otherwise add the clipped indirect term of the subtree and return true.
+    PG3_DEV bool FoldChain(const SceneView &S) {
+        const int32_t lo = max(0, top + 1 - (int32_t)kChainStack);
+        F3 R = (target >= 0) ? rSuffix : mk(0.f);
+        for (int32_t k = top; k >= lo; --k) {
+            const uint32_t sl = Slot(k);
+            const F3 A = (k == top) ? topA : mk(stA[sl][0], stA[sl][1], stA[sl][2]);
             F3 Rk = A;
-            if (!((cutMask >> k) & 1ull) && !IsZero(R))
-                Rk += Indirect(mk(stW[k][0], stW[k][1], stW[k][2]), R, stDen[k], S.iic);
+            if (!((cutMask >> sl) & 1ull) && !IsZero(R))
+                Rk += Indirect(mk(stW[sl][0], stW[sl][1], stW[sl][2]), R, stDen[sl], S.iic);
             R = Rk;
         }
-        if (!rCut && !IsZero(R)) reflRoot += Indirect(rAttCos, R, rDen, S.iic);
         top = -1;
+        if (lo > 0) {
+            rSuffix = R;
+            target = lo;
+            return false;
+        }
+        if (!rCut && !IsZero(R)) reflRoot += Indirect(rAttCos, R, rDen, S.iic);
+        target = -1;
         ++s;
+        return true;
     }
     PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, uint32_t &segments) {
         const RngKey k = Branch(branch);
@@ -1226,11 +1249,13 @@ struct SplitPath {
             if (len > 1) lpdf = BgPdf(S, spec, dir, FrameFromZ(pN), pSides);
             lid = S.bgLight;
         }
+        // (a re-traced pass delivers the emission of vertex 2 to the first vertex only once)
         if (len == 1) em1 = em;
-        else if (len == 2) reflRoot += ParentEmission(S, spec, em, lpdf, lid);
+        else if (len == 2) { if (target < 0) reflRoot += ParentEmission(S, spec, em, lpdf, lid); }
         else topA += ParentEmission(S, spec, em, lpdf, lid);
-        const bool stop = !hit || !refl || ReflectanceZero(mat) || (S.maxPL > 0 && len >= S.maxPL) || len >= 1200u ||
-                          len > kTreeDepth;
+        const bool atTarget = (target >= 0) && ((int32_t)len - 2 == target);
+        const bool stop = atTarget || !hit || !refl || ReflectanceZero(mat) || (S.maxPL > 0 && len >= S.maxPL) ||
+                          len >= 1200u;
         // what the step samples next: the first vertex's next BSDF sample or the chain vertex's one
         bool wantRoot = false, wantChain = false;
         float rr = 1.0f;
@@ -1244,7 +1269,7 @@ struct SplitPath {
             FoldChain(S);
             wantRoot = true;
         } else {
-            ++segments;
+            if (target < 0) ++segments;
             if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
             const bool first = (len == 1);
             if (first) {                                                              // :240-250
@@ -1267,9 +1292,10 @@ struct SplitPath {
                 wantRoot = true;
             } else {
                 if (top >= 0) {
-                    stA[top][0] = topA.x;
-                    stA[top][1] = topA.y;
-                    stA[top][2] = topA.z;
+                    const uint32_t sl = Slot(top);
+                    stA[sl][0] = topA.x;
+                    stA[sl][1] = topA.y;
+                    stA[sl][2] = topA.z;
                 }
                 ++top;
                 topA = A;
@@ -1306,11 +1332,12 @@ struct SplitPath {
                 continue;
             }
             if (ok) {
-                stW[top][0] = attCos.x;
-                stW[top][1] = attCos.y;
-                stW[top][2] = attCos.z;
-                stDen[top] = den;
-                if (cut) cutMask |= 1ull << top;
+                const uint32_t sl = Slot(top);
+                stW[sl][0] = attCos.x;
+                stW[sl][1] = attCos.y;
+                stW[sl][2] = attCos.z;
+                stDen[sl] = den;
+                cutMask = (cutMask & ~(1ull << sl)) | ((uint64_t)cut << sl);
                 return false;
             }
             FoldChain(S);                                   // a blocker: this vertex has no subtree
