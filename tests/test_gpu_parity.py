@@ -547,3 +547,18 @@ def test_every_reference_scene_id(ctx, scene_id, mode):
         sb.set_params(algorithm="pt", splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0, seed=9)
     g, o = render_pair(ctx, sb, 4)
     assert_image_parity(g, o, frac=0.03)
+
+
+@pytest.mark.parametrize("alg", ["ptn", "dbs", "dlsa", "dlss", "dmis"])
+@pytest.mark.parametrize("scene_id", ALL_SCENE_IDS)
+def test_every_reference_scene_id_other_algorithms(ctx, scene_id, alg):
+    """The naive path tracer and the four direct-illumination estimators (path_tracer.hxx:45-165,
+    direct_illumination.hxx:15-156) on every supported reference scene ID."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.reference(scene_id, resolution=24)
+    sb.set_params(algorithm=alg, max_path_length=8, seed=10)
+    g, o = render_pair(ctx, sb, 4)
+    if o.sum() == 0:      # e.g. point lights only, which ptn and dbs cannot hit
+        assert g.sum() == 0
+        return
+    assert_image_parity(g, o, frac=0.03)
