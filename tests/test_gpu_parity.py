@@ -562,3 +562,32 @@ def test_every_reference_scene_id_other_algorithms(ctx, scene_id, alg):
         assert g.sum() == 0
         return
     assert_image_parity(g, o, frac=0.03)
+
+
+EM_SCENE_IDS = [6, 7, 8, 9, 10, 12, 13, 14, 15, 20, 21, 22, 23, 24, 25, 26, 28, 29, 37]   # kLightEnv scenes
+
+
+@pytest.mark.parametrize("sampler", [_ffi.EM_SIMPLE, _ffi.EM_STEERABLE], ids=["simple", "steerable"])
+@pytest.mark.parametrize("mode", ["chain", "defaults"])
+@pytest.mark.parametrize("scene_id", EM_SCENE_IDS)
+def test_every_em_scene_with_image_em(ctx, scene_id, mode, sampler, tmp_path):
+    """The reference's environment-light scene IDs with an image EM read from a file (em type 7,
+    rotation 0.83 and scale 1.5, scene.hxx:891-897): a 128x64 probe of a sky gradient and three
+    bright lobes, under both EM samplers, with the chain settings and with the reference's
+    defaults (-sb 4 -iic 16, Russian roulette)."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    yy, xx = np.meshgrid(np.arange(64), np.arange(128), indexing="ij")
+    img = 0.2 + 0.8 * (1 - yy / 64)[..., None] * np.array([0.5, 0.75, 1.0])
+    for cx, cy, peak in ((30, 12, 60.0), (90, 20, 25.0), (64, 40, 10.0)):
+        img = img + peak * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / 18.0)[..., None]
+    path = str(tmp_path / "probe.pfm")
+    with open(path, "wb") as f:
+        f.write(b"PF\n128 64\n-1.0\n" + np.ascontiguousarray(img[::-1]).astype("<f4").tobytes())
+    sb = SceneBuild.reference(scene_id, em_type=7, resolution=24, em_path=path)
+    sb.desc.em.sampler = sampler
+    if mode == "chain":
+        sb.set_params(algorithm="pt", splitting_budget=1.0, indirect_clipping=0.0, max_path_length=8, seed=11)
+    else:
+        sb.set_params(algorithm="pt", splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0, seed=11)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
