@@ -533,19 +533,20 @@ def test_occluded_parity(ctx, build):
 ALL_SCENE_IDS = [i for i in range(38) if i != 27]   # 27: the SRA layered material, outside the GPU path
 
 
-@pytest.mark.parametrize("mode", ["chain", "defaults"])
+@pytest.mark.parametrize("mode", ["chain", "chain_wavefront", "defaults"])
 @pytest.mark.parametrize("scene_id", ALL_SCENE_IDS)
 def test_every_reference_scene_id(ctx, scene_id, mode):
     """Every reference scene ID the GPU path supports (config.hxx:45-115, scene.hxx:176-956),
-    with the chain settings (-sb 1 -iic 0 -maxpl 8) and with the reference's command-line
-    defaults (-sb 4 -iic 16, Russian roulette)."""
-    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    with the chain settings (-sb 1 -iic 0 -maxpl 8; fused kernel and wavefront pipeline) and with
+    the reference's command-line defaults (-sb 4 -iic 16, Russian roulette)."""
+    ctx.set_kernel_mode(_ffi.KERNEL_WAVEFRONT if mode == "chain_wavefront" else _ffi.KERNEL_MEGAKERNEL)
     sb = SceneBuild.reference(scene_id, resolution=24)
-    if mode == "chain":
+    if mode.startswith("chain"):
         sb.set_params(algorithm="pt", splitting_budget=1.0, indirect_clipping=0.0, max_path_length=8, seed=9)
     else:
         sb.set_params(algorithm="pt", splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0, seed=9)
     g, o = render_pair(ctx, sb, 4)
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
     assert_image_parity(g, o, frac=0.03)
 
 
