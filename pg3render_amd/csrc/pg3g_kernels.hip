@@ -1,6 +1,8 @@
 // gfx950 kernels of the path-tracing hot path and their launchers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pg3g_internal.hpp"
 #include "pt_kernels.hpp"
 
@@ -380,6 +382,18 @@ static PTKernelFn PickPTKernel(const SceneView &S, bool split) {
 hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
                             unsigned long long *counters, uint32_t *work, int numCUs, hipStream_t stream) {
     if (nIter == 0) return hipSuccess;
+    // work-item and thread indices are 32-bit: a range of more than 2^30 (tile, iteration)
+    // pairs is rendered as consecutive launches (iterations add independently)
+    const uint64_t tiles64 = (uint64_t)((S.resX + 7u) / 8u) * ((S.resY + 7u) / 8u);
+    const uint32_t maxIter = (uint32_t)std::max<uint64_t>(1u, (1ull << 30) / tiles64);
+    if (nIter > maxIter) {
+        for (uint32_t f = 0; f < nIter; f += maxIter) {
+            const hipError_t e = LaunchMegakernel(S, Sdev, firstIter + f, std::min(maxIter, nIter - f), fb, counters, work,
+                                                  numCUs, stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     const uint32_t npix = S.resX * S.resY;
     // Enough lanes to fill 256 CUs several times over; each thread keeps >= 1 iteration.
     const uint64_t targetThreads = 1u << 21;
