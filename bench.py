@@ -246,7 +246,7 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sharding design is in the docs)", file=sys.stderr)
         sys.exit(2)
     from pg3render_amd import Context, SceneBuild, _ffi
 
