@@ -14,7 +14,8 @@
 // atomic per queue per workgroup and a popcount prefix give each lane its slot, so the lanes of a
 // wave write consecutive records.  Every launch is a grid-stride loop over a device-side count, so
 // the host never waits for the GPU between bounces and the whole batch is a fixed launch sequence
-// (capturable in a HIP graph).  Per-vertex shading is the same ShadeVertex the fused kernel uses.
+// (capturable in a HIP graph); under Russian roulette the host reads the live count back every
+// 8 bounces instead.  Per-vertex shading is the same ShadeVertex the fused kernel uses.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -383,7 +384,8 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
                            unsigned long long *counters, uint32_t *work, int numCUs, WavefrontContext *W,
                            hipStream_t stream) {
     if (!W) return hipErrorInvalidValue;
-    if (S.algorithm != 7 || S.maxPL == 0 || S.splitBudget != 1.f || S.iic > 0.f) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, work, numCUs, stream);
+    if (S.algorithm != 7 || S.splitBudget != 1.f || S.iic > 0.f)
+        return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, work, numCUs, stream);
     if (nIter == 0) return hipSuccess;
     if (W->gridBlocks == 0) {
         int dev = 0, cus = 256;
@@ -402,17 +404,28 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
     const ShadeFn shade = PickShadeKernel(S);
     if (!shade) return LaunchMegakernel(S, Sdev, firstIter, nIter, fb, counters, work, numCUs, stream);
     const dim3 grid(W->gridBlocks), block(kWfBlock);
+    // With a path-length limit the batch is a fixed launch sequence.  Under Russian roulette
+    // (-maxpl 0) the number of bounces is open: bounces run in groups of 8 and the live-path
+    // count is read back after each group; every path stops at 1200 vertices at the latest.
+    const uint32_t maxBounces = S.maxPL > 0 ? S.maxPL : 1200u;
     for (uint64_t base = 0; base < total; base += cap) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(cap, total - base);
         hipLaunchKernelGGL(WfClearCounts, dim3(1), dim3(64), 0, stream, Q.counts, 0xFu);
         hipLaunchKernelGGL(WfGenerate, grid, block, 0, stream, S, Q, base, count, firstIter);
-        for (uint32_t b = 1; b <= S.maxPL; ++b) {
+        for (uint32_t b = 1; b <= maxBounces; ++b) {
             const uint32_t qi = (b - 1) & 1u, qo = qi ^ 1u;
             hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, qi);
             hipLaunchKernelGGL(WfClearCounts, dim3(1), dim3(64), 0, stream, Q.counts, (1u << qo) | 4u);
             hipLaunchKernelGGL(shade, dim3(W->shadeBlocks), block, 0, stream, Sdev, Q, qi, qo, b, base,
                                firstIter);
             hipLaunchKernelGGL(WfShadow, grid, block, 0, stream, S, Q, qo);
+            if (S.maxPL == 0 && (b % 8u) == 0u && b < maxBounces) {
+                uint32_t live = 0;
+                e = hipMemcpyAsync(&live, Q.counts + qo, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(stream);
+                if (e != hipSuccess) return e;
+                if (live == 0) break;
+            }
         }
         hipLaunchKernelGGL(WfAccumulate, grid, block, 0, stream, S, Q, base, count, firstIter, fb, counters);
         e = hipGetLastError();

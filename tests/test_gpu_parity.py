@@ -241,6 +241,31 @@ def test_wavefront_equals_megakernel(ctx, workload, sampler):
     assert abs(a.sum() - b.sum()) / a.sum() < 1e-6
 
 
+@pytest.mark.parametrize("build", [pytest.param(lambda: SceneBuild.workload(2, seed=28, res=48), id="workload2"),
+                                   pytest.param(lambda: SceneBuild.reference(33, resolution=32), id="reference_scene33"),
+                                   pytest.param(lambda: SceneBuild.reference(12, resolution=32), id="reference_scene12")])
+def test_wavefront_russian_roulette(ctx, build):
+    """-maxpl 0: the wavefront pipeline runs bounces until no path is live (the host reads the
+    live count back every 8 bounces) and matches the fused kernel on the same RNG streams, also
+    where paths bounce inside a dielectric sphere for dozens of vertices (scene 33)."""
+    sb = build()
+    sb.set_params(algorithm="pt", splitting_budget=1.0, indirect_clipping=0.0, max_path_length=0, seed=5)
+    out = []
+    for k in (_ffi.KERNEL_MEGAKERNEL, _ffi.KERNEL_WAVEFRONT):
+        ctx.set_kernel_mode(k)
+        ctx.upload(sb)
+        ctx.synchronize()
+        ctx.reset_counters()
+        ctx.render(0, 8)
+        out.append((ctx.framebuffer(0, sb.resolution)[0], ctx.counters()))
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    (a, ca), (b, cb) = out
+    assert ca == cb                      # the same segments and samples
+    rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-6)
+    assert np.mean(rel > 1e-6) < 1e-3
+    assert abs(a.sum() - b.sum()) / a.sum() < 1e-6
+
+
 @pytest.mark.parametrize("res", [37, 9])
 def test_wavefront_ragged_resolution(ctx, res):
     """Resolutions that leave 8x8 tiles partly outside the image: padding samples are never
