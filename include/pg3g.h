@@ -114,8 +114,8 @@ typedef struct pg3g_render_params {
     uint32_t max_path_length;    /* -maxpl (0 = Russian roulette) */
     float    indirect_clipping;  /* -iic (0 = off) */
     float    splitting_budget;   /* -sb (the GPU path supports 1) */
-    float    splitting_level;    /* -slbr (default 1) */
-    float    light_to_bsdf_ratio;/* -sl  (default 1) */
+    float    splitting_level;    /* -sl (default 1) */
+    float    light_to_bsdf_ratio;/* -slbr (default 1) */
     uint64_t seed;               /* counter-based RNG key */
     uint32_t scene_id;           /* RNG stream id of the scene (batched renders) */
 } pg3g_render_params;

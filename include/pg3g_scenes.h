@@ -47,6 +47,11 @@ int pg3g_build_reference_scene(uint32_t scene_id, uint32_t em_type, uint32_t res
  * (LoadEnvironmentMap arguments, scene.hxx:850-943), in place of its OpenEXR file. */
 int pg3g_build_reference_scene_em(uint32_t scene_id, uint32_t em_type, uint32_t resolution, const char *em_path,
                                   pg3g_scene_build **out);
+/* The short name of a reference scene ID with environment map em_type, as the reference builds
+ * it for default output file names (Scene::GetSceneName's acronym, src/scene.hxx:1035-1273:
+ * geometry, lights with the EM type number, materials, separated by '_').  Writes at most len
+ * bytes including the terminating zero. */
+int pg3g_reference_scene_acronym(uint32_t scene_id, uint32_t em_type, char *buf, size_t len);
 
 /* Description view of a built scene (valid until the build is freed; callers may edit params and
  * the material/light arrays in place through pg3g_scene_build_mutable). */

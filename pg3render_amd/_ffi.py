@@ -142,6 +142,7 @@ _SIGNATURES = {
     "pg3g_build_workload_scene": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "pg3g_build_reference_scene": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "pg3g_build_reference_scene_em": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_char_p, C.POINTER(_P)]),
+    "pg3g_reference_scene_acronym": (C.c_int, [C.c_uint32, C.c_uint32, C.c_char_p, C.c_size_t]),
     "pg3g_scene_build_desc": (C.POINTER(SceneDesc), [_P]),
     "pg3g_scene_build_mutable": (C.POINTER(SceneDesc), [_P]),
     "pg3g_scene_build_em_texels": (_FP, [_P]),

@@ -591,6 +591,51 @@ int pg3g_build_workload_scene(int workload, uint64_t seed, uint32_t variant, uin
     return PG3G_OK;
 }
 
+int pg3g_reference_scene_acronym(uint32_t scene_id, uint32_t em_type, char *buf, size_t len) {
+    if (!buf || len == 0 || scene_id >= sizeof(kSceneConfigs) / sizeof(kSceneConfigs[0])) return PG3G_ERR_INVALID_ARG;
+    const uint32_t M = kSceneConfigs[scene_id];
+    std::string a;
+    // geometry
+    const size_t g0 = a.size();
+    if (M & kWalls) a += "w";
+    if (M & k2Spheres) a += "2s";
+    if (M & k1Sphere) a += "1s";
+    if (M & kLayeredSphere) a += "ls";
+    if (M & kVerticalRectangle) a += "vr";
+    if (M & kDiagonalRectangles) a += "dr";
+    if (a.size() == g0 && !(M & (kWalls | kFloor | k2Spheres | k1Sphere | kLayeredSphere | kRects))) a += "e";
+    a += "_";
+    // lights (the environment light carries its EM type number)
+    if (M & kLightCeiling) a += "c";
+    if (M & kLightBox) a += "b";
+    if (M & kLightPoint) a += "p";
+    if (M & kLightEnv) a += "e" + std::to_string(em_type);
+    a += "_";
+    // materials
+    if (M & (kSpheresPhongDiffuse | kSpheresPhongGlossy)) {
+        a += "Sp";
+        if (M & kSpheresPhongDiffuse) a += "d";
+        if (M & kSpheresPhongGlossy) a += "g";
+    } else if (M & kSpheresFresnelConductor) a += "Sffc";
+    else if (M & kSpheresFresnelDielectric) a += "Sffd";
+    else if (M & kSpheresMicrofacetGGXConductor) a += "Smgc";
+    else if (M & kSpheresMicrofacetGGXDielectric) a += "Smgd";
+    else if (M & kSpheresSraLayers) a += "Ssral";
+    if (M & (kWallsPhongDiffuse | kWallsPhongGlossy)) {
+        a += "Wp";
+        if (M & kWallsPhongDiffuse) a += "d";
+        if (M & kWallsPhongGlossy) a += "g";
+    }
+    if (M & kVerticalRectangle) {
+        if (M & kVertRectFresnelDielectric) a += "Rffd";
+        else if (M & kVertRectMicrofacetGGXDielectric) a += "Rmgd";
+        else a += "Rpd";
+    }
+    if (a.size() + 1 > len) return PG3G_ERR_INVALID_ARG;
+    std::memcpy(buf, a.c_str(), a.size() + 1);
+    return PG3G_OK;
+}
+
 int pg3g_build_reference_scene(uint32_t scene_id, uint32_t em_type, uint32_t resolution, pg3g_scene_build **out) {
     if (!out) return PG3G_ERR_INVALID_ARG;
     *out = nullptr;
