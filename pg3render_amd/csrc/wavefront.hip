@@ -167,7 +167,11 @@ __global__ void __launch_bounds__(kWfBlock) WfGenerate(SceneView S, WfQueues Q, 
     }
 }
 
-__global__ void __launch_bounds__(kWfBlock) WfIntersect(SceneView S, WfQueues Q, uint32_t qi) {
+// Also zeroes the counters selected by clearMask (bit k: counts[k]; never qi): the shading stage
+// that follows appends to them, and every earlier reader has finished (stream order), so no
+// launch of its own is needed for the clear.
+__global__ void __launch_bounds__(kWfBlock) WfIntersect(SceneView S, WfQueues Q, uint32_t qi, uint32_t clearMask) {
+    if (blockIdx.x == 0 && threadIdx.x < 4u && (clearMask >> threadIdx.x & 1u)) Q.counts[threadIdx.x] = 0u;
     const uint32_t count = Q.counts[qi];
     const WfPaths P = Q.P[qi];
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
@@ -414,8 +418,7 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
         hipLaunchKernelGGL(WfGenerate, grid, block, 0, stream, S, Q, base, count, firstIter);
         for (uint32_t b = 1; b <= maxBounces; ++b) {
             const uint32_t qi = (b - 1) & 1u, qo = qi ^ 1u;
-            hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, qi);
-            hipLaunchKernelGGL(WfClearCounts, dim3(1), dim3(64), 0, stream, Q.counts, (1u << qo) | 4u);
+            hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, qi, (1u << qo) | 4u);
             hipLaunchKernelGGL(shade, dim3(W->shadeBlocks), block, 0, stream, Sdev, Q, qi, qo, b, base,
                                firstIter);
             hipLaunchKernelGGL(WfShadow, grid, block, 0, stream, S, Q, qo);
