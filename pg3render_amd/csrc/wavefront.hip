@@ -143,18 +143,23 @@ __device__ __forceinline__ uint64_t PixelToSample(const SceneView &S, uint32_t p
 }
 
 // Camera rays of samples [batchBase, batchBase + count) into live set 0; samples of the tile
-// padding outside the image are not enqueued.
+// padding outside the image are not enqueued.  Without padding (both image sides multiples of
+// 8, `dense`) every sample is valid and sample i takes slot i: no append, and the first lane
+// sets the count.
 __global__ void __launch_bounds__(kWfBlock) WfGenerate(SceneView S, WfQueues Q, uint64_t batchBase, uint32_t count,
-                                                       uint32_t firstIter) {
+                                                       uint32_t firstIter, bool dense) {
+    if (dense && blockIdx.x == 0 && threadIdx.x == 0) Q.counts[0] = count;
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < count; i0 += gridDim.x * blockDim.x) {
         const uint32_t i = i0 + threadIdx.x;
         uint32_t pixel = 0xFFFFFFFFu, it = 0;
         if (i < count) SampleToPixel(S, batchBase + i, firstIter, pixel, it);
         const bool valid = pixel != 0xFFFFFFFFu;
-        const bool p[1] = {valid};
-        uint32_t *const c[1] = {&Q.counts[0]};
-        uint32_t s[1];
-        BlockAppend<kWfBlock / 64, 1>(p, c, s);
+        uint32_t s[1] = {i};
+        if (!dense) {
+            const bool p[1] = {valid};
+            uint32_t *const c[1] = {&Q.counts[0]};
+            BlockAppend<kWfBlock / 64, 1>(p, c, s);
+        }
         if (valid) {
             RngKey key{S.k0, S.k1, pixel, it, S.sceneId << 16};
             const float jx = RngGet(key, 0, 0, 0), jy = RngGet(key, 0, 0, 1);   // path_tracer_base.hxx:74-77
@@ -412,10 +417,11 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
     // (-maxpl 0) the number of bounces is open: bounces run in groups of 8 and the live-path
     // count is read back after each group; every path stops at 1200 vertices at the latest.
     const uint32_t maxBounces = S.maxPL > 0 ? S.maxPL : 1200u;
+    const bool dense = (S.resX % 8u) == 0u && (S.resY % 8u) == 0u;
     for (uint64_t base = 0; base < total; base += cap) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(cap, total - base);
         hipLaunchKernelGGL(WfClearCounts, dim3(1), dim3(64), 0, stream, Q.counts, 0xFu);
-        hipLaunchKernelGGL(WfGenerate, grid, block, 0, stream, S, Q, base, count, firstIter);
+        hipLaunchKernelGGL(WfGenerate, grid, block, 0, stream, S, Q, base, count, firstIter, dense);
         for (uint32_t b = 1; b <= maxBounces; ++b) {
             const uint32_t qi = (b - 1) & 1u, qo = qi ^ 1u;
             hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, qi, (1u << qo) | 4u);
