@@ -852,15 +852,16 @@ PG3_DEV void CameraSample(const SceneView &S, uint32_t pixel, uint32_t iteration
 }
 
 // Stages the inner levels of a steerable EM's triangle tree in `lds` (12 floats per node as F4v
-// triples, at most kTreeLdsNodes nodes: the levels from the root down that fit, which are
+// triples, at most kMaxNodes nodes: the levels from the root down that fit, which are
 // contiguous at the end of nodeW), cooperatively by the workgroup; the caller synchronises the
 // workgroup before the returned cache is read.
 constexpr uint32_t kTreeLdsNodes = 1280;
+template <uint32_t kMaxNodes = kTreeLdsNodes>
 PG3_DEV TreeCache StageTree(const DevEm &E, F4v *lds) {
     TreeCache tc;
     uint32_t minLevel = E.numLevels, count = 0;
     for (int32_t l = (int32_t)E.numLevels - 1; l >= 1; --l) {
-        if (count + E.levelCount[l] > kTreeLdsNodes) break;
+        if (count + E.levelCount[l] > kMaxNodes) break;
         count += E.levelCount[l];
         minLevel = (uint32_t)l;
     }

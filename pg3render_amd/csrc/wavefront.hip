@@ -193,14 +193,21 @@ __global__ void __launch_bounds__(kWfBlock) WfIntersect(SceneView S, WfQueues Q,
 // and kept at 4 waves per SIMD.  The scene is read through a device copy (held by value in
 // kernel arguments its fields would occupy SGPRs and spill), and a scene whose only light is a
 // steerable EM takes the vertex that shares the EM lookups of escaped paths and light samples.
-// Unlike the fused kernel it does not stage the steerable tree in LDS: that needs one 1024-lane
-// workgroup per CU, whose queue-append barriers then hold all 16 wavefronts of the CU in step,
-// and measured no faster.  The simple sampler's marginal CDF is staged.
+// Each 256-lane workgroup stages the top levels of the steerable tree that fit in
+// PG3_WF_TREE_NODES nodes in LDS (the fused kernel's one 1024-lane workgroup per CU would stage
+// the whole 1280-node budget once, but its queue-append barriers then hold all 16 wavefronts of
+// the CU in step, which measured slower).  The simple sampler's marginal CDF is staged.
 // Register budget of the shading stage in waves per SIMD (its grid is the resident workgroups).
 // C2 frame: 2, 3, 4, 5, 6, 8 waves: 22.5, 21.3, 20.9, 24.9, 29.9, 40.7 ms (fewer waves hide less
 // load latency; more spill).
 #ifndef PG3_WF_SHADE_WAVES
 #define PG3_WF_SHADE_WAVES 4
+#endif
+// Steerable tree nodes staged in LDS per shading workgroup (48 B each; the 4 resident workgroups
+// of a CU share its 160 KiB); 0 reads the tree from the caches.  C2 frame: 0, 400, 800 nodes:
+// 20.4, 19.4, 19.4 ms.
+#ifndef PG3_WF_TREE_NODES
+#define PG3_WF_TREE_NODES 400
 #endif
 template <int kEmKind, bool kOneLight>
 __global__ void __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(PG3_WF_SHADE_WAVES)))
@@ -213,6 +220,13 @@ WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, 
         StageMarginal(S.em, margLds, spec.tree);
         __syncthreads();
     }
+#if PG3_WF_TREE_NODES > 0
+    if constexpr (kEmKind == kEmSteerable) {
+        __shared__ F4v treeLds[PG3_WF_TREE_NODES * 3];
+        spec.tree = StageTree<PG3_WF_TREE_NODES>(S.em, treeLds);
+        __syncthreads();
+    }
+#endif
     const uint32_t count = Q.counts[qi];
     const WfPaths Pi = Q.P[qi], Po = Q.P[qo];
     uint32_t segments = 0;
