@@ -25,7 +25,7 @@ BYTES_PER_SAMPLE, BYTES_PER_SEGMENT = 112, 304
 MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 8375358 + 6484874) * 1024)}
 # HBM bytes per frame of the wavefront pipeline on the default workload (all Wf* launches of a
 # frame, same calibration): profiles/wavefront_box_steerable_em_512px_64spp.md
-MEASURED_WF_TRAFFIC = int((2.0 * 17476832 + 18135104) * 1024)
+MEASURED_WF_TRAFFIC = int((2.0 * 17421712 + 18143837) * 1024)
 
 
 def parse():
