@@ -45,7 +45,8 @@ struct WfQueues {
     float4 *sB;      //   dir.xyz, destination of the contribution (below)
     float4 *sC;      //   contrib.xyz
     float4 *done;    // finished paths by sample position in the batch: L.xyz
-    uint32_t *counts;   // [0], [1]: live paths in P[0], P[1]; [2]: shadow queue; [3]: segments
+    uint32_t *counts;   // [0], [1]: live paths in P[0], P[1]; [2], [4]: shadow queue of odd / even
+                        // bounces (ShadowCounter); [3]: segments
 };
 
 // Destination of a shadow contribution: slot of the continuing path in the next live set, or
@@ -175,11 +176,13 @@ __global__ void __launch_bounds__(kWfBlock) WfGenerate(SceneView S, WfQueues Q, 
 // Also zeroes the counters selected by clearMask (bit k: counts[k]; never qi): the shading stage
 // that follows appends to them, and every earlier reader has finished (stream order), so no
 // launch of its own is needed for the clear.
-__global__ void __launch_bounds__(kWfBlock) WfIntersect(SceneView S, WfQueues Q, uint32_t qi, uint32_t clearMask) {
-    if (blockIdx.x == 0 && threadIdx.x < 4u && (clearMask >> threadIdx.x & 1u)) Q.counts[threadIdx.x] = 0u;
+// The body runs as workgroup `blk` of `nblk` (the merged launch below gives it part of its grid).
+__device__ __forceinline__ void IntersectStage(const SceneView &S, const WfQueues &Q, uint32_t qi, uint32_t clearMask,
+                                               uint32_t blk, uint32_t nblk) {
+    if (blk == 0 && threadIdx.x < 8u && (clearMask >> threadIdx.x & 1u)) Q.counts[threadIdx.x] = 0u;
     const uint32_t count = Q.counts[qi];
     const WfPaths P = Q.P[qi];
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
+    for (uint32_t j = blk * blockDim.x + threadIdx.x; j < count; j += nblk * blockDim.x) {
         const float4 a = P.A[j], b = P.B[j];
         Hit h;
         h.dist = 1e36f;
@@ -188,6 +191,14 @@ __global__ void __launch_bounds__(kWfBlock) WfIntersect(SceneView S, WfQueues Q,
         Q.hMat[j] = hit ? h.mat : -1;
     }
 }
+__global__ void __launch_bounds__(kWfBlock) WfIntersect(SceneView S, WfQueues Q, uint32_t qi, uint32_t clearMask) {
+    IntersectStage(S, Q, qi, clearMask, blockIdx.x, gridDim.x);
+}
+
+// Shadow queue counter of bounce b: odd and even bounces alternate between counts[2] and
+// counts[4], so that one launch can test bounce b's shadow rays while it intersects (and clears
+// the counters for) bounce b + 1.
+__host__ __device__ constexpr uint32_t ShadowCounter(uint32_t b) { return (b & 1u) ? 2u : 4u; }
 
 // Shading stage, specialised like the fused kernel (kEmKind: EM sampler or kEmAbsent; kOneLight)
 // and kept at 4 waves per SIMD.  The scene is read through a device copy (held by value in
@@ -251,7 +262,7 @@ WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, 
         }
         // workgroup-uniform appends (inactive tail lanes pass false)
         const bool p[2] = {cont, sr.valid};
-        uint32_t *const c[2] = {&Q.counts[qo], &Q.counts[2]};
+        uint32_t *const c[2] = {&Q.counts[qo], &Q.counts[ShadowCounter(bounce)]};
         uint32_t s[2];
         BlockAppend<kWfBlock / 64, 2>(p, c, s);
         uint32_t fin = 0;
@@ -274,12 +285,13 @@ WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, 
     if (__lane_id() == 0 && w) atomicAdd(&Q.counts[3], w);
 }
 
-// Occlusion of the light samples of one bounce; qo is the live set the continuing paths were
-// written to.
-__global__ void __launch_bounds__(kWfBlock) WfShadow(SceneView S, WfQueues Q, uint32_t qo) {
-    const uint32_t count = Q.counts[2];
+// Occlusion of the light samples of bounce b; qo is the live set the continuing paths were
+// written to.  Runs as workgroup `blk` of `nblk`.
+__device__ __forceinline__ void ShadowStage(const SceneView &S, const WfQueues &Q, uint32_t qo, uint32_t b,
+                                            uint32_t blk, uint32_t nblk) {
+    const uint32_t count = Q.counts[ShadowCounter(b)];
     float4 *const live = Q.P[qo].E;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
+    for (uint32_t j = blk * blockDim.x + threadIdx.x; j < count; j += nblk * blockDim.x) {
         const float4 a = Q.sA[j], b = Q.sB[j];
         if (!Occluded(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w)) {
             const uint32_t dst = AsU(b.w);
@@ -292,6 +304,17 @@ __global__ void __launch_bounds__(kWfBlock) WfShadow(SceneView S, WfQueues Q, ui
             *L = v;
         }
     }
+}
+__global__ void __launch_bounds__(kWfBlock) WfShadow(SceneView S, WfQueues Q, uint32_t qo, uint32_t b) {
+    ShadowStage(S, Q, qo, b, blockIdx.x, gridDim.x);
+}
+// Bounce b's shadow rays (the first shadowBlocks workgroups) and bounce b + 1's intersections
+// (the rest) in one launch: the two touch disjoint data (the radiance of the live paths and the
+// shadow queue; the rays and hit records of the live paths) and disjoint counters.
+__global__ void __launch_bounds__(kWfBlock) WfShadowIntersect(SceneView S, WfQueues Q, uint32_t qo, uint32_t b,
+                                                              uint32_t clearMask, uint32_t shadowBlocks) {
+    if (blockIdx.x < shadowBlocks) ShadowStage(S, Q, qo, b, blockIdx.x, shadowBlocks);
+    else IntersectStage(S, Q, qo, clearMask, blockIdx.x - shadowBlocks, gridDim.x - shadowBlocks);
 }
 
 // Adds the finished paths of samples [batchBase, batchBase + count) to the FP64 framebuffer.  Lane
@@ -333,7 +356,7 @@ __global__ void __launch_bounds__(kWfBlock) WfAccumulate(SceneView S, WfQueues Q
 
 // Zeroes the counters selected by `mask` (bit k: counts[k]).
 __global__ void WfClearCounts(uint32_t *counts, uint32_t mask) {
-    if (threadIdx.x < 4 && (mask >> threadIdx.x & 1u)) counts[threadIdx.x] = 0u;
+    if (threadIdx.x < 8 && (mask >> threadIdx.x & 1u)) counts[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -434,14 +457,21 @@ hipError_t LaunchWavefront(const SceneView &S, const SceneView *Sdev, uint32_t f
     const bool dense = (S.resX % 8u) == 0u && (S.resY % 8u) == 0u;
     for (uint64_t base = 0; base < total; base += cap) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(cap, total - base);
-        hipLaunchKernelGGL(WfClearCounts, dim3(1), dim3(64), 0, stream, Q.counts, 0xFu);
+        hipLaunchKernelGGL(WfClearCounts, dim3(1), dim3(64), 0, stream, Q.counts, 0xFFu);
         hipLaunchKernelGGL(WfGenerate, grid, block, 0, stream, S, Q, base, count, firstIter, dense);
+        hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, 0u, (1u << 1) | (1u << ShadowCounter(1)));
         for (uint32_t b = 1; b <= maxBounces; ++b) {
+            // bounce b: live set qi (already intersected) -> qo; the clear mask of bounce b + 1
+            // names the counters its shading appends to (qi's and its shadow queue's), whose
+            // readers (bounce b's shading, bounce b - 1's shadow rays) have finished
             const uint32_t qi = (b - 1) & 1u, qo = qi ^ 1u;
-            hipLaunchKernelGGL(WfIntersect, grid, block, 0, stream, S, Q, qi, (1u << qo) | 4u);
             hipLaunchKernelGGL(shade, dim3(W->shadeBlocks), block, 0, stream, Sdev, Q, qi, qo, b, base,
                                firstIter);
-            hipLaunchKernelGGL(WfShadow, grid, block, 0, stream, S, Q, qo);
+            if (b < maxBounces)
+                hipLaunchKernelGGL(WfShadowIntersect, dim3(2u * grid.x), block, 0, stream, S, Q, qo, b,
+                                   (1u << qi) | (1u << ShadowCounter(b + 1u)), grid.x);
+            else
+                hipLaunchKernelGGL(WfShadow, grid, block, 0, stream, S, Q, qo, b);
             if (S.maxPL == 0 && (b % 8u) == 0u && b < maxBounces) {
                 uint32_t live = 0;
                 e = hipMemcpyAsync(&live, Q.counts + qo, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
