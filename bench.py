@@ -45,9 +45,29 @@ def parse():
     return ap.parse_args()
 
 
+def roofline_legs(desc, samples, segments, kernel_s):
+    """The two algorithmic legs of one kernel (per launch): HBM bytes of the SoA queue model
+    (BYTES_PER_SAMPLE per camera sample + BYTES_PER_SEGMENT per path segment) and FP32 FLOP of the
+    brute-force intersection + shading model, each against its MI355X peak.  `bound` is the leg
+    with the larger fraction of its peak; `frac` = max of the two."""
+    ray_flop = TRI_FLOP * desc.num_triangles + SPH_FLOP * desc.num_spheres
+    flops = samples * ray_flop + segments * (2 * ray_flop + SHADE_FLOP)
+    bytes_alg = BYTES_PER_SAMPLE * samples + BYTES_PER_SEGMENT * segments
+    gbs = bytes_alg / kernel_s / 1e9
+    tflops = flops / kernel_s / 1e12
+    legs = {"hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": int(bytes_alg)},
+            "valu": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "flop_per_launch": int(flops),
+                     "flop_per_segment": 2 * ray_flop + SHADE_FLOP}}
+    bound = max(legs, key=lambda k: legs[k]["frac"])
+    return bound, legs
+
+
 def defaults_measure(ctx):
     """The reference's command-line defaults (-sb 4 -iic 16, Russian roulette) on the C1 and C2
-    scenes: first-vertex splitting with indirect clipping (the split path of the fused kernel)."""
+    scenes: first-vertex splitting with indirect clipping (the split path of the fused kernel),
+    with the roofline legs of the split kernel."""
     from pg3render_amd import SceneBuild
     out = {}
     for name, wl in (("box_area_light_256px_16spp", 1), ("box_steerable_em_512px_64spp", 2)):
@@ -60,9 +80,13 @@ def defaults_measure(ctx):
         n = sb.spp
         ms = [ctx.render_timed(i * n, n, 1) for i in range(3)]
         seg, smp = ctx.counters()
+        ext, shadow = ctx.ray_counters()
         t = sum(ms) / len(ms)
+        bound, legs = roofline_legs(sb.desc, smp / 3, seg / 3, t / 1e3)
         out[name] = {"ms_per_frame": round(t, 3), "msamples_per_s": round(smp / 3 / t / 1e3, 1),
-                     "msegments_per_s": round(seg / 3 / t / 1e3, 1)}
+                     "msegments_per_s": round(seg / 3 / t / 1e3, 1),
+                     "rays_per_s": round((ext + shadow) / 3 / t * 1e3, 1),
+                     "roofline": {"bound": bound, "frac": legs[bound]["frac"], "legs": legs}}
     return out
 
 
@@ -81,11 +105,13 @@ def wavefront_measure(ctx, steps=2):
         ctx.reset_counters()
         ms = [ctx.render_timed((i + 1) * n, n, 1) for i in range(steps)]
         seg, smp = ctx.counters()
+        ext, shadow = ctx.ray_counters()
     finally:
         ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
     t = sum(ms) / len(ms) / 1000.0
     qbytes = (BYTES_PER_SAMPLE * smp + BYTES_PER_SEGMENT * seg) / steps
     return {"ms_per_frame": round(t * 1000.0, 3), "msamples_per_s": round(smp / steps / t / 1e6, 1),
+            "rays_per_s": round((ext + shadow) / steps / t, 1),
             "queue_model_gbs": round(qbytes / t / 1e9, 1), "queue_model_frac": round(qbytes / t / 1e9 / HBM_PEAK_GBS, 4),
             "hbm_measured_gbs": round(MEASURED_WF_TRAFFIC / t / 1e9, 1),
             "hbm_measured_frac": round(MEASURED_WF_TRAFFIC / t / 1e9 / HBM_PEAK_GBS, 4)}
@@ -155,6 +181,13 @@ def batch_measure(ctx, tune_candidates):
         ctx.synchronize()
         return time.perf_counter() - t0
 
+    def scene_kernel_ms(n_iter):
+        """HIP-event time of each scene's launch rendering n_iter x its spp iterations."""
+        out = []
+        for s, n in zip(slots, spp):
+            out.append(ctx.render_timed(0, n * n_iter, s))
+        return out
+
     def run_iteration_pass():
         # the drop-in renderer as PG3Render's Render() loop drives it: RunIteration, one sample per
         # pixel per call (pg3render.cxx:84-128), each call one launch
@@ -168,6 +201,18 @@ def batch_measure(ctx, tune_candidates):
 
     plain_pass()                                   # warm-up (code objects, caches)
     plain_s = min(plain_pass() for _ in range(2))
+    # What batching can recover at most: a per-scene launch pays its ramp-up and drain (lanes idle
+    # while the last paths of the launch finish) once, so a launch of 2 x spp iterations takes
+    # t(2n) = 2 t_steady(n) + t_drain and t_steady(n) = t(2n) - t(n); the launch gaps are the
+    # wall time of the pass outside its kernels.  A batched pass that removed every gap and drain
+    # would take sum t_steady(n).
+    k1 = scene_kernel_ms(1)
+    k2 = scene_kernel_ms(2)
+    steady_ms = sum(max(b - a, 0.0) for a, b in zip(k1, k2))
+    ceiling = {"plain_wall_ms": round(plain_s * 1e3, 3), "plain_kernels_ms": round(sum(k1), 3),
+               "launch_gaps_ms": round(plain_s * 1e3 - sum(k1), 3),
+               "drain_ms": round(sum(2 * a - b for a, b in zip(k1, k2)), 3), "steady_ms": round(steady_ms, 3),
+               "max_speedup": round(plain_s * 1e3 / steady_ms, 3)}
     run_it_s = run_iteration_pass()
     batch = Batch(ctx, slots, spp)
     t0 = time.perf_counter()
@@ -190,7 +235,8 @@ def batch_measure(ctx, tune_candidates):
             "batched_msamples_per_s": round(total / batched_s / 1e6, 3),
             "speedup": round(plain_s / batched_s, 3), "speedup_vs_run_iteration": round(run_it_s / batched_s, 3),
             "batched_gpu_ms_per_pass": round(min(gpu_ms), 3),
-            "plain_ms_per_pass": round(plain_s * 1e3, 3), "autotuned_config": cfg, "autotune_s": round(tune_s, 3)}
+            "plain_ms_per_pass": round(plain_s * 1e3, 3), "autotuned_config": cfg, "autotune_s": round(tune_s, 3),
+            "ceiling": ceiling}
 
 
 def diff_measure(ctx):
@@ -272,38 +318,33 @@ def main():
     ctx.synchronize()
     wall = time.perf_counter() - t0
     segments, samples = ctx.counters()
+    ext_rays, shadow_rays = ctx.ray_counters()
 
     ms_per_step = wall * 1000.0 / args.steps
     total_samples = w * h * spp * args.steps
     msps = total_samples / wall / 1e6
-    # algorithmic work model (one path segment = one extension ray + its shadow ray)
-    ray_flop = TRI_FLOP * d.num_triangles + SPH_FLOP * d.num_spheres
-    flops = samples * ray_flop + segments * (2 * ray_flop + SHADE_FLOP)
-    bytes_alg = BYTES_PER_SAMPLE * samples + BYTES_PER_SEGMENT * segments
-    kernel_s = sum(kernel_ms) / 1000.0
-    tflops = flops / kernel_s / 1e12
-    gbs = bytes_alg / kernel_s / 1e9
-    rays_per_s = (samples + 2 * segments) / wall
-    traffic = MEASURED_TRAFFIC.get((args.workload, args.kernel))
+    # algorithmic work of one launch (one path segment = one extension ray + its shadow ray),
+    # over the dominant kernel's average HIP-event time on the library stream
     kernel_ms_avg = sum(kernel_ms) / len(kernel_ms)
-    roofline_hbm = ({"hbm_measured_gbs": round(traffic / (kernel_ms_avg / 1000.0) / 1e9, 1),
-                     "hbm_measured_frac": round(traffic / (kernel_ms_avg / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)}
-                    if traffic else {})
-    roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
-                "traffic": MEASURED_TRAFFIC.get((args.workload, args.kernel)),
-                "compute": "fp32 VALU (no matrix work on this path); peak = MI355X FP32 157.3 TFLOP/s",
-                "hbm_model_gbs": round(gbs, 1), "hbm_model_frac": round(gbs / HBM_PEAK_GBS, 4),
-                "kernel_ms_per_step": round(sum(kernel_ms) / len(kernel_ms), 3),
-                "flop_per_segment": 2 * ray_flop + SHADE_FLOP, "segments_per_step": segments // args.steps,
-                **roofline_hbm}
+    bound, legs = roofline_legs(d, samples / args.steps, segments / args.steps, kernel_ms_avg / 1000.0)
+    rays_per_s = (ext_rays + shadow_rays) / wall
+    traffic = MEASURED_TRAFFIC.get((args.workload, args.kernel))
+    measured = ({"hbm_measured_gbs": round(traffic / (kernel_ms_avg / 1000.0) / 1e9, 1),
+                 "hbm_measured_frac": round(traffic / (kernel_ms_avg / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)}
+                if traffic else {})
+    roofline = {"bound": bound, "achieved": legs[bound]["achieved"], "peak": legs[bound]["peak"],
+                "unit": legs[bound]["unit"], "frac": legs[bound]["frac"], "traffic": traffic,
+                "legs": legs, "kernel_ms_per_step": round(kernel_ms_avg, 3),
+                "segments_per_step": segments // args.steps, **measured}
     out = {"metric": "Msamples/s", "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "box + 1024x512 sky/sun EM (steerable, levels<=4), GGX dielectric + Lambert "
                                   "spheres, pt -sb 1 -iic 0 -maxpl 10", "resolution": [w, h], "spp": spp,
                       "kernel": args.kernel},
-           "rays_per_s": round(rays_per_s, 1), "roofline": roofline}
+           "rays_per_s": round(rays_per_s, 1),
+           "rays": {"extension_per_step": ext_rays // args.steps, "shadow_per_step": shadow_rays // args.steps},
+           "roofline": roofline}
     if not args.no_batch:
         out["batched"] = batch_measure(ctx, args.batch_tune)
     out["reference_defaults"] = defaults_measure(ctx)

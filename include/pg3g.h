@@ -113,7 +113,7 @@ typedef struct pg3g_render_params {
     uint32_t min_path_length;    /* -minpl (default 1) */
     uint32_t max_path_length;    /* -maxpl (0 = Russian roulette) */
     float    indirect_clipping;  /* -iic (0 = off) */
-    float    splitting_budget;   /* -sb (the GPU path supports 1) */
+    float    splitting_budget;   /* -sb (1..64; 1 with -iic 0 is the plain path tracer) */
     float    splitting_level;    /* -sl (default 1) */
     float    light_to_bsdf_ratio;/* -slbr (default 1) */
     uint64_t seed;               /* counter-based RNG key */
@@ -183,8 +183,13 @@ int pg3g_clear_framebuffer(pg3g_context *ctx, uint32_t slot);
 int pg3g_framebuffer_device_ptr(pg3g_context *ctx, uint32_t slot, double **fb_dev);
 int pg3g_synchronize(pg3g_context *ctx);
 
-/* Number of path segments (extension ray + its shadow ray) traced since the last reset. */
+/* Number of path segments (shaded path vertices: an extension ray and at most one shadow ray each)
+ * and camera samples traced since the last reset. */
 int pg3g_read_counters(pg3g_context *ctx, uint64_t *segments, uint64_t *camera_samples);
+/* Rays the path-tracing kernels actually traced since the last reset: extension rays (camera rays
+ * included; Scene::Intersect, src/scene.hxx:42-59) and shadow rays (Scene::Occluded, :61-74).
+ * The differentiable kernels (pg3g_render_diff, pg3g_adjoint) count segments and samples only. */
+int pg3g_read_ray_counters(pg3g_context *ctx, uint64_t *extension_rays, uint64_t *shadow_rays);
 int pg3g_reset_counters(pg3g_context *ctx);
 
 /* ---- function-level entry points (tier-1 parity; host arrays, n records) ----------------- */

@@ -62,16 +62,11 @@ template <int kEmKind, bool kOneLight>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 BatchKernel(const SceneView *__restrict__ views, const BatchScene *__restrict__ info, uint32_t numScenes,
             uint32_t numItems, uint32_t *__restrict__ work, unsigned long long *__restrict__ counters) {
-    const uint32_t lane = __lane_id();
     __shared__ double accLds[4][2 * 64 * 3];
-    uint32_t segments = 0, samples = 0;
+    RayTally tally;
     BatchItems<kEmKind, kOneLight> src{views, info, numScenes, numItems, work[1], work};
-    TraceItems<ChainPath<BatchItems<kEmKind, kOneLight>::kMergeEm>>(src, accLds[threadIdx.x >> 6], segments, samples);
-    const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
-    if (lane == 0 && counters) {
-        atomicAdd(counters + 0, (unsigned long long)segSum);
-        atomicAdd(counters + 1, (unsigned long long)smpSum);
-    }
+    TraceItems<ChainPath<BatchItems<kEmKind, kOneLight>::kMergeEm>>(src, accLds[threadIdx.x >> 6], tally);
+    FlushTally(tally, counters);
 }
 
 // Pass bookkeeping nodes of the captured graph (plain kernels: no memset nodes in graphs).

@@ -651,9 +651,9 @@ int pg3g_create(pg3g_context **ctx, int device) {
     c->device = device;
     PG3_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     PG3_HIP(hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device));
-    PG3_HIP(c->counters.Alloc(sizeof(unsigned long long) * 2));
+    PG3_HIP(c->counters.Alloc(sizeof(unsigned long long) * 4));
     PG3_HIP(c->work.Alloc(sizeof(uint32_t)));
-    PG3_HIP(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned long long) * 2, c->stream));
+    PG3_HIP(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned long long) * 4, c->stream));
     PG3_HIP(hipStreamSynchronize(c->stream));
     *ctx = c.release();
     return PG3G_OK;
@@ -788,6 +788,17 @@ int pg3g_read_counters(pg3g_context *ctx, uint64_t *segments, uint64_t *camera_s
     PG3_HIP(hipStreamSynchronize(ctx->stream));
     if (segments) *segments = h[0];
     if (camera_samples) *camera_samples = h[1];
+    return PG3G_OK;
+}
+
+int pg3g_read_ray_counters(pg3g_context *ctx, uint64_t *extension_rays, uint64_t *shadow_rays) {
+    if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    unsigned long long h[4] = {0, 0, 0, 0};
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(hipMemcpyAsync(h, ctx->counters.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    PG3_HIP(hipStreamSynchronize(ctx->stream));
+    if (extension_rays) *extension_rays = h[2];
+    if (shadow_rays) *shadow_rays = h[3];
     return PG3G_OK;
 }
 

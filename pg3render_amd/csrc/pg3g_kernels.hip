@@ -27,18 +27,18 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
     const uint32_t chunk = (uint32_t)(waveId / numTiles);
     const uint32_t x = (tile % tilesX) * 8u + (lane & 7u);
     const uint32_t y = (tile / tilesX) * 8u + (lane >> 3);
-    uint32_t segments = 0, samples = 0;
+    RayTally tally;
     if (chunk < numChunks && x < S.resX && y < S.resY) {
         const uint32_t pixel = y * S.resX + x;
         const uint32_t itBegin = firstIter + chunk * itersPerThread;
         const uint32_t itEnd = min(firstIter + nIter, itBegin + itersPerThread);
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
         for (uint32_t it = itBegin; it < itEnd; ++it) {
-            const F3 L = RadianceSample(S, pixel, it, segments);
+            const F3 L = RadianceSample(S, pixel, it, tally);
             acc0 += L.x;
             acc1 += L.y;
             acc2 += L.z;
-            ++samples;
+            ++tally.samples;
         }
         if (itEnd > itBegin) {
             atomicAdd(fb + (size_t)pixel * 3 + 0, acc0);
@@ -47,11 +47,7 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
         }
     }
     (void)npix;
-    const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
-    if (lane == 0 && counters) {
-        atomicAdd(counters + 0, (unsigned long long)segSum);
-        atomicAdd(counters + 1, (unsigned long long)smpSum);
-    }
+    FlushTally(tally, counters);
 }
 
 // Path tracer (-sb 1) specialised on the EM sampler and the light count, with path
@@ -108,19 +104,14 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
         __syncthreads();
     }
     const uint32_t tilesX = (S.resX + 7u) / 8u, numTiles = tilesX * ((S.resY + 7u) / 8u);
-    const uint32_t lane = __lane_id();
-    uint32_t segments = 0, samples = 0;
+    RayTally tally;
     SceneItems<kOneLight && kEmKind == kEmSteerable> src{Sp,        spec,     work,     fb,      firstIter,
                                                          nIter,     itersPerThread, numTiles * numChunks, numTiles,
                                                          tilesX,    0u,       0u};
     src.SetTail(numChunks);
-    if constexpr (kSplit) TraceItems<SplitPath>(src, accLds[threadIdx.x >> 6], segments, samples);
-    else TraceItems<ChainPath<kOneLight && kEmKind == kEmSteerable>>(src, accLds[threadIdx.x >> 6], segments, samples);
-    const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
-    if (lane == 0 && counters) {
-        atomicAdd(counters + 0, (unsigned long long)segSum);
-        atomicAdd(counters + 1, (unsigned long long)smpSum);
-    }
+    if constexpr (kSplit) TraceItems<SplitPath>(src, accLds[threadIdx.x >> 6], tally);
+    else TraceItems<ChainPath<kOneLight && kEmKind == kEmSteerable>>(src, accLds[threadIdx.x >> 6], tally);
+    FlushTally(tally, counters);
 }
 
 // Path tracer with splitting (-sb > 1) or indirect clipping (-iic > 0): one thread per (pixel,
@@ -139,7 +130,7 @@ __global__ void __launch_bounds__(256) RenderTreeKernel(SceneView S, uint32_t fi
     const uint32_t chunk = (uint32_t)(waveId / numTiles);
     const uint32_t x = (tile % tilesX) * 8u + (lane & 7u);
     const uint32_t y = (tile / tilesX) * 8u + (lane >> 3);
-    uint32_t segments = 0, samples = 0;
+    RayTally tally;
     if (chunk < numChunks && x < S.resX && y < S.resY) {
         const uint32_t pixel = y * S.resX + x;
         const uint32_t itBegin = firstIter + chunk * itersPerThread;
@@ -149,11 +140,11 @@ __global__ void __launch_bounds__(256) RenderTreeKernel(SceneView S, uint32_t fi
             RngKey key;
             F3 org, dir;
             CameraSample(S, pixel, it, key, org, dir);
-            const F3 L = RadiancePTTree(S, spec, key, org, dir, segments);
+            const F3 L = RadiancePTTree(S, spec, key, org, dir, tally);
             acc0 += L.x;
             acc1 += L.y;
             acc2 += L.z;
-            ++samples;
+            ++tally.samples;
         }
         if (itEnd > itBegin) {
             atomicAdd(fb + (size_t)pixel * 3 + 0, acc0);
@@ -161,11 +152,8 @@ __global__ void __launch_bounds__(256) RenderTreeKernel(SceneView S, uint32_t fi
             atomicAdd(fb + (size_t)pixel * 3 + 2, acc2);
         }
     }
-    const uint32_t segSum = WaveSum(segments), smpSum = WaveSum(samples);
-    if (lane == 0 && counters) {
-        atomicAdd(counters + 0, (unsigned long long)segSum);
-        atomicAdd(counters + 1, (unsigned long long)smpSum);
-    }
+    (void)lane;
+    FlushTally(tally, counters);
 }
 
 // ---------------------------------------------------------------------------------------------

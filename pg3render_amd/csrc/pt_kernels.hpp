@@ -22,6 +22,27 @@ __device__ __forceinline__ uint32_t WaveSum(uint32_t v) {
     return v;
 }
 
+// Work of one lane, summed per wavefront into a launch's counters (FlushTally): shaded path
+// vertices (segments), camera samples, and the rays actually traced -- extension rays
+// (Scene::Intersect, scene.hxx:42-59) and shadow rays (Scene::Occluded, :61-74).  The w* fields
+// are wavefront totals, only ever changed in wave-uniform code (they stay in scalar registers;
+// the persistent item loop counts samples and extension rays from its ballots).
+struct RayTally {
+    uint32_t segments = 0, samples = 0, ext = 0, shadow = 0;
+    uint32_t wSamples = 0, wExt = 0, wShadow = 0;
+};
+// counters: [0] segments, [1] camera samples, [2] extension rays, [3] shadow rays (64-bit).
+__device__ __forceinline__ void FlushTally(const RayTally &t, unsigned long long *counters) {
+    const uint32_t a = WaveSum(t.segments), b = WaveSum(t.samples) + t.wSamples, c = WaveSum(t.ext) + t.wExt,
+                   d = WaveSum(t.shadow) + t.wShadow;
+    if (__lane_id() == 0 && counters) {
+        if (a) atomicAdd(counters + 0, (unsigned long long)a);
+        if (b) atomicAdd(counters + 1, (unsigned long long)b);
+        if (c) atomicAdd(counters + 2, (unsigned long long)c);
+        if (d) atomicAdd(counters + 3, (unsigned long long)d);
+    }
+}
+
 // MISWeight2Power, path_tracer_base.hxx:522-541 (sample counts 1)
 PG3_DEV float MisPower(float p1, float p2) {
     const float q1 = p1 * p1, q2 = p2 * p2;
@@ -465,7 +486,7 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey
 }
 
 // One sample of PathTracer::EstimateIncomingRadiancePT with -sb 1, traced in one thread.
-PG3_DEV F3 RadiancePT(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
+PG3_DEV F3 RadiancePT(const SceneView &S, const RngKey &key, F3 org, F3 dir, RayTally &t) {
     const LightSpec spec = SceneLightSpec(S);
     PathState ps;
     InitPath(ps, org, dir);
@@ -473,22 +494,27 @@ PG3_DEV F3 RadiancePT(const SceneView &S, const RngKey &key, F3 org, F3 dir, uin
         Hit h;
         h.dist = 1e36f;
         const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
+        ++t.ext;
         ShadowReq sr;
-        const bool cont = ShadeVertex(S, spec, key, ps, hit, h, sr, segments);
-        if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
+        const bool cont = ShadeVertex(S, spec, key, ps, hit, h, sr, t.segments);
+        if (sr.valid) {
+            ++t.shadow;
+            if (!Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
+        }
         if (!cont) break;
     }
     return ps.L;
 }
 
 // PathTracer::EstimateIncomingRadiancePTNaive, path_tracer.hxx:45-165
-PG3_DEV F3 RadiancePTNaive(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
+PG3_DEV F3 RadiancePTNaive(const SceneView &S, const RngKey &key, F3 org, F3 dir, RayTally &t) {
     const LightSpec spec = SceneLightSpec(S);
     F3 out = mk(0.f), thr = mk(1.0f);
     float tmin = 0.f;
     for (uint32_t len = 1;; ++len) {
         Hit h;
         h.dist = 1e36f;
+        ++t.ext;
         if (!Intersect(S, org, dir, tmin, h)) {
             if (len >= S.minPL && S.bgLight >= 0) out += BgRadiance(S, spec, dir) * thr;
             return out;
@@ -504,7 +530,7 @@ PG3_DEV F3 RadiancePTNaive(const SceneView &S, const RngKey &key, F3 org, F3 dir
         if (ReflectanceZero(mat)) return out;
         if (S.maxPL > 0 && len >= S.maxPL) return out;
         if (len >= 1200u) return out;
-        ++segments;
+        ++t.segments;
         VertexRng vr;
         RngVertex(key, len, vr);
         float rr = 1.0f;
@@ -524,12 +550,13 @@ PG3_DEV F3 RadiancePTNaive(const SceneView &S, const RngKey &key, F3 org, F3 dir
 }
 
 // DirectIllumination::EstimateIncomingRadiance, direct_illumination.hxx:15-156
-PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir, uint32_t &segments) {
+PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir, RayTally &t) {
     const LightSpec spec = SceneLightSpec(S);
     Hit h;
     h.dist = 1e36f;
+    ++t.ext;
     if (!Intersect(S, org, dir, 0.f, h)) return (S.bgLight >= 0) ? BgRadiance(S, spec, dir) : mk(0.f);
-    ++segments;
+    ++t.segments;
     const F3 sp = org + dir * h.dist;
     const Frame fr = FrameFromZ(h.n);
     const F3 wol = ToLocal(fr, -dir);
@@ -549,7 +576,7 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
             const float u0 = RngGet(key, vtx, i, kDimLight0), u1 = RngGet(key, vtx, i, kDimLight1),
                         u2 = RngGet(key, vtx, i, kDimLight2);
             if (SampleIllumination(S, spec, i, sp, fr, sides, u0, u1, u2, ls) && MaxC(ls.sample) > 0.f &&
-                !Occluded(S, sp, ls.wig, ls.dist)) {                                    // :315-348
+                (++t.shadow, !Occluded(S, sp, ls.wig, ls.dist))) {                       // :315-348
                 BsdfRec r;
                 EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
                 Lo += (ls.pdfW != kInf) ? ls.sample * r.att / (ls.pdfW * 1.f) : ls.sample * r.att / 1.f;
@@ -562,7 +589,7 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
         if (li >= 0 && SampleIllumination(S, spec, (uint32_t)li, sp, fr, sides, vr.u[kDimLight0], vr.u[kDimLight1],
                                           vr.u[kDimLight2], ls)) {
             ls.prob = prob;
-            if (MaxC(ls.sample) > 0.f && !Occluded(S, sp, ls.wig, ls.dist)) {
+            if (MaxC(ls.sample) > 0.f && (++t.shadow, !Occluded(S, sp, ls.wig, ls.dist))) {
                 BsdfRec r;
                 EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
                 if (alg == 4) {
@@ -587,6 +614,7 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
             F3 Li = mk(0.f);
             float lpdf = 0.f;
             int32_t lid = -1;
+            ++t.ext;
             if (Intersect(S, sp, wig, EpsRayCos(r.wil.z), h2)) {
                 const int32_t l2 = LoadGlobal(S.mats + h2.mat).lightId;
                 if (l2 >= 0) {
@@ -665,7 +693,7 @@ struct TreeFrame {              // one vertex of the recursion, in its BSDF-samp
 };
 
 PG3_DEV F3 RadiancePTTree(const SceneView &S, const LightSpec &spec, const RngKey &key0, F3 org, F3 dir,
-                          uint32_t &segments) {
+                          RayTally &t) {
     TreeFrame stk[kTreeDepth];
     int32_t top = -1;
     // pending call EstimateIncomingRadiancePT(ray, len, computeRefl, budget) on branch `cBranch`
@@ -690,6 +718,7 @@ PG3_DEV F3 RadiancePTTree(const SceneView &S, const LightSpec &spec, const RngKe
             mode = kReturn;
             Hit h;
             h.dist = 1e36f;
+            ++t.ext;
             if (Intersect(S, cOrg, cDir, cTmin, h)) {
                 const F3 sp = cOrg + cDir * h.dist;
                 const Frame fr = FrameFromZ(h.n);
@@ -708,7 +737,7 @@ PG3_DEV F3 RadiancePTTree(const SceneView &S, const LightSpec &spec, const RngKe
                 const bool stop = !cRefl || ReflectanceZero(mat) || (S.maxPL > 0 && cLen >= S.maxPL) ||
                                   cLen >= 1200u || top + 1 >= (int32_t)kTreeDepth;
                 if (!stop) {
-                    ++segments;
+                    ++t.segments;
                     TreeFrame &F = stk[top + 1];
                     // ComputeSplittingCounts, :432-454
                     const uint32_t nB = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * cBudget);
@@ -741,7 +770,7 @@ PG3_DEV F3 RadiancePTTree(const SceneView &S, const LightSpec &spec, const RngKe
                                 SampleIllumination(S, spec, (uint32_t)li, sp, fr, mat.sides, RngGet(key, cLen, l, kDimLight0),
                                                    RngGet(key, cLen, l, kDimLight1), RngGet(key, cLen, l, kDimLight2), ls)) {
                                 ls.prob = prob;                                       // :350-425
-                                if (MaxC(ls.sample) > 0.f && !Occluded(S, sp, ls.wig, ls.dist)) {
+                                if (MaxC(ls.sample) > 0.f && (++t.shadow, !Occluded(S, sp, ls.wig, ls.dist))) {
                                     BsdfRec r;
                                     EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
                                     if (ls.pdfW != kInf) {
@@ -927,10 +956,12 @@ struct TileItem {
 // nothing: a sample's path depends only on its (pixel, iteration) RNG stream.
 //   src.Next(TileItem&) -> bool   fetches the next item (false: none left; wave-uniform)
 //   src.View(scene) -> const SceneView *, src.Spec(scene) -> LightSpec
-//   Path: a lane's sample; Start(S, pixel, iteration), Step(S, spec, segments) -> true when the
-//   sample is done, Result() -> its radiance
+//   Path: a lane's sample; Start(S, pixel, iteration), Step(S, spec, tally, shadowed) -> true
+//   when the sample is done, Result() -> its radiance.  Step traces exactly one extension ray
+//   (counted here) and counts its segments; a path with at most one shadow ray per step sets
+//   `shadowed` when it traced one (counted here), others count their own in the tally.
 template <class Path, class Src>
-PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &samples) {
+PG3_DEV void TraceItems(Src &src, double *acc, RayTally &tally) {
     const uint32_t lane = __lane_id();
 #pragma unroll
     for (uint32_t k = 0; k < 6; ++k) acc[lane + 64u * k] = 0.0;
@@ -987,21 +1018,24 @@ PG3_DEV void TraceItems(Src &src, double *acc, uint32_t &segments, uint32_t &sam
             if (cur) penB += started;
             else penA += started;
         }
-        if (__ballot(busy) != 0) {
-            bool done = false;
+        const uint64_t stepping = __ballot(busy);
+        if (stepping != 0) {
+            tally.wExt += (uint32_t)__popcll(stepping);    // one Intersect per Step
+            bool done = false, shadowed = false;
             if (busy) {
-                if (path.Step(S, spec, segments)) {
+                if (path.Step(S, spec, tally, shadowed)) {
                     const F3 L = path.Result();
                     double *a = acc + slot * 192u + pix * 3u;
                     atomicAdd(a + 0, (double)L.x);
                     atomicAdd(a + 1, (double)L.y);
                     atomicAdd(a + 2, (double)L.z);
-                    ++samples;
                     busy = false;
                     done = true;
                 }
             }
             const uint64_t fin = __ballot(done), finB = __ballot(done && slot != 0);
+            tally.wSamples += (uint32_t)__popcll(fin);
+            tally.wShadow += (uint32_t)__popcll(__ballot(shadowed));
             penB -= (uint32_t)__popcll(finB);
             penA -= (uint32_t)__popcll(fin & ~finB);
         }
@@ -1032,12 +1066,13 @@ struct ChainPath {
         CameraSample(S, pixel, iteration, key, org, dir);
         InitPath(ps, org, dir);
     }
-    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, uint32_t &segments) {
+    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &t, bool &shadowed) {
         Hit h;
         h.dist = 1e36f;
         const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
         ShadowReq sr;
-        const bool cont = ShadeVertex<kMergeEm>(S, spec, key, ps, hit, h, sr, segments);
+        const bool cont = ShadeVertex<kMergeEm>(S, spec, key, ps, hit, h, sr, t.segments);
+        shadowed = sr.valid;
         if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
         return !cont;
     }
@@ -1113,7 +1148,7 @@ struct SplitPath {
     // samples (MIS counts).
     PG3_DEV F3 LightSamples(const SceneView &S, const LightSpec &spec, const RngKey &k, uint32_t vlen, F3 sp,
                             const Frame &fr, F3 wol, const DevMaterial &mat, uint32_t nLv, uint32_t nBv,
-                            const LightCache &lc) const {
+                            const LightCache &lc, uint32_t &shadows) const {
         F3 acc = mk(0.f);
         if ((vlen + 1) < S.minPL) return acc;
         for (uint32_t l = 0; l < nLv; ++l) {
@@ -1123,7 +1158,7 @@ struct SplitPath {
             if (li >= 0 && SampleIllumination(S, spec, (uint32_t)li, sp, fr, mat.sides, RngGet(k, vlen, l, kDimLight0),
                                               RngGet(k, vlen, l, kDimLight1), RngGet(k, vlen, l, kDimLight2), ls)) {
                 ls.prob = prob;
-                if (MaxC(ls.sample) > 0.f && !Occluded(S, sp, ls.wig, ls.dist)) {
+                if (MaxC(ls.sample) > 0.f && (++shadows, !Occluded(S, sp, ls.wig, ls.dist))) {
                     BsdfRec r;
                     EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
                     if (ls.pdfW != kInf) {
@@ -1219,7 +1254,7 @@ struct SplitPath {
         ++s;
         return true;
     }
-    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, uint32_t &segments) {
+    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &t, bool &) {
         const RngKey k = Branch(branch);
         Hit h;
         h.dist = 1e36f;
@@ -1270,7 +1305,7 @@ struct SplitPath {
             FoldChain(S);
             wantRoot = true;
         } else {
-            if (target < 0) ++segments;
+            if (target < 0) ++t.segments;
             if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
             const bool first = (len == 1);
             if (first) {                                                              // :240-250
@@ -1278,7 +1313,7 @@ struct SplitPath {
                 nL = (uint32_t)lroundf(fmaxf(S.lightRatio * nB, 1.f));
             }
             const uint32_t nLv = first ? nL : (uint32_t)lroundf(fmaxf(S.lightRatio * 1u, 1.f));
-            const F3 A = LightSamples(S, spec, k, len, sp, fr, wol, mat, nLv, first ? nB : 1u, lc);
+            const F3 A = LightSamples(S, spec, k, len, sp, fr, wol, mat, nLv, first ? nB : 1u, lc, t.shadow);
             rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
             if (first) {
                 reflRoot = A;
@@ -1397,13 +1432,13 @@ struct SceneItems {
     PG3_DEV LightSpec Spec(uint32_t) const { return spec; }
 };
 
-PG3_DEV F3 RadianceSample(const SceneView &S, uint32_t pixel, uint32_t iteration, uint32_t &segments) {
+PG3_DEV F3 RadianceSample(const SceneView &S, uint32_t pixel, uint32_t iteration, RayTally &t) {
     RngKey key;
     F3 org, dir;
     CameraSample(S, pixel, iteration, key, org, dir);
-    if (S.algorithm == 7) return RadiancePT(S, key, org, dir, segments);
-    if (S.algorithm == 6) return RadiancePTNaive(S, key, org, dir, segments);
-    return RadianceDirect(S, key, org, dir, segments);
+    if (S.algorithm == 7) return RadiancePT(S, key, org, dir, t);
+    if (S.algorithm == 6) return RadiancePTNaive(S, key, org, dir, t);
+    return RadianceDirect(S, key, org, dir, t);
 }
 
 }  // namespace pg3

@@ -46,7 +46,8 @@ struct WfQueues {
     float4 *sC;      //   contrib.xyz
     float4 *done;    // finished paths by sample position in the batch: L.xyz
     uint32_t *counts;   // [0], [1]: live paths in P[0], P[1]; [2], [4]: shadow queue of odd / even
-                        // bounces (ShadowCounter); [3]: segments
+                        // bounces (ShadowCounter); [3]: segments; [5], [6]: extension and shadow
+                        // rays traced (the intersected live paths, the shadow requests)
 };
 
 // Destination of a shadow contribution: slot of the continuing path in the next live set, or
@@ -181,6 +182,7 @@ __device__ __forceinline__ void IntersectStage(const SceneView &S, const WfQueue
                                                uint32_t blk, uint32_t nblk) {
     if (blk == 0 && threadIdx.x < 8u && (clearMask >> threadIdx.x & 1u)) Q.counts[threadIdx.x] = 0u;
     const uint32_t count = Q.counts[qi];
+    if (blk == 0 && threadIdx.x == 0 && count) atomicAdd(&Q.counts[5], count);   // extension rays
     const WfPaths P = Q.P[qi];
     for (uint32_t j = blk * blockDim.x + threadIdx.x; j < count; j += nblk * blockDim.x) {
         const float4 a = P.A[j], b = P.B[j];
@@ -279,9 +281,7 @@ WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, 
             Q.sC[s[1]] = make_float4(sr.contrib.x, sr.contrib.y, sr.contrib.z, 0.f);
         }
     }
-    uint32_t w = segments;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+    const uint32_t w = WaveSum(segments);
     if (__lane_id() == 0 && w) atomicAdd(&Q.counts[3], w);
 }
 
@@ -290,6 +290,7 @@ WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, 
 __device__ __forceinline__ void ShadowStage(const SceneView &S, const WfQueues &Q, uint32_t qo, uint32_t b,
                                             uint32_t blk, uint32_t nblk) {
     const uint32_t count = Q.counts[ShadowCounter(b)];
+    if (blk == 0 && threadIdx.x == 0 && count) atomicAdd(&Q.counts[6], count);   // shadow rays
     float4 *const live = Q.P[qo].E;
     for (uint32_t j = blk * blockDim.x + threadIdx.x; j < count; j += nblk * blockDim.x) {
         const float4 a = Q.sA[j], b = Q.sB[j];
@@ -351,6 +352,8 @@ __global__ void __launch_bounds__(kWfBlock) WfAccumulate(SceneView S, WfQueues Q
     if (__lane_id() == 0 && w) atomicAdd(counters + 1, (unsigned long long)w);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(counters + 0, (unsigned long long)Q.counts[3]);
+        atomicAdd(counters + 2, (unsigned long long)Q.counts[5]);
+        atomicAdd(counters + 3, (unsigned long long)Q.counts[6]);
     }
 }
 

@@ -214,6 +214,12 @@ class Context:
         check(self._lib.pg3g_read_counters(self.ptr, C.byref(s), C.byref(c)), "pg3g_read_counters")
         return int(s.value), int(c.value)
 
+    def ray_counters(self):
+        """(extension rays, shadow rays) traced since the last reset_counters()."""
+        e, sh = C.c_uint64(), C.c_uint64()
+        check(self._lib.pg3g_read_ray_counters(self.ptr, C.byref(e), C.byref(sh)), "pg3g_read_ray_counters")
+        return int(e.value), int(sh.value)
+
     def reset_counters(self):
         check(self._lib.pg3g_reset_counters(self.ptr), "pg3g_reset_counters")
 

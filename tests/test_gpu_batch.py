@@ -63,6 +63,60 @@ def test_batch_equals_per_scene_renders(ctx):
         assert_same(got[i], want[i], f"scene {i}")
 
 
+def full_batch_scenes():
+    """The 64-scene batch workload at its own sizes (sides 64..512, spp 4/16/64/256)."""
+    return [SceneBuild.workload(3, seed=2024, variant=v) for v in range(64)]
+
+
+def oracle_band_parity(sb, img, rows=8):
+    """The batched image of one scene against the CPU oracle on a band of `rows` image rows at the
+    scene's full spp (RunIteration x spp, path_tracer_base.hxx:57-90): the same RNG streams, so
+    pixels agree to FP32 rounding except where rounding sends a path another way."""
+    from oracle import oracle as O
+    w, h = sb.resolution
+    r0 = h // 2 - rows // 2
+    ref, _ = O.OracleScene(sb).render(0, sb.spp, threads=0, pixel_begin=r0 * w, pixel_end=(r0 + rows) * w)
+    g, o = img[r0:r0 + rows], ref[r0:r0 + rows]
+    lum_g, lum_o = g.sum(-1), o.sum(-1)
+    diff = np.abs(lum_g - lum_o) / np.maximum(np.abs(lum_o), 1e-3 * max(1e-6, lum_o.mean()))
+    frac = float(np.mean(diff > 1e-3))
+    band_rel = abs(lum_g.sum() - lum_o.sum()) / lum_o.sum()
+    print(f"  {w}x{h} x {sb.spp} spp, rows {r0}..{r0 + rows - 1}: pixels off by >1e-3 {frac:.4f}, "
+          f"max {diff.max():.2e}, band sum rel {band_rel:.2e}")
+    assert frac < 0.03, frac
+    assert band_rel < 5e-3, band_rel
+
+
+def test_full_64_scene_batch_equals_per_scene_and_oracle(ctx):
+    """The whole 64-scene workload (784.5 M samples) in one batched pass: every scene equals its
+    per-scene render (pg3g_render_slot of the same iterations), and four scenes -- the largest
+    (507^2 x 256 spp), the smallest, and one each of spp 4 and 64 -- equal the CPU oracle on a band
+    of rows at full spp."""
+    scenes = full_batch_scenes()
+    slots = list(range(1, 65))
+    for s, sb in zip(slots, scenes):
+        ctx.upload(sb, s)
+    batch = Batch(ctx, slots, [sb.spp for sb in scenes])
+    batch.set_config(iters_per_item=8, waves_per_cu=16, use_graph=True)
+    batch.render(1)
+    got = [ctx.framebuffer(s, sb.resolution) for s, sb in zip(slots, scenes)]
+    batch.close()
+    want = plain_images(ctx, scenes, 100, passes=1)
+    for i in range(len(scenes)):
+        assert_same(got[i], want[i], f"scene {i}")
+    work = [sb.resolution[0] * sb.resolution[1] * sb.spp for sb in scenes]
+    picks = [int(np.argmax(work)), int(np.argmin(work))]
+    for spp in (4, 64):
+        cand = [i for i, sb in enumerate(scenes) if sb.spp == spp and i not in picks]
+        picks.append(max(cand, key=lambda i: work[i]))
+    assert scenes[picks[0]].resolution == (507, 507) and scenes[picks[0]].spp == 256
+    for i in picks:
+        img, it = got[i]
+        assert it == scenes[i].spp
+        print(f"scene {i}:")
+        oracle_band_parity(scenes[i], img)
+
+
 def test_batch_graph_and_direct_launch_agree(ctx):
     scenes = c3_scenes(3)
     for i, sb in enumerate(scenes):
