@@ -342,6 +342,22 @@ int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
 int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
                  const float *dloss_dimage, double *grad_scalars, double *grad_em, float *elapsed_ms);
 
+/* Allocation-free variant of pg3g_adjoint.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight optimisation loops,
+ * harnesses that forbid hipMalloc.  It calls no hipMalloc / hipFree and does not synchronise:
+ * asynchronous on the context stream (pg3g_synchronize waits for it).  Inputs and outputs are
+ * DEVICE pointers: dloss_dimage_dev (res_x*res_y*3 floats, dLoss/dImage with image = sum / n_iter),
+ * grad_scalars_dev (num_scalars doubles, may be null) and grad_em_dev (w*h*3 doubles, may be null).
+ * The gradients accumulate (FP64) in one fixed device-side scratch array of the library, 769 KiB per
+ * GPU held until the process ends, which limits the EM to PG3G_NOALLOC_MAX_EM_CHANNELS FP64
+ * channels (256x128 RGB); a larger EM is rejected with PG3G_ERR_UNSUPPORTED.  Every call on one GPU
+ * shares that GPU's scratch array: calls that overlap on one GPU race (serialise them, e.g. on one
+ * stream).  It reuses the per-sample radiance of the last pg3g_render_diff of the same iterations
+ * when there is one, else replays each path twice.  The gradients equal pg3g_adjoint's. */
+#define PG3G_NOALLOC_MAX_EM_CHANNELS (256 * 128 * 3)
+int pg3g_adjoint_noalloc(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
+                         const float *dloss_dimage_dev, double *grad_scalars_dev, double *grad_em_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,7 @@ struct pg3g_context {
     WavefrontContext *wavefront = nullptr;
     uint64_t sceneEpoch = 0;   // bumped by every upload / parameter change (invalidates batches)
     int numCUs = 0;
+    bool noAllocReady = false; // the scratch array of pg3g_adjoint_noalloc was found on this device
 };
 
 struct pg3g_batch {
@@ -653,6 +654,9 @@ int pg3g_create(pg3g_context **ctx, int device) {
     PG3_HIP(hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device));
     PG3_HIP(c->counters.Alloc(sizeof(unsigned long long) * 4));
     PG3_HIP(c->work.Alloc(sizeof(uint32_t)));
+    // device address of the allocation-free adjoint's scratch array on this device (once per
+    // device); a failure only disables pg3g_adjoint_noalloc
+    c->noAllocReady = (InitNoAllocScratch(device) == hipSuccess);
     PG3_HIP(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned long long) * 4, c->stream));
     PG3_HIP(hipStreamSynchronize(c->stream));
     *ctx = c.release();
@@ -1427,6 +1431,36 @@ int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t
         if (emN == 0) return Fail(PG3G_ERR_INVALID_ARG, "the scene has no EM image");
         PG3_HIP(hipMemcpy(grad_em, S->gradEm.p, sizeof(double) * emN, hipMemcpyDeviceToHost));
     }
+    return PG3G_OK;
+}
+
+int pg3g_adjoint_noalloc(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
+                         const float *dloss_dimage_dev, double *grad_scalars_dev, double *grad_em_dev) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (!dloss_dimage_dev) return Fail(PG3G_ERR_INVALID_ARG, "dloss_dimage_dev is null");
+    if (n_iter == 0) return Fail(PG3G_ERR_INVALID_ARG, "n_iter must be >= 1");
+    if ((uint64_t)first_iter + n_iter > 0xFFFFFFFFull) return Fail(PG3G_ERR_INVALID_ARG, "iteration range overflows");
+    const SceneView &v = S->view;
+    const uint64_t emN = (uint64_t)v.em.w * v.em.h * 3u;
+    if (emN > kNoAllocMaxEmChannels)
+        return Fail(PG3G_ERR_UNSUPPORTED, "pg3g_adjoint_noalloc holds EM gradients of at most 256x128 RGB texels "
+                                          "(PG3G_NOALLOC_MAX_EM_CHANNELS); use pg3g_adjoint");
+    if (grad_em_dev && emN == 0) return Fail(PG3G_ERR_INVALID_ARG, "the scene has no EM image");
+    if (!ctx->noAllocReady) return Fail(PG3G_ERR_HIP, "the scratch array of pg3g_adjoint_noalloc is not available");
+    // scalars with no gradient: GGX alpha outside the clamp range [0.001, 1] (materials.hxx:805)
+    uint64_t zeroMask = 0;
+    for (uint32_t m : S->diffMats)
+        if (S->evalHostMats[m].type == PG3G_MAT_GGX_CONDUCTOR) {
+            const float a = S->evalHostMats[m].alpha;
+            if (a < 0.001f || a > 1.0f) zeroMask |= 1ull << (S->gradBaseHost[m] + 3);
+        }
+    const bool stored = S->sampleValid && S->sampleFirst == first_iter && S->sampleCount == n_iter &&
+                        S->sampleEpoch == S->epoch;
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(LaunchAdjointNoAlloc(ctx->device, v, S->viewDev.as<SceneView>(), first_iter, n_iter, dloss_dimage_dev,
+                                 S->numScalars, zeroMask, stored ? S->sampleL.as<float>() : nullptr, grad_scalars_dev,
+                                 grad_em_dev, ctx->numCUs, ctx->counters.as<unsigned long long>(), ctx->stream));
     return PG3G_OK;
 }
 

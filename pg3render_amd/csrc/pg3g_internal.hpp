@@ -58,6 +58,19 @@ hipError_t LaunchDiff(const SceneView &S, const SceneView *Sdev, uint32_t firstI
                       const float *adjImg, double *gradEm, double *gradScal, uint32_t numScal, float *sampleL,
                       uint32_t *work, int numCUs, unsigned long long *counters, hipStream_t stream);
 
+// Allocation-free adjoint (noalloc.hip).  kNoAllocMaxEmChannels: FP64 EM-gradient channels the
+// fixed scratch array holds (an RGB EM of up to 256 x 128 texels).  InitNoAllocScratch looks up
+// the scratch array's address on `device` (the current device) once; LaunchAdjointNoAlloc clears
+// the scratch, runs the adjoint of LaunchDiff into it and copies the gradients to outScal (numScal
+// doubles, zeroMask bits set to 0) and outEm (S.em.w * S.em.h * 3 doubles); either may be null.
+// sampleL (may be null): per-sample radiance of a differentiable forward of the same iterations.
+constexpr uint32_t kNoAllocMaxEmChannels = 256u * 128u * 3u;
+hipError_t InitNoAllocScratch(int device);
+hipError_t LaunchAdjointNoAlloc(int device, const SceneView &S, const SceneView *Sdev, uint32_t firstIter,
+                                uint32_t nIter, const float *adjImg, uint32_t numScal, uint64_t zeroMask,
+                                const float *sampleL, double *outScal, double *outEm, int numCUs,
+                                unsigned long long *counters, hipStream_t stream);
+
 hipError_t LaunchBsdfEval(const DevMaterial &m, uint32_t n, const float *wil, const float *wol, float *f, float *pdf,
                           float *comp, hipStream_t s);
 hipError_t LaunchBsdfSample(const DevMaterial &m, uint32_t n, const float *wol, const float *u, float *wil, float *f,
