@@ -3,7 +3,7 @@
 // scene per process (pg3render.cxx:54-174, one AbstractRenderer per thread); here every scene is
 // a set of work items (one 8x8 pixel tile x a run of iterations) that resident wavefronts take
 // from a device counter until none is left.  Each item is traced exactly as the per-scene kernel
-// traces it (TraceItems: same samples, same RNG streams), so a batched pass and per-scene renders
+// traces it (TraceTasks: same samples, same RNG streams), so a batched pass and per-scene renders
 // of the same iterations produce the same per-pixel sums.
 #include <hip/hip_runtime.h>
 
@@ -62,10 +62,9 @@ template <int kEmKind, bool kOneLight>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 BatchKernel(const SceneView *__restrict__ views, const BatchScene *__restrict__ info, uint32_t numScenes,
             uint32_t numItems, uint32_t *__restrict__ work, unsigned long long *__restrict__ counters) {
-    __shared__ double accLds[4][2 * 64 * 3];
     RayTally tally;
     BatchItems<kEmKind, kOneLight> src{views, info, numScenes, numItems, work[1], work};
-    TraceItems<ChainPath<BatchItems<kEmKind, kOneLight>::kMergeEm>>(src, accLds[threadIdx.x >> 6], tally);
+    TraceTasks<ChainPath<BatchItems<kEmKind, kOneLight>::kMergeEm>, true>(src, tally);
     FlushTally(tally, counters);
 }
 

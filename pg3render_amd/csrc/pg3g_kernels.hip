@@ -62,14 +62,17 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
 #define PG3_PT_WAVES_PER_EU 4
 #endif
 // Persistent: each CU holds 4 x PTWavesPerEU wavefronts for the whole launch, and every wavefront takes work
-// items (8x8 tile x iteration chunk) from a device counter until none is left, so a wavefront
-// that finishes early starts the next item instead of waiting for its workgroup.  With the
-// steerable sampler the workgroup is 1024 lanes (one per CU) and stages the inner levels of the
-// EM's triangle tree in LDS once (kTreeLdsNodes nodes, 60 KiB): the tree descent of every light
-// sample then gathers from LDS instead of from the L2.  With the simple sampler the marginal CDF
+// items (8x8 tile x iteration chunk) from a device counter until none is left (TraceTasks: a lane
+// whose task ends adds its radiance to the framebuffer and takes the next task at once).  With the
+// steerable sampler one workgroup per CU holds all its resident wavefronts (1024 lanes at 4 waves
+// per SIMD) and stages the inner levels of the EM's triangle tree in LDS once (kTreeLdsNodes
+// nodes, 60 KiB): the tree descent of every light sample then gathers from LDS instead of from the
+// L2.  With the simple sampler the marginal CDF
 // is staged instead (up to kMargLdsMax entries, an EM of up to 4095 rows).
+// The split path keeps more state per lane: 3 waves per SIMD (168 VGPRs) measured faster than 4
+// (C2 frame at the reference's defaults: 62.4 against 64.7 ms; 2 waves: 90 ms).
 #ifndef PG3_SPLIT_WAVES_PER_EU
-#define PG3_SPLIT_WAVES_PER_EU 4
+#define PG3_SPLIT_WAVES_PER_EU 3
 #endif
 // Waves per SIMD of the chain path and of the split path (the split path keeps more state)
 template <bool kSplit>
@@ -92,7 +95,6 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     // its ~450 bytes of fields spill into VGPR lanes of this register-heavy kernel.
     const SceneView &S = *Sp;
     LightSpec spec{kEmKind, kOneLight ? 1u : S.numLights};
-    __shared__ double accLds[PTBlock<kEmKind, kSplit>() / 64u][2 * 64 * 3];
     if constexpr (kEmKind == kEmSteerable) {
         __shared__ F4v treeLds[kTreeLdsNodes * 3];
         spec.tree = StageTree(S.em, treeLds);
@@ -109,8 +111,8 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
                                                          nIter,     itersPerThread, numTiles * numChunks, numTiles,
                                                          tilesX,    0u,       0u};
     src.SetTail(numChunks);
-    if constexpr (kSplit) TraceItems<SplitPath>(src, accLds[threadIdx.x >> 6], tally);
-    else TraceItems<ChainPath<kOneLight && kEmKind == kEmSteerable>>(src, accLds[threadIdx.x >> 6], tally);
+    if constexpr (kSplit) TraceTasks<SplitPath, false>(src, tally);
+    else TraceTasks<ChainPath<kOneLight && kEmKind == kEmSteerable>, false>(src, tally);
     FlushTally(tally, counters);
 }
 

@@ -1,10 +1,13 @@
 // Per-sample radiance estimators of the reference renderers, written as iterative device code.
-// PathTracer::EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with splitting budget 1 is a
-// chain: pixel = emitted(1) + sum_v B_v * (NEE_v + E_v), where B_v is the path throughput up to
-// vertex v, NEE_v the MIS-weighted light sample (path_tracer_base.hxx:350-425) and E_v the
-// MIS-weighted emission found by the BSDF sample of v (path_tracer.hxx:327-369).  The sums are
-// accumulated forward; the indirect clipping of path_tracer.hxx:400-402 needs the whole suffix and
-// is therefore rejected by the host when -iic > 0.
+// PathTracer::EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with splitting budget 1 and no
+// indirect clipping is a chain: pixel = emitted(1) + sum_v B_v * (NEE_v + E_v), where B_v is the
+// path throughput up to vertex v, NEE_v the MIS-weighted light sample (path_tracer_base.hxx:
+// 350-425) and E_v the MIS-weighted emission found by the BSDF sample of v (path_tracer.hxx:
+// 327-369); the sums accumulate forward (ChainPath, RadiancePT).  The indirect clipping of
+// path_tracer.hxx:400-402 needs each vertex's whole suffix: with splitting at the first vertex
+// only (every integer -sb at splitting level 1, the reference's defaults) SplitPath folds per-lane
+// records of each chain back to front; every other splitting configuration takes the general
+// recursion walker RadiancePTTree.
 #pragma once
 
 #include "dev_scene.hpp"
@@ -916,27 +919,6 @@ PG3_DEV void StageMarginal(const DevEm &E, float *lds, TreeCache &tc) {
     tc.marg = lds;
 }
 
-// Adds LDS slot `slot` (64 pixels x 3 sums of one tile) to the framebuffer and clears it.
-PG3_DEV void FlushSlot(double *slot, uint32_t x0, uint32_t y0, uint32_t resX, uint32_t resY, double *fb) {
-    const uint32_t lane = __lane_id();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t x = x0 + (lane & 7u), y = y0 + (lane >> 3);
-    if (x < resX && y < resY) {
-        double *p = fb + (size_t)(y * resX + x) * 3;
-        atomicAdd(p + 0, slot[lane * 3 + 0]);
-        atomicAdd(p + 1, slot[lane * 3 + 1]);
-        atomicAdd(p + 2, slot[lane * 3 + 2]);
-    }
-    slot[lane * 3 + 0] = 0.0;
-    slot[lane * 3 + 1] = 0.0;
-    slot[lane * 3 + 2] = 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // A work item of PathTracer -sb 1: the 8x8 pixel tile with corner (x0, y0) of scene `scene`
 // x iterations [it, it + total / 64).  Its samples q = 0 .. total-1 are (tile pixel q % 64,
 // iteration it + q / 64).
@@ -945,113 +927,85 @@ struct TileItem {
     double *fb;
 };
 
-// Persistent wavefront loop over work items.  Samples are handed out in order, one to each lane
-// whose path has ended (path regeneration across pixels and iterations), and the wavefront keeps
-// two items open: once the newer item has handed out its last sample, lanes whose paths end
-// start samples of a further item in the other slot while the older item's last paths still
-// run, so no lane waits for the longest path of an item.  Each item's sums are gathered in its
-// LDS slot (`acc`: 2 x 64 pixels x 3 FP64) and added to its framebuffer as soon as its last
-// sample ends.  Open items share one scene: an item of another scene waits until the older item
-// has drained.  All slot bookkeeping is wave-uniform.  Which lane traces which sample changes
-// nothing: a sample's path depends only on its (pixel, iteration) RNG stream.
+// Persistent wavefront loop over work items.  Tasks are handed out in order, one to each lane
+// whose task has ended (path regeneration across pixels and iterations), and a lane whose task
+// ends adds its radiance to its pixel of the framebuffer at once (FP64 atomics), so an item is
+// done with as soon as it has handed out its last task: no lane waits for another lane's path,
+// however long (Russian roulette, deep chains).  The lanes of a wavefront share one scene (its
+// tables are read through scalar loads): with kMultiScene, an item of another scene waits until
+// the running lanes have finished.  All bookkeeping is wave-uniform.  Which lane traces which
+// task changes nothing: a sample's path depends only on its (pixel, iteration) RNG stream.
 //   src.Next(TileItem&) -> bool   fetches the next item (false: none left; wave-uniform)
 //   src.View(scene) -> const SceneView *, src.Spec(scene) -> LightSpec
-//   Path: a lane's sample; Start(S, pixel, iteration), Step(S, spec, tally, shadowed) -> true
-//   when the sample is done, Result() -> its radiance.  Step traces exactly one extension ray
-//   (counted here) and counts its segments; a path with at most one shadow ray per step sets
-//   `shadowed` when it traced one (counted here), others count their own in the tally.
-template <class Path, class Src>
-PG3_DEV void TraceItems(Src &src, double *acc, RayTally &tally) {
+//   Path: a lane's task.  Path::Subs(S) tasks per camera sample (sub-samples whose radiances add
+//   up to the sample's); Start(S, pixel, iteration, sub); Step(S, spec, tally, shadowed, step) ->
+//   true when the task is done, with `step` the wavefront's step number (the same on every lane);
+//   Result() -> its radiance; CountsSample() -> whether the task counts as a camera sample.  Step
+//   traces exactly one extension ray (counted here) and counts its segments; a path with at most
+//   one shadow ray per step sets `shadowed` when it traced one (counted here), others count
+//   their own in the tally.
+template <class Path, bool kMultiScene, class Src>
+PG3_DEV void TraceTasks(Src &src, RayTally &tally) {
     const uint32_t lane = __lane_id();
-#pragma unroll
-    for (uint32_t k = 0; k < 6; ++k) acc[lane + 64u * k] = 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    TileItem A{}, B{};                 // the two slots (wave-uniform)
-    uint32_t penA = 0, penB = 0;       // samples of the slot in flight
-    bool liveA = false, liveB = false, more = true;
-    uint32_t cur = 0, next = 0;        // slot handing out samples, its first sample not handed out
-    bool held = false;                 // cur's scene differs from the scene of the running paths
-    uint32_t scene = 0;                // scene of the running paths
+    TileItem T{};                      // item handing out tasks (wave-uniform)
+    uint32_t next = 0;                 // its first task not handed out
+    uint32_t scene = 0;                // scene of the running lanes
+    bool more = true;
     bool busy = false;
-    uint32_t pix = 0, slot = 0;        // this lane's sample: tile pixel and slot
+    uint32_t px = 0;                   // this lane's pixel
+    double *fb = nullptr;              // and its framebuffer
+    uint32_t step = 0;                 // steps taken by the wavefront (wave-uniform)
     Path path;
     for (;;) {
         const uint64_t idle = __ballot(!busy);
-        if (idle != 0 && more && !(liveA && liveB) && !((cur ? liveB : liveA) && next < (cur ? B.total : A.total))) {
-            TileItem t;
-            if (!src.Next(t)) more = false;
-            else {
-                cur = liveA ? 1u : 0u;
-                if (cur) B = t;
-                else A = t;
+        if (idle != 0 && more && next >= T.total) {
+            if (src.Next(T)) {
+                T.total *= Path::Subs(*src.View(T.scene));
                 next = 0;
-                held = (t.scene != scene) && (liveA || liveB);
-                if (cur) { penB = 0; liveB = true; }
-                else { penA = 0; liveA = true; }
+            } else {
+                more = false;
             }
         }
-        if (held && !(cur ? liveA : liveB)) held = false;   // the older item has drained
-        const bool curOpen = !held && (cur ? liveB : liveA) && next < (cur ? B.total : A.total);
-        if (curOpen) scene = cur ? B.scene : A.scene;
+        if (kMultiScene && T.scene != scene && idle == ~0ull) scene = T.scene;   // drained: switch
         // the scene is looked up by index at each use: a pointer variable carried around the
         // loop would lose its address space (flat loads in place of scalar and LDS loads)
         const SceneView &S = *src.View(scene);
         const LightSpec spec = src.Spec(scene);
-        if (idle != 0 && curOpen) {
-            const TileItem &T = cur ? B : A;
+        if (idle != 0 && next < T.total && (!kMultiScene || T.scene == scene)) {
+            const uint32_t subs = Path::Subs(S);
             const uint32_t q = next + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-            bool start = false;
             if (!busy && q < T.total) {
-                pix = q & 63u;
+                const uint32_t pix = q & 63u;
                 const uint32_t x = T.x0 + (pix & 7u), y = T.y0 + (pix >> 3);
                 if (x < S.resX && y < S.resY) {
-                    path.Start(S, y * S.resX + x, T.it + (q >> 6));
+                    const uint32_t r = q >> 6;
+                    px = y * S.resX + x;
+                    fb = T.fb;
+                    path.Start(S, px, T.it + r / subs, r % subs);
                     busy = true;
-                    slot = cur;
-                    start = true;
                 }
             }
             next += (uint32_t)__popcll(idle);
-            const uint32_t started = (uint32_t)__popcll(__ballot(start));
-            if (cur) penB += started;
-            else penA += started;
         }
         const uint64_t stepping = __ballot(busy);
-        if (stepping != 0) {
-            tally.wExt += (uint32_t)__popcll(stepping);    // one Intersect per Step
-            bool done = false, shadowed = false;
-            if (busy) {
-                if (path.Step(S, spec, tally, shadowed)) {
-                    const F3 L = path.Result();
-                    double *a = acc + slot * 192u + pix * 3u;
-                    atomicAdd(a + 0, (double)L.x);
-                    atomicAdd(a + 1, (double)L.y);
-                    atomicAdd(a + 2, (double)L.z);
-                    busy = false;
-                    done = true;
-                }
-            }
-            const uint64_t fin = __ballot(done), finB = __ballot(done && slot != 0);
-            tally.wSamples += (uint32_t)__popcll(fin);
-            tally.wShadow += (uint32_t)__popcll(__ballot(shadowed));
-            penB -= (uint32_t)__popcll(finB);
-            penA -= (uint32_t)__popcll(fin & ~finB);
+        if (stepping == 0) {
+            if (!more && next >= T.total) break;
+            continue;
         }
-        // an item is complete once it has handed out every sample and none is in flight (a held
-        // item has handed out none)
-        if (liveA && penA == 0 && !(cur == 0 && (held || next < A.total))) {
-            const SceneView *V = src.View(A.scene);
-            FlushSlot(acc, A.x0, A.y0, V->resX, V->resY, A.fb);
-            liveA = false;
+        tally.wExt += (uint32_t)__popcll(stepping);        // one Intersect per Step
+        bool shadowed = false, counted = false;
+        if (busy && path.Step(S, spec, tally, shadowed, step)) {
+            const F3 L = path.Result();
+            double *f = fb + (size_t)px * 3;
+            atomicAdd(f + 0, (double)L.x);
+            atomicAdd(f + 1, (double)L.y);
+            atomicAdd(f + 2, (double)L.z);
+            busy = false;
+            counted = path.CountsSample();
         }
-        if (liveB && penB == 0 && !(cur == 1 && (held || next < B.total))) {
-            const SceneView *V = src.View(B.scene);
-            FlushSlot(acc + 192, B.x0, B.y0, V->resX, V->resY, B.fb);
-            liveB = false;
-        }
-        if (!more && !liveA && !liveB) break;
+        tally.wSamples += (uint32_t)__popcll(__ballot(counted));
+        tally.wShadow += (uint32_t)__popcll(__ballot(shadowed));
+        ++step;
     }
 }
 
@@ -1061,12 +1015,14 @@ template <bool kMergeEm>
 struct ChainPath {
     RngKey key;
     PathState ps;
-    PG3_DEV void Start(const SceneView &S, uint32_t pixel, uint32_t iteration) {
+    PG3_DEV static constexpr uint32_t Subs(const SceneView &) { return 1u; }
+    PG3_DEV static constexpr bool CountsSample() { return true; }
+    PG3_DEV void Start(const SceneView &S, uint32_t pixel, uint32_t iteration, uint32_t) {
         F3 org, dir;
         CameraSample(S, pixel, iteration, key, org, dir);
         InitPath(ps, org, dir);
     }
-    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &t, bool &shadowed) {
+    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &t, bool &shadowed, uint32_t) {
         Hit h;
         h.dist = 1e36f;
         const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
@@ -1079,79 +1035,87 @@ struct ChainPath {
     PG3_DEV F3 Result() const { return ps.L; }
 };
 
-// A lane's sample of EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with splitting at the
-// first vertex only and indirect clipping: the case of every integer splitting budget at the
-// reference's splitting level 1 (its defaults -sb 4 -iic 16), where ComputeSplittingCounts
-// (:432-454) gives the first vertex nB BSDF and nL light samples and budget 1 to every later
-// vertex.  The first vertex's estimate is its light samples plus, per BSDF sample, the MIS-
-// weighted emission found and the clipped indirect estimate of that sample's subtree; the terms
-// of different BSDF samples add, so the nB subtrees are traced one after another, each a chain.
-// Clipping makes a chain vertex's estimate R_v = A_v + clip(w_v R_{v+1}) depend on the whole
-// suffix (A_v: its light samples and the emission its BSDF sample finds), so each vertex pushes
-// (A_v, the factors of w_v) on a per-lane ring of kChainStack records and the chain's end folds
-// them back to front.  A chain deeper than the ring (Russian roulette lets a path bounce inside
-// a dielectric for many vertices; the reference stops at 1200) folds its deepest kChainStack
-// records, then re-traces the chain from its start up to the folded depth (the counter-based RNG
-// replays it exactly) and folds the next segment onto the folded suffix, and so on to the root:
-// the same operations in the same order as one fold of the whole chain.  One vertex per step,
-// the same RNG streams, operations and order as RadiancePTTree, which handles every other
-// splitting configuration.
-constexpr uint32_t kChainStack = kTreeDepth - 1;
+// A lane's share of one sample of EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with
+// splitting at the first vertex only and indirect clipping: the case of every integer splitting
+// budget at the reference's splitting level 1 (its defaults -sb 4 -iic 16), where
+// ComputeSplittingCounts (:432-454) gives the first vertex nB BSDF and nL light samples and
+// budget 1 to every later vertex.  The first vertex's estimate is its light samples plus, per BSDF
+// sample s, the MIS-weighted emission s finds and the clipped indirect estimate of s's subtree, a
+// chain; these terms add, so a sample splits into nB sub-samples that run on their own lanes:
+// sub-sample s traces the camera ray, takes the first vertex's light samples l = s, s + nB, ...
+// and its BSDF sample s, and follows s's chain.  Every lane runs the same per-vertex code (one
+// light-sample set and one BSDF sample per step), and the pixel's sum adds the sub-samples.
+// Clipping makes a vertex's estimate R_d = A_d + clip(W_d R_{d+1}) depend on the whole suffix (A_d:
+// its light samples and the emission its BSDF sample finds, W_d: that sample's weight), so each
+// vertex's record is pushed on a per-lane ring and the chain's end folds the ring back to front.
+// The ring slot of a push is the wavefront's step number (a lane pushes at most once per step), so
+// the lanes of a wavefront store to and load from the same slot: scratch accesses coalesce into
+// whole cache lines.  A chain deeper than the ring (Russian roulette lets a path bounce inside a
+// dielectric for many vertices; the reference stops at 1200) folds the records the ring still
+// holds, re-traces its sub-sample from the camera ray up to the folded depth (the counter-based RNG
+// replays it exactly) and folds the next part onto the folded suffix, and so on to the first
+// vertex: the same operations in the same order as one fold of the whole chain.  RNG streams as
+// RadiancePTTree (the chain of BSDF sample s is branch s + 1 when nB > 1).
+constexpr uint32_t kChainRing = 32;
 struct SplitPath {
-    // ray in flight: it reaches vertex `len` on RNG branch `branch`
+    // ray in flight: it reaches vertex `len` (1: the camera ray's hit) of sub-sample `sub`
     F3 org, dir;
     float tmin;
-    uint32_t len, branch;
+    uint32_t len, sub;
     bool refl;                   // the vertex it reaches is shaded (not cut by Russian roulette)
-    RngKey key;                  // branch 0 (c3 low half = 0)
-    // first vertex
-    F3 rSp, rN, rWol, em1, reflRoot;
-    int32_t rMat;
-    float rRR, rLcSum;
-    uint32_t nB, nL, s, rSides;
-    F3 rAttCos;                  // current first-vertex BSDF sample: clip((rAttCos * R) / rDen)
-    float rDen;
-    bool rCut;
-    // parent of the vertex the ray reaches (MIS of the emission found there)
-    F3 pAttCos, pSp, pN;
-    float pPdfW, pComp, pLcSum;
-    uint32_t pSides, pLen, pBranch, pnB, pnL;
-    bool pFinite;
-    // chain ring: record d (chain depth d = vertex d + 2) in slot d % kChainStack; `top` is the
-    // newest record's depth (-1: none) and its A is kept in topA.  Re-tracing a deep chain:
-    // `target` is the depth at which the pass stops (-1: first pass) and rSuffix the folded
-    // estimate of the chain beyond it.
-    int32_t top, target;
-    uint64_t cutMask;            // per slot: the record's BSDF sample was cut by Russian roulette
-    F3 topA, rSuffix;
-    float stA[kChainStack][3], stW[kChainStack][3], stDen[kChainStack];
-    F3 L;
+    uint32_t pixel, iteration;   // RNG key of the sample (with the scene's key and id)
+    F3 L;                        // the sub-sample's radiance (sub-sample 0 adds the camera hit's emission)
+    // newest record (vertex len - 1): A, the indirect factors of its BSDF sample (attCos, den;
+    // den < 0: no indirect term -- cut by Russian roulette or no sample), and what the MIS weight
+    // of the emission found at vertex len needs of that vertex (its position is `org`): its normal,
+    // pdfW * compProb of its BSDF sample (compProb for a Dirac sample), light-cache sum, flags
+    F3 tA, tAttCos, pN;
+    float tDen, pPdf, pLcSum;
+    uint32_t pFlags;             // bit 0: finite BSDF sample; sides << 4
+    // records of depths [0, nPush) (vertex = depth + 1) pushed at consecutive steps.  Re-tracing a
+    // deep chain: the pass stops at depth `target` (-1: first pass), whose estimate is rSuffix.
+    uint32_t nPush;
+    int32_t target;
+    F3 rSuffix;
+    float ring[kChainRing][7];   // A.xyz, attCos.xyz, den
 
-    PG3_DEV void Start(const SceneView &S, uint32_t pixel, uint32_t iteration) {
-        CameraSample(S, pixel, iteration, key, org, dir);
+    // BSDF samples of the first vertex (ComputeSplittingCounts, :440-446): the sub-samples per sample
+    PG3_DEV static uint32_t Subs(const SceneView &S) {
+        return (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * S.splitBudget);
+    }
+    PG3_DEV static uint32_t LightCount(const SceneView &S, uint32_t nBv) {
+        return (uint32_t)lroundf(fmaxf(S.lightRatio * nBv, 1.f));
+    }
+    PG3_DEV bool CountsSample() const { return sub == 0u; }
+    PG3_DEV void Start(const SceneView &S, uint32_t px, uint32_t it, uint32_t s) {
+        RngKey key;
+        CameraSample(S, px, it, key, org, dir);
+        pixel = px;
+        iteration = it;
+        sub = s;
         tmin = 0.f;
         len = 1;
-        branch = 0;
         refl = true;
-        em1 = mk(0.f);
-        reflRoot = mk(0.f);
-        top = -1;
+        L = mk(0.f);
+        nPush = 0;
         target = -1;
     }
-    PG3_DEV static uint32_t Slot(int32_t d) { return (uint32_t)d % kChainStack; }
-    PG3_DEV RngKey Branch(uint32_t b) const {
-        RngKey k = key;
-        k.c3 = (key.c3 & 0xFFFF0000u) | b;
-        return k;
+    PG3_DEV RngKey Branch(const SceneView &S, uint32_t b) const {
+        return RngKey{S.k0, S.k1, pixel, iteration, (S.sceneId << 16) | b};
     }
-    // Light-sample estimate of a shaded vertex (:252-265, :350-425) with nLv light and nBv BSDF
-    // samples (MIS counts).
+    // RNG branch of vertex v of this sub-sample
+    PG3_DEV uint32_t BranchOf(uint32_t v, uint32_t nB) const {
+        return (v == 1u || nB <= 1u) ? 0u : ((sub + 1u) & 0xFFFFu);
+    }
+    // Light samples l = l0, l0 + dl, ... < nLv of a shaded vertex (:252-265, :350-425) with MIS
+    // counts nLv, nBv.
     PG3_DEV F3 LightSamples(const SceneView &S, const LightSpec &spec, const RngKey &k, uint32_t vlen, F3 sp,
-                            const Frame &fr, F3 wol, const DevMaterial &mat, uint32_t nLv, uint32_t nBv,
-                            const LightCache &lc, uint32_t &shadows) const {
+                            const Frame &fr, F3 wol, const DevMaterial &mat, uint32_t l0, uint32_t dl, uint32_t nLv,
+                            uint32_t nBv, const LightCache &lc, uint32_t &shadows) const {
         F3 acc = mk(0.f);
         if ((vlen + 1) < S.minPL) return acc;
-        for (uint32_t l = 0; l < nLv; ++l) {
+#pragma unroll 1
+        for (uint32_t l = l0; l < nLv; l += dl) {
             float prob = 1.f;
             const int32_t li = PickLight(spec, lc, RngGet(k, vlen, l, kDimLightPick), prob);
             LightSample ls;
@@ -1173,89 +1137,64 @@ struct SplitPath {
         }
         return acc;
     }
-    // BSDF sample `sub` of a vertex; false for a blocker.  Sets the ray to the next vertex, the
-    // parent record for that vertex's emission, and the indirect-term factors (attCos, den, cut).
-    PG3_DEV bool SampleAt(const SceneView &S, const RngKey &k, uint32_t vlen, uint32_t vbranch, uint32_t sub, F3 sp,
-                          const Frame &fr, F3 wol, const DevMaterial &mat, uint32_t nBv, uint32_t nLv, float rr,
-                          float lcSum, F3 &attCos, float &den, bool &cut) {
-        cut = (S.maxPL == 0) && (RngGet(k, vlen, sub, kDimRR) > rr);
-        BsdfRec r;
-        SampleBsdf(mat, wol, RngGet(k, vlen, sub, kDimBsdf0), RngGet(k, vlen, sub, kDimBsdf1),
-                   RngGet(k, vlen, sub, kDimBsdf2), r);
-        if (IsBlocker(r)) return false;
-        const bool finite = (r.pdfW != kInf);
-        const float cosAbs = fabsf(r.wil.z);
-        attCos = finite ? r.att * cosAbs : r.att;
-        den = finite ? (r.pdfW * nBv * rr * r.compProb) : (nBv * rr * r.compProb);
-        pAttCos = attCos;
-        pPdfW = r.pdfW;
-        pComp = r.compProb;
-        pFinite = finite;
-        pSp = sp;
-        pN = fr.z;
-        pSides = mat.sides;
-        pLen = vlen;
-        pBranch = vbranch;
-        pnB = nBv;
-        pnL = nLv;
-        pLcSum = lcSum;
-        org = sp;
-        dir = ToWorld(fr, r.wil);
-        tmin = EpsRayCos(cosAbs);
-        len = vlen + 1;
-        branch = (nBv > 1) ? ((vbranch * 7u + sub + 1u) & 0xFFFFu) : vbranch;
-        refl = !cut;
-        return true;
-    }
-    // MIS-weighted emission delivered to the parent (path_tracer.hxx:327-369)
-    PG3_DEV F3 ParentEmission(const SceneView &S, const LightSpec &spec, F3 em, float lpdf, int32_t lid) const {
+    // MIS-weighted emission that the BSDF sample of vertex len - 1 finds at vertex len
+    // (path_tracer.hxx:327-369)
+    PG3_DEV F3 ParentEmission(const SceneView &S, const LightSpec &spec, F3 em, float lpdf, int32_t lid,
+                              uint32_t nB) const {
         if (IsZero(em) || lid < 0) return mk(0.f);
-        if (!pFinite) return FastDiv(pAttCos * em, static_cast<float>(pnB) * pComp);
+        const uint32_t pnB = (len == 2u) ? nB : 1u;
+        if (!(pFlags & 1u)) return FastDiv(tAttCos * em, static_cast<float>(pnB) * pPdf);
         float pick = 1.f;
         if (spec.numLights == 0) {
             pick = 0.f;
         } else if (spec.numLights > 1) {
-            pick = (pLcSum > 0.f) ? EstimateContribution(S, spec, (uint32_t)lid, pSp, FrameFromZ(pN), pSides,
-                                                         Branch(pBranch), pLen) / pLcSum
+            pick = (pLcSum > 0.f) ? EstimateContribution(S, spec, (uint32_t)lid, org, FrameFromZ(pN), pFlags >> 4,
+                                                         Branch(S, BranchOf(len - 1u, nB)), len - 1u) / pLcSum
                                   : 1.f / spec.numLights;
         }
         const float lp = lpdf * pick;
-        const float bp = pPdfW * pComp;
-        return FastDiv(pAttCos * em * MisPowerN(bp, pnB, lp, pnL), bp);
+        return FastDiv(tAttCos * em * MisPowerN(pPdf, pnB, lp, LightCount(S, pnB)), pPdf);
     }
     PG3_DEV static F3 Indirect(F3 attCos, F3 R, float den, float iic) {
         F3 ind = FastDiv(attCos * R, den);
         if (iic > 0.f) ind = ClipProportionally(ind, iic);
         return ind;
     }
-    // End of a chain pass (records [0, top] pushed): fold the records the ring still holds onto
-    // the suffix beyond them.  If older records were overwritten, set up the re-trace up to the
-    // oldest folded depth and return false (the caller restarts the chain from the same
-    // first-vertex sample); otherwise add the clipped indirect term of the subtree and return true.
-    PG3_DEV bool FoldChain(const SceneView &S) {
-        const int32_t lo = max(0, top + 1 - (int32_t)kChainStack);
-        F3 R = (target >= 0) ? rSuffix : mk(0.f);
-        for (int32_t k = top; k >= lo; --k) {
-            const uint32_t sl = Slot(k);
-            const F3 A = (k == top) ? topA : mk(stA[sl][0], stA[sl][1], stA[sl][2]);
-            F3 Rk = A;
-            if (!((cutMask >> sl) & 1ull) && !IsZero(R))
-                Rk += Indirect(mk(stW[sl][0], stW[sl][1], stW[sl][2]), R, stDen[sl], S.iic);
+    // End of a chain pass at wavefront step t.  The newest record is in registers and rChild is the
+    // estimate of the vertex after it (0, or the folded suffix at a re-trace's target); the ring
+    // holds the older records, the newest of them pushed at step t - jStart.  Folds them back to
+    // front; if the oldest records were overwritten, sets up the re-trace up to the oldest folded
+    // depth and returns false, else adds the first vertex's estimate to L and returns true.
+    PG3_DEV bool Fold(const SceneView &S, uint32_t t, uint32_t jStart, F3 rChild) {
+        F3 R = tA;
+        if (tDen > 0.f && !IsZero(rChild)) R += Indirect(tAttCos, rChild, tDen, S.iic);
+        const uint32_t n = min(nPush, kChainRing);
+#pragma unroll 1
+        for (uint32_t i = 0; i < n; ++i) {
+            const float *r = ring[(t - jStart - i) % kChainRing];
+            F3 Rk = mk(r[0], r[1], r[2]);
+            const float den = r[6];
+            if (den > 0.f && !IsZero(R)) Rk += Indirect(mk(r[3], r[4], r[5]), R, den, S.iic);
             R = Rk;
         }
-        top = -1;
-        if (lo > 0) {
+        if (nPush > kChainRing) {
+            target = (int32_t)(nPush - kChainRing);
             rSuffix = R;
-            target = lo;
+            RngKey key;
+            CameraSample(S, pixel, iteration, key, org, dir);
+            tmin = 0.f;
+            len = 1;
+            refl = true;
+            nPush = 0;
             return false;
         }
-        if (!rCut && !IsZero(R)) reflRoot += Indirect(rAttCos, R, rDen, S.iic);
-        target = -1;
-        ++s;
+        L += R;
         return true;
     }
-    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &t, bool &) {
-        const RngKey k = Branch(branch);
+    // One vertex per step (t: the wavefront's step number, the same on every lane).
+    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &tally, bool &, uint32_t t) {
+        const uint32_t nB = Subs(S);
+        const RngKey k = Branch(S, BranchOf(len, nB));
         Hit h;
         h.dist = 1e36f;
         const bool hit = Intersect(S, org, dir, tmin, h);
@@ -1282,104 +1221,63 @@ struct SplitPath {
             }
         } else if (len >= S.minPL && S.bgLight >= 0) {                               // :408-422
             em = BgRadiance(S, spec, dir);
-            if (len > 1) lpdf = BgPdf(S, spec, dir, FrameFromZ(pN), pSides);
+            if (len > 1) lpdf = BgPdf(S, spec, dir, FrameFromZ(pN), pFlags >> 4);
             lid = S.bgLight;
         }
-        // (a re-traced pass delivers the emission of vertex 2 to the first vertex only once)
-        if (len == 1) em1 = em;
-        else if (len == 2) { if (target < 0) reflRoot += ParentEmission(S, spec, em, lpdf, lid); }
-        else topA += ParentEmission(S, spec, em, lpdf, lid);
-        const bool atTarget = (target >= 0) && ((int32_t)len - 2 == target);
+        if (len == 1) {
+            if (sub == 0u && target < 0) L += em;      // the camera hit's emission, once per sample
+        } else {
+            tA += ParentEmission(S, spec, em, lpdf, lid, nB);
+        }
+        const bool atTarget = (target >= 0) && ((int32_t)len - 1 == target);
         const bool stop = atTarget || !hit || !refl || ReflectanceZero(mat) || (S.maxPL > 0 && len >= S.maxPL) ||
                           len >= 1200u;
-        // what the step samples next: the first vertex's next BSDF sample or the chain vertex's one
-        bool wantRoot = false, wantChain = false;
-        float rr = 1.0f;
+        if (stop) {
+            if (len == 1) return true;
+            return Fold(S, t, 1u, atTarget ? rSuffix : mk(0.f));
+        }
+        const bool first = (len == 1);
+        if (target < 0 && (!first || sub == 0u)) ++tally.segments;   // the shared first vertex once
+        if (!first) {                                   // the parent's record is complete: push it
+            float *r = ring[t % kChainRing];
+            r[0] = tA.x;
+            r[1] = tA.y;
+            r[2] = tA.z;
+            r[3] = tAttCos.x;
+            r[4] = tAttCos.y;
+            r[5] = tAttCos.z;
+            r[6] = tDen;
+            ++nPush;
+        }
         LightCache lc;
         lc.sum = 0.f;
-        if (stop) {
-            if (len == 1) {
-                L = em1 + reflRoot;
-                return true;
-            }
-            FoldChain(S);
-            wantRoot = true;
-        } else {
-            if (target < 0) ++t.segments;
-            if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
-            const bool first = (len == 1);
-            if (first) {                                                              // :240-250
-                nB = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * S.splitBudget);
-                nL = (uint32_t)lroundf(fmaxf(S.lightRatio * nB, 1.f));
-            }
-            const uint32_t nLv = first ? nL : (uint32_t)lroundf(fmaxf(S.lightRatio * 1u, 1.f));
-            const F3 A = LightSamples(S, spec, k, len, sp, fr, wol, mat, nLv, first ? nB : 1u, lc, t.shadow);
-            rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
-            if (first) {
-                reflRoot = A;
-                rSp = sp;
-                rN = fr.z;
-                rWol = wol;
-                rMat = h.mat;
-                rSides = mat.sides;
-                rLcSum = lc.sum;
-                rRR = rr;
-                s = 0;
-                wantRoot = true;
-            } else {
-                if (top >= 0) {
-                    const uint32_t sl = Slot(top);
-                    stA[sl][0] = topA.x;
-                    stA[sl][1] = topA.y;
-                    stA[sl][2] = topA.z;
-                }
-                ++top;
-                topA = A;
-                wantChain = true;
-            }
+        if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
+        const uint32_t nBv = first ? nB : 1u;
+        tA = LightSamples(S, spec, k, len, sp, fr, wol, mat, first ? sub : 0u, nBv, LightCount(S, nBv), nBv, lc,
+                          tally.shadow);
+        const float rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
+        const uint32_t si = first ? sub : 0u;           // :281-404, BSDF sample si of this vertex
+        const bool cut = (S.maxPL == 0) && (RngGet(k, len, si, kDimRR) > rr);
+        BsdfRec r;
+        SampleBsdf(mat, wol, RngGet(k, len, si, kDimBsdf0), RngGet(k, len, si, kDimBsdf1), RngGet(k, len, si, kDimBsdf2),
+                   r);
+        if (IsBlocker(r)) {
+            tDen = -1.f;
+            return Fold(S, t, 0u, mk(0.f));
         }
-        // One call site of the BSDF sampler for both kinds of vertex.  A chain vertex whose sample
-        // is a blocker ends its chain, and the first vertex's next sample follows in the same pass.
-        const uint32_t nLc = (uint32_t)lroundf(fmaxf(S.lightRatio * 1u, 1.f));
-        for (int pass = 0; pass < 2 && (wantRoot || wantChain); ++pass) {
-            if (wantRoot && s >= nB) {
-                L = em1 + reflRoot;
-                return true;
-            }
-            F3 attCos;
-            float den;
-            bool cut;
-            const DevMaterial vm = wantRoot ? LoadGlobal(S.mats + rMat) : mat;
-            const Frame vfr = wantRoot ? FrameFromZ(rN) : fr;
-            const bool ok = SampleAt(S, wantRoot ? key : k, wantRoot ? 1u : len, wantRoot ? 0u : branch, wantRoot ? s : 0u,
-                                     wantRoot ? rSp : sp, vfr, wantRoot ? rWol : wol, vm, wantRoot ? nB : 1u,
-                                     wantRoot ? nL : nLc, wantRoot ? rRR : rr, wantRoot ? rLcSum : lc.sum, attCos, den,
-                                     cut);
-            if (wantRoot) {
-                if (ok) {
-                    rAttCos = attCos;
-                    rDen = den;
-                    rCut = cut;
-                    cutMask = 0;
-                    return false;
-                }
-                ++s;                                        // a blocker: try the next sample
-                --pass;
-                continue;
-            }
-            if (ok) {
-                const uint32_t sl = Slot(top);
-                stW[sl][0] = attCos.x;
-                stW[sl][1] = attCos.y;
-                stW[sl][2] = attCos.z;
-                stDen[sl] = den;
-                cutMask = (cutMask & ~(1ull << sl)) | ((uint64_t)cut << sl);
-                return false;
-            }
-            FoldChain(S);                                   // a blocker: this vertex has no subtree
-            wantChain = false;
-            wantRoot = true;
-        }
+        const bool finite = (r.pdfW != kInf);
+        const float cosAbs = fabsf(r.wil.z);
+        tAttCos = finite ? r.att * cosAbs : r.att;
+        tDen = cut ? -1.f : (finite ? (r.pdfW * nBv * rr * r.compProb) : (nBv * rr * r.compProb));
+        pPdf = finite ? r.pdfW * r.compProb : r.compProb;
+        pN = fr.z;
+        pLcSum = lc.sum;
+        pFlags = (finite ? 1u : 0u) | (mat.sides << 4);
+        org = sp;
+        dir = ToWorld(fr, r.wil);
+        tmin = EpsRayCos(cosAbs);
+        len = len + 1;
+        refl = !cut;
         return false;
     }
     PG3_DEV F3 Result() const { return L; }
