@@ -116,9 +116,9 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
     FlushTally(tally, counters);
 }
 
-// Path tracer with splitting (-sb > 1) or indirect clipping (-iic > 0): one thread per (pixel,
-// iteration chunk), each sample traced by RadiancePTTree (its vertex stack lives in scratch).
-__global__ void __launch_bounds__(256) RenderTreeKernel(SceneView S, uint32_t firstIter, uint32_t nIter,
+// Path tracer with splitting beyond the first vertex: one thread per (pixel, iteration chunk),
+// each sample traced by RadiancePTTree (its vertex stack and chain records live in scratch).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) RenderTreeKernel(SceneView S, uint32_t firstIter, uint32_t nIter,
                                                          uint32_t itersPerThread, uint32_t numChunks,
                                                          double *__restrict__ fb,
                                                          unsigned long long *__restrict__ counters) {

@@ -660,11 +660,16 @@ PG3_DEV F3 RadianceDirect(const SceneView &S, const RngKey &key, F3 org, F3 dir,
 // ---------------------------------------------------------------------------------------------
 // EstimateIncomingRadiancePT with splitting (-sb > 1) and indirect clipping (-iic > 0): the
 // recursion of path_tracer.hxx:167-430 run as a post-order walk over an explicit stack of
-// vertices, since clipping needs each child's reflected estimate before it is weighted.  Paths
-// are cut at kTreeDepth vertices (the reference cuts at 1200, kMaxPathLengthHard); with
-// -maxpl <= kTreeDepth the two agree exactly.  (SplitPath, the first-vertex splitting of the
-// reference's defaults, re-traces deep chains instead and keeps the reference's 1200.)
-constexpr uint32_t kTreeDepth = 48;
+// vertices, since clipping needs each child's reflected estimate before it is weighted.
+// ComputeSplittingCounts (:432-454) gives a vertex with budget b nB = round(1 - l + l b) BSDF
+// samples and its children budget 1 + (b - nB) / nB = b / nB: nB never grows along a path, a
+// vertex with nB >= 2 halves the budget at least, and a vertex with nB = 1 passes its budget on
+// unchanged, so below at most 6 branching vertices (budgets <= 64) every subtree is a chain.  The
+// walker keeps stack frames for the branching vertices only and evaluates a chain with
+// ChainReflected, which folds a ring of per-vertex records and re-traces deep chains, so paths
+// keep the reference's limit of 1200 vertices (kMaxPathLengthHard, :232-238).
+constexpr uint32_t kTreeDepth = 8;            // stack frames: branching vertices (at most 6)
+constexpr uint32_t kTreeChainRing = 32;       // per-thread fold records of a chain
 
 // MISWeight2Power with sample counts, path_tracer_base.hxx:522-541
 PG3_DEV float MisPowerN(float p1, uint32_t n1, float p2, uint32_t n2) {
@@ -682,6 +687,196 @@ PG3_DEV F3 ClipProportionally(F3 v, float maxAllowed) {
         v.z *= s;
     }
     return v;
+}
+// Indirect term of a BSDF sample: (att |cos| R) / (pdfW nB rr compProb), clipped proportionally
+// to -iic (path_tracer.hxx:385-402).
+PG3_DEV F3 ClippedIndirect(F3 attCos, F3 R, float den, float iic) {
+    F3 ind = FastDiv(attCos * R, den);
+    if (iic > 0.f) ind = ClipProportionally(ind, iic);
+    return ind;
+}
+
+// Light samples l = l0, l0 + dl, ... < nLv of a shaded vertex (path_tracer.hxx:252-265,
+// path_tracer_base.hxx:350-425) with MIS sample counts nLv, nBv; counts the shadow rays traced.
+PG3_DEV F3 LightSampleSum(const SceneView &S, const LightSpec &spec, const RngKey &k, uint32_t vlen, F3 sp,
+                          const Frame &fr, F3 wol, const DevMaterial &mat, uint32_t l0, uint32_t dl, uint32_t nLv,
+                          uint32_t nBv, const LightCache &lc, uint32_t &shadows) {
+    F3 acc = mk(0.f);
+    if ((vlen + 1) < S.minPL) return acc;
+#pragma unroll 1
+    for (uint32_t l = l0; l < nLv; l += dl) {
+        float prob = 1.f;
+        const int32_t li = PickLight(spec, lc, RngGet(k, vlen, l, kDimLightPick), prob);
+        LightSample ls;
+        if (li >= 0 && SampleIllumination(S, spec, (uint32_t)li, sp, fr, mat.sides, RngGet(k, vlen, l, kDimLight0),
+                                          RngGet(k, vlen, l, kDimLight1), RngGet(k, vlen, l, kDimLight2), ls)) {
+            ls.prob = prob;
+            if (MaxC(ls.sample) > 0.f && (++shadows, !Occluded(S, sp, ls.wig, ls.dist))) {
+                BsdfRec r;
+                EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
+                if (ls.pdfW != kInf) {
+                    const float bsdfPdf = r.pdfW * r.compProb;
+                    const float lightPdf = ls.pdfW * ls.prob;
+                    acc += FastDiv(ls.sample * r.att * MisPowerN(lightPdf, nLv, bsdfPdf, nBv), lightPdf);
+                } else {
+                    acc += FastDiv(ls.sample * r.att, ls.prob * nLv);
+                }
+            }
+        }
+    }
+    return acc;
+}
+
+// The parent vertex of an emission hit, as the MIS weight of that emission needs it
+// (path_tracer.hxx:327-369): its position and normal, the factor att * |cos| (att for a Dirac
+// sample) and pdfW * compProb (compProb for a Dirac sample) of its BSDF sample, its sample counts,
+// light-cache sum, RNG key and vertex index.
+struct EmissionParent {
+    F3 sp, n, attCos;
+    float pdf, lcSum;
+    uint32_t flags;            // bit 0: finite BSDF sample; sides << 4
+    uint32_t nB, nL, len;
+    RngKey key;
+};
+// Emission `em` of light `lid` (pdf lpdf of the hit under light sampling) found by the parent's
+// BSDF sample, MIS-weighted.
+PG3_DEV F3 EmissionMis(const SceneView &S, const LightSpec &spec, F3 em, float lpdf, int32_t lid,
+                       const EmissionParent &P) {
+    if (IsZero(em) || lid < 0) return mk(0.f);
+    if (!(P.flags & 1u)) return FastDiv(P.attCos * em, static_cast<float>(P.nB) * P.pdf);
+    float pick = 1.f;
+    if (spec.numLights == 0) {
+        pick = 0.f;
+    } else if (spec.numLights > 1) {
+        pick = (P.lcSum > 0.f) ? EstimateContribution(S, spec, (uint32_t)lid, P.sp, FrameFromZ(P.n), P.flags >> 4, P.key,
+                                                      P.len) / P.lcSum
+                               : 1.f / spec.numLights;
+    }
+    return FastDiv(P.attCos * em * MisPowerN(P.pdf, P.nB, lpdf * pick, P.nL), P.pdf);
+}
+
+// Reflected radiance estimate of a shaded vertex whose subtree is a chain (every vertex one BSDF
+// sample: path_tracer.hxx:240-404 with nB = 1): R_d = A_d + clip(W_d R_{d+1}), where A_d holds the
+// vertex's light samples and the MIS-weighted emission its BSDF sample finds, and W_d that
+// sample's weight.  The chain is traced forward pushing (A, att|cos|, denominator) per vertex on a
+// ring of kTreeChainRing records and folded back to front at its end; when it is deeper than the
+// ring, the folded estimate of the oldest record still held becomes the suffix of a re-trace from
+// the chain's first vertex (the counter-based RNG replays it exactly), which stops at that depth:
+// the same operations in the same order as one fold of the whole chain.  Vertex 0 of the chain is
+// at `sp0` (normal n0, outgoing direction wol0, material mat0), is vertex `len0` of the path and
+// uses RNG key k.
+PG3_DEV F3 ChainReflected(const SceneView &S, const LightSpec &spec, const RngKey &k, uint32_t len0, F3 sp0, F3 n0,
+                          F3 wol0, int32_t mat0, RayTally &t) {
+    const uint32_t nL = (uint32_t)lroundf(fmaxf(S.lightRatio * 1u, 1.f));
+    float ring[kTreeChainRing][7];      // depth d in slot d % kTreeChainRing: A.xyz, attCos.xyz, den
+    int32_t target = -1;                // depth this pass stops at (-1: first pass)
+    F3 rSuffix = mk(0.f);               // folded estimate of that depth
+    for (;;) {
+        F3 sp = sp0, wol = wol0, nrm = n0;
+        int32_t mi = mat0;
+        uint32_t d = 0;                 // depth of the vertex being shaded (vertex len0 + d)
+        F3 tA = mk(0.f), tAttCos = mk(0.f), rChild = mk(0.f);
+        float tDen = -1.f;
+        for (;;) {                      // shade vertex d
+            const uint32_t len = len0 + d;
+            const Frame fr = FrameFromZ(nrm);
+            const DevMaterial mat = LoadGlobal(S.mats + mi);
+            if (target < 0) ++t.segments;
+            LightCache lc;
+            lc.sum = 0.f;
+            if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
+            if (d > 0) {                // the parent's record is complete: push it
+                float *r = ring[(d - 1) % kTreeChainRing];
+                r[0] = tA.x;
+                r[1] = tA.y;
+                r[2] = tA.z;
+                r[3] = tAttCos.x;
+                r[4] = tAttCos.y;
+                r[5] = tAttCos.z;
+                r[6] = tDen;
+            }
+            tA = LightSampleSum(S, spec, k, len, sp, fr, wol, mat, 0u, 1u, nL, 1u, lc, t.shadow);
+            const float rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
+            const bool cut = (S.maxPL == 0) && (RngGet(k, len, 0, kDimRR) > rr);
+            BsdfRec r;
+            SampleBsdf(mat, wol, RngGet(k, len, 0, kDimBsdf0), RngGet(k, len, 0, kDimBsdf1), RngGet(k, len, 0, kDimBsdf2),
+                       r);
+            if (IsBlocker(r)) {
+                tDen = -1.f;
+                break;
+            }
+            const bool finite = (r.pdfW != kInf);
+            const float cosAbs = fabsf(r.wil.z);
+            EmissionParent P;
+            P.sp = sp;
+            P.n = nrm;
+            P.attCos = finite ? r.att * cosAbs : r.att;
+            P.pdf = finite ? r.pdfW * r.compProb : r.compProb;
+            P.lcSum = lc.sum;
+            P.flags = (finite ? 1u : 0u) | (mat.sides << 4);
+            P.nB = 1u;
+            P.nL = nL;
+            P.len = len;
+            P.key = k;
+            tAttCos = P.attCos;
+            tDen = cut ? -1.f : (finite ? (r.pdfW * 1u * rr * r.compProb) : (1u * rr * r.compProb));
+            // the child vertex d + 1 (:176-238, :408-422)
+            const F3 dir = ToWorld(fr, r.wil);
+            Hit h;
+            h.dist = 1e36f;
+            ++t.ext;
+            const bool hit = Intersect(S, sp, dir, EpsRayCos(cosAbs), h);
+            F3 em = mk(0.f);
+            float lpdf = 0.f;
+            int32_t lid = -1;
+            const uint32_t clen = len + 1;
+            DevMaterial cm{};
+            F3 csp = mk(0.f), cwol = mk(0.f);
+            if (hit) {
+                csp = sp + dir * h.dist;
+                cwol = ToLocal(FrameFromZ(h.n), -dir);
+                cm = LoadGlobal(S.mats + h.mat);
+                if (cm.lightId >= 0 && clen >= S.minPL) {
+                    const DevLight Lt = LoadGlobal(S.lights + cm.lightId);
+                    if (Lt.type == kLightArea) {
+                        lpdf = AreaEmissionPdf(Lt, csp, sp);
+                        if (cwol.z > 0.f) em = Lt.radiance;
+                    } else if (Lt.type == kLightPoint) {
+                        lpdf = kInf;
+                    }
+                    lid = cm.lightId;
+                }
+            } else if (clen >= S.minPL && S.bgLight >= 0) {
+                em = BgRadiance(S, spec, dir);
+                lpdf = BgPdf(S, spec, dir, fr, mat.sides);
+                lid = S.bgLight;
+            }
+            tA += EmissionMis(S, spec, em, lpdf, lid, P);
+            const bool atTarget = (target >= 0) && ((int32_t)d + 1 == target);
+            if (atTarget) rChild = rSuffix;
+            if (atTarget || !hit || cut || ReflectanceZero(cm) || (S.maxPL > 0 && clen >= S.maxPL) || clen >= 1200u)
+                break;
+            sp = csp;
+            wol = cwol;
+            nrm = h.n;
+            mi = h.mat;
+            ++d;
+        }
+        // fold: the newest record (depth d) is in registers, depths [0, d) in the ring
+        F3 R = tA;
+        if (tDen > 0.f && !IsZero(rChild)) R += ClippedIndirect(tAttCos, rChild, tDen, S.iic);
+        const uint32_t lo = (d > kTreeChainRing) ? d - kTreeChainRing : 0u;
+#pragma unroll 1
+        for (uint32_t e = d; e > lo; --e) {
+            const float *r = ring[(e - 1) % kTreeChainRing];
+            F3 Rk = mk(r[0], r[1], r[2]);
+            if (r[6] > 0.f && !IsZero(R)) Rk += ClippedIndirect(mk(r[3], r[4], r[5]), R, r[6], S.iic);
+            R = Rk;
+        }
+        if (lo == 0u) return R;
+        target = (int32_t)lo;           // depths [0, lo) were overwritten: re-trace up to depth lo
+        rSuffix = R;
+    }
 }
 
 struct TreeFrame {              // one vertex of the recursion, in its BSDF-sample loop
@@ -737,14 +932,18 @@ PG3_DEV F3 RadiancePTTree(const SceneView &S, const LightSpec &spec, const RngKe
                     }
                     rLid = mat.lightId;
                 }
+                // ComputeSplittingCounts, :432-454
+                const uint32_t nB = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * cBudget);
+                const uint32_t nL = (uint32_t)lroundf(fmaxf(S.lightRatio * nB, 1.f));
+                // (a branching vertex beyond the stack cannot occur for budgets <= 64: cut defensively)
                 const bool stop = !cRefl || ReflectanceZero(mat) || (S.maxPL > 0 && cLen >= S.maxPL) ||
-                                  cLen >= 1200u || top + 1 >= (int32_t)kTreeDepth;
-                if (!stop) {
+                                  cLen >= 1200u || (nB != 1u && top + 1 >= (int32_t)kTreeDepth);
+                if (!stop && nB == 1u) {
+                    // one BSDF sample here, so in the whole subtree: a chain
+                    rRefl = ChainReflected(S, spec, key, cLen, sp, fr.z, wol, h.mat, t);
+                } else if (!stop) {
                     ++t.segments;
                     TreeFrame &F = stk[top + 1];
-                    // ComputeSplittingCounts, :432-454
-                    const uint32_t nB = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * cBudget);
-                    const uint32_t nL = (uint32_t)lroundf(fmaxf(S.lightRatio * nB, 1.f));
                     F.sp = sp;
                     F.n = fr.z;
                     F.wol = wol;
@@ -1107,58 +1306,21 @@ struct SplitPath {
     PG3_DEV uint32_t BranchOf(uint32_t v, uint32_t nB) const {
         return (v == 1u || nB <= 1u) ? 0u : ((sub + 1u) & 0xFFFFu);
     }
-    // Light samples l = l0, l0 + dl, ... < nLv of a shaded vertex (:252-265, :350-425) with MIS
-    // counts nLv, nBv.
-    PG3_DEV F3 LightSamples(const SceneView &S, const LightSpec &spec, const RngKey &k, uint32_t vlen, F3 sp,
-                            const Frame &fr, F3 wol, const DevMaterial &mat, uint32_t l0, uint32_t dl, uint32_t nLv,
-                            uint32_t nBv, const LightCache &lc, uint32_t &shadows) const {
-        F3 acc = mk(0.f);
-        if ((vlen + 1) < S.minPL) return acc;
-#pragma unroll 1
-        for (uint32_t l = l0; l < nLv; l += dl) {
-            float prob = 1.f;
-            const int32_t li = PickLight(spec, lc, RngGet(k, vlen, l, kDimLightPick), prob);
-            LightSample ls;
-            if (li >= 0 && SampleIllumination(S, spec, (uint32_t)li, sp, fr, mat.sides, RngGet(k, vlen, l, kDimLight0),
-                                              RngGet(k, vlen, l, kDimLight1), RngGet(k, vlen, l, kDimLight2), ls)) {
-                ls.prob = prob;
-                if (MaxC(ls.sample) > 0.f && (++shadows, !Occluded(S, sp, ls.wig, ls.dist))) {
-                    BsdfRec r;
-                    EvalBsdf(mat, ToLocal(fr, ls.wig), wol, r);
-                    if (ls.pdfW != kInf) {
-                        const float bsdfPdf = r.pdfW * r.compProb;
-                        const float lightPdf = ls.pdfW * ls.prob;
-                        acc += FastDiv(ls.sample * r.att * MisPowerN(lightPdf, nLv, bsdfPdf, nBv), lightPdf);
-                    } else {
-                        acc += FastDiv(ls.sample * r.att, ls.prob * nLv);
-                    }
-                }
-            }
-        }
-        return acc;
-    }
     // MIS-weighted emission that the BSDF sample of vertex len - 1 finds at vertex len
-    // (path_tracer.hxx:327-369)
     PG3_DEV F3 ParentEmission(const SceneView &S, const LightSpec &spec, F3 em, float lpdf, int32_t lid,
                               uint32_t nB) const {
-        if (IsZero(em) || lid < 0) return mk(0.f);
-        const uint32_t pnB = (len == 2u) ? nB : 1u;
-        if (!(pFlags & 1u)) return FastDiv(tAttCos * em, static_cast<float>(pnB) * pPdf);
-        float pick = 1.f;
-        if (spec.numLights == 0) {
-            pick = 0.f;
-        } else if (spec.numLights > 1) {
-            pick = (pLcSum > 0.f) ? EstimateContribution(S, spec, (uint32_t)lid, org, FrameFromZ(pN), pFlags >> 4,
-                                                         Branch(S, BranchOf(len - 1u, nB)), len - 1u) / pLcSum
-                                  : 1.f / spec.numLights;
-        }
-        const float lp = lpdf * pick;
-        return FastDiv(tAttCos * em * MisPowerN(pPdf, pnB, lp, LightCount(S, pnB)), pPdf);
-    }
-    PG3_DEV static F3 Indirect(F3 attCos, F3 R, float den, float iic) {
-        F3 ind = FastDiv(attCos * R, den);
-        if (iic > 0.f) ind = ClipProportionally(ind, iic);
-        return ind;
+        EmissionParent P;
+        P.sp = org;
+        P.n = pN;
+        P.attCos = tAttCos;
+        P.pdf = pPdf;
+        P.lcSum = pLcSum;
+        P.flags = pFlags;
+        P.nB = (len == 2u) ? nB : 1u;
+        P.nL = LightCount(S, P.nB);
+        P.len = len - 1u;
+        P.key = Branch(S, BranchOf(len - 1u, nB));
+        return EmissionMis(S, spec, em, lpdf, lid, P);
     }
     // End of a chain pass at wavefront step t.  The newest record is in registers and rChild is the
     // estimate of the vertex after it (0, or the folded suffix at a re-trace's target); the ring
@@ -1167,14 +1329,14 @@ struct SplitPath {
     // depth and returns false, else adds the first vertex's estimate to L and returns true.
     PG3_DEV bool Fold(const SceneView &S, uint32_t t, uint32_t jStart, F3 rChild) {
         F3 R = tA;
-        if (tDen > 0.f && !IsZero(rChild)) R += Indirect(tAttCos, rChild, tDen, S.iic);
+        if (tDen > 0.f && !IsZero(rChild)) R += ClippedIndirect(tAttCos, rChild, tDen, S.iic);
         const uint32_t n = min(nPush, kChainRing);
 #pragma unroll 1
         for (uint32_t i = 0; i < n; ++i) {
             const float *r = ring[(t - jStart - i) % kChainRing];
             F3 Rk = mk(r[0], r[1], r[2]);
             const float den = r[6];
-            if (den > 0.f && !IsZero(R)) Rk += Indirect(mk(r[3], r[4], r[5]), R, den, S.iic);
+            if (den > 0.f && !IsZero(R)) Rk += ClippedIndirect(mk(r[3], r[4], r[5]), R, den, S.iic);
             R = Rk;
         }
         if (nPush > kChainRing) {
@@ -1253,8 +1415,8 @@ struct SplitPath {
         lc.sum = 0.f;
         if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
         const uint32_t nBv = first ? nB : 1u;
-        tA = LightSamples(S, spec, k, len, sp, fr, wol, mat, first ? sub : 0u, nBv, LightCount(S, nBv), nBv, lc,
-                          tally.shadow);
+        tA = LightSampleSum(S, spec, k, len, sp, fr, wol, mat, first ? sub : 0u, nBv, LightCount(S, nBv), nBv, lc,
+                            tally.shadow);
         const float rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
         const uint32_t si = first ? sub : 0u;           // :281-404, BSDF sample si of this vertex
         const bool cut = (S.maxPL == 0) && (RngGet(k, len, si, kDimRR) > rr);

@@ -383,6 +383,93 @@ def test_splitting_and_clipping_parity(ctx, params):
     assert_image_parity(g, o, frac=0.03)
 
 
+@pytest.mark.parametrize("scene_id", [33, 35, 36])
+def test_tree_walker_keeps_the_1200_vertex_limit(ctx, scene_id):
+    """Splitting below the first vertex (-sb 4 -sl 0.5: three BSDF samples at the first vertex,
+    then chains) under Russian roulette on the reference scenes with a GGX dielectric sphere, where
+    paths bounce inside the sphere for many vertices: the general tree walker against the oracle,
+    which cuts paths at the reference's 1200 vertices (path_tracer.hxx:232-238)."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.reference(scene_id, resolution=24)
+    sb.set_params(algorithm="pt", splitting_budget=4.0, splitting_level=0.5, indirect_clipping=16.0,
+                  max_path_length=0, seed=13)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+
+
+def closed_white_box(albedo=0.99, res=16):
+    """A closed box of Lambert walls of albedo 0.99 (Russian roulette keeps a path with probability
+    0.99 per vertex: chains of ~100 vertices, far beyond the 32 fold records a lane keeps) lit by a
+    small one-sided area light under the ceiling; the camera is inside."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], np.float32)
+    faces = [(0, 1, 2, 3), (4, 7, 6, 5), (0, 4, 5, 1), (1, 5, 6, 2), (2, 6, 7, 3), (3, 7, 4, 0)]   # inward
+    tris = []
+    for a, b, c, d in faces:
+        tris += [np.concatenate([v[a], v[b], v[c]]), np.concatenate([v[a], v[c], v[d]])]
+    # light quad just below the ceiling, facing down (-z)
+    l0, l1, l2, l3 = (0.4, 0.4, 0.99), (0.6, 0.4, 0.99), (0.6, 0.6, 0.99), (0.4, 0.6, 0.99)
+    tris += [np.array(l0 + l2 + l1, np.float32), np.array(l0 + l3 + l2, np.float32)]
+    mats = [material(_ffi.MAT_LAMBERT, diffuse=(albedo,) * 3),
+            material(_ffi.MAT_LAMBERT, diffuse=(0.0, 0.0, 0.0), light_id=0),
+            material(_ffi.MAT_LAMBERT, diffuse=(0.0, 0.0, 0.0), light_id=1)]
+    lights = [light(_ffi.LIGHT_AREA, p0=l0, p1=l2, p2=l1, power=(2.0, 2.0, 2.0)),
+              light(_ffi.LIGHT_AREA, p0=l0, p1=l3, p2=l2, power=(2.0, 2.0, 2.0))]
+    return CustomScene(triangles=tris, triangle_mat=[0] * 12 + [1, 2], materials=mats, lights=lights,
+                       camera={"position": (0.5, 0.05, 0.5), "forward": (0.0, 1.0, 0.0), "res": (res, res)})
+
+
+def _box_image(ctx, first, n, res=16, **params):
+    sc = closed_white_box(res=res)
+    sc.set_params(algorithm="pt", max_path_length=0, seed=17, **params)
+    ctx.upload(sc)
+    ctx.render(first, n)
+    return ctx.framebuffer(0, sc.resolution)[0], sc
+
+
+@pytest.mark.parametrize("params", [
+    pytest.param(dict(splitting_budget=1.0, indirect_clipping=1e30), id="split_path"),
+    pytest.param(dict(splitting_budget=1.2, indirect_clipping=1e30), id="tree_walker_chain")])
+def test_deep_chain_fold_equals_the_forward_chain(ctx, params):
+    """Chains of ~100 vertices, far beyond the 32 fold records a lane keeps: with a clipping
+    threshold no estimate reaches, the split path (-sb 1, one sub-sample) and the tree walker's
+    chain (-sb 1.2: one BSDF sample per vertex) compute the forward chain's estimator on the same
+    RNG streams (branch 0), so their folds -- with every re-trace of the deep chains -- must give the
+    chain kernel's image up to FP32 summation order."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    a, _ = _box_image(ctx, 0, 4, splitting_budget=1.0, indirect_clipping=0.0)
+    b, _ = _box_image(ctx, 0, 4, **params)
+    rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-6)
+    print(f"max per-channel relative difference {rel.max():.2e}")
+    assert rel.max() < 1e-4
+    assert abs(a.sum() - b.sum()) / a.sum() < 1e-6
+
+
+@pytest.mark.parametrize("params", [
+    pytest.param(dict(splitting_budget=4.0, indirect_clipping=16.0), id="split_first_vertex"),
+    pytest.param(dict(splitting_budget=4.0, splitting_level=0.5, indirect_clipping=16.0), id="tree_walker")])
+def test_deep_chains_agree_with_the_oracle(ctx, params):
+    """The reference's defaults (splitting, clipping, Russian roulette) in the closed box: the GPU's
+    mean radiance over 8192 samples per pixel against the oracle's over 512 others, each with
+    its batch standard error (after ~100 bounces the two sides' paths part ways at ulp-level
+    differences, so the same samples do not give the same pixels; their expectations must agree).
+    A path cut at 48 vertices would lose 40 % of the box's radiance."""
+    from oracle import oracle as O
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    g = []
+    for b in range(32):
+        img, sc = _box_image(ctx, b * 256, 256, res=8, **params)
+        g.append(img.sum() / 256)
+    orc = O.OracleScene(sc)
+    o = [orc.render(1000000 + b * 32, 32, threads=0)[0].sum() / 32 for b in range(16)]
+    g, o = np.array(g), np.array(o)
+    se = np.sqrt(g.var(ddof=1) / len(g) + o.var(ddof=1) / len(o))
+    z = (g.mean() - o.mean()) / se
+    print(f"mean image sum gpu {g.mean():.6g} oracle {o.mean():.6g}: relative {g.mean() / o.mean() - 1:+.2e} "
+          f"+- {se / o.mean():.1e} (z {z:+.2f})")
+    assert abs(z) < 4.5
+    assert se / o.mean() < 0.02
+
+
 @pytest.mark.parametrize("scene_id", [3, 11, 18, 32, 34, 37])
 def test_reference_defaults_on_reference_scenes(ctx, scene_id):
     """The reference's command-line defaults (-sb 4 -iic 16, Russian roulette: first-vertex
