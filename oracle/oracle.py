@@ -57,6 +57,8 @@ def load():
         "oracle_latlong2dir": (None, [C.c_float, C.c_float, _FP]),
         "oracle_dir2latlong": (None, [_FP, _FP]),
         "oracle_sample_tri_bilinear": (None, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _FP]),
+        "oracle_sample_tri_st": (None, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _FP]),
+        "oracle_steerable_pdf_parts": (None, [C.c_void_p, _FP, C.c_uint32, _FP, _FP]),
         "oracle_dist1d": (None, [_FP, C.c_uint32, C.c_uint32, _FP, _FP, _U32P, _FP, _FP, _FP]),
         "oracle_steerable_info": (None, [C.c_void_p, _U32P, _U32P]),
         "oracle_steerable_leaves": (None, [C.c_void_p, _FP]),
@@ -169,6 +171,13 @@ class OracleScene:
         self.lib.oracle_steerable_root_weight(self.ptr, fp(w))
         return leaves, int(nv.value), w
 
+    def steerable_pdf_parts(self, normal, dirs):
+        """(GetBasePdf(d), GetBasePdf(-d), ComputeWholeIntegral) per direction for a normal's lobe."""
+        dirs = f32(dirs)
+        out = np.zeros((len(dirs), 3), np.float32)
+        self.lib.oracle_steerable_pdf_parts(self.ptr, fp(f32(normal)), len(dirs), fp(dirs), fp(out))
+        return out
+
     def steerable_pick(self, normal, u):
         u = f32(u)
         leaf = np.zeros(len(u), np.uint32)
@@ -258,6 +267,10 @@ def dir2latlong(d):
 
 def sample_tri_bilinear(ux, uy, a, b, c):
     o = np.zeros(2, np.float32); load().oracle_sample_tri_bilinear(ux, uy, a, b, c, fp(o)); return o
+
+
+def sample_tri_st(ux, uy, a, b, c):
+    o = np.zeros(2, np.float32); load().oracle_sample_tri_st(ux, uy, a, b, c, fp(o)); return o
 
 
 def dist1d(f, u):

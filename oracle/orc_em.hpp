@@ -450,16 +450,21 @@ struct SteerableSampler {
         }
         return x;
     }
-    static void SampleTriBilinear(float ux, float uy, float va, float vb, float vc, float &b0, float &b1) {
+    // s (SampleTriangleFFunction) and t (SampleTriangleGFunction) of the barycentric sample
+    static void SampleTriST(float ux, float uy, float va, float vb, float vc, float &s, float &t) {
         const float x = (vb - va) / 3 + (vc - vb) / 6;
         const float y = va / 2;
         const float alpha = x / (x + y), beta = y / (x + y);
-        const float s = NewtonCubic(alpha, beta, 0, -ux);
+        s = NewtonCubic(alpha, beta, 0, -ux);
         const float t0 = (s * (vc - vb)) + (2 * (1 - s) * va) + (s * vb);
         const float gamma = s * (vc - vb) / t0;
         const float rho = 2 * ((1 - s) * va + s * vb) / t0;
         const float discr = rho * rho + 4 * gamma * uy;
-        const float t = (2 * uy) / (rho + SafeSqrt(discr));
+        t = (2 * uy) / (rho + SafeSqrt(discr));
+    }
+    static void SampleTriBilinear(float ux, float uy, float va, float vb, float vc, float &b0, float &b1) {
+        float s, t;
+        SampleTriST(ux, uy, va, vb, vc, s, t);
         b0 = float(1 - s);
         b1 = float(s * (1 - t));
     }

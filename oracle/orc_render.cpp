@@ -823,6 +823,10 @@ void oracle_dir2latlong(const float *d, float *uv) { const V2 r = Dir2LatLong(V3
 void oracle_sample_tri_bilinear(float ux, float uy, float a, float b, float c, float *bary2) {
     SteerableSampler::SampleTriBilinear(ux, uy, a, b, c, bary2[0], bary2[1]);
 }
+// s and t of SampleTriangleBilinear (em_steerable_sampler.hxx:3709-3747, 3882-3905)
+void oracle_sample_tri_st(float ux, float uy, float a, float b, float c, float *st) {
+    SteerableSampler::SampleTriST(ux, uy, a, b, c, st[0], st[1]);
+}
 // 1-D distribution: builds from f (n), samples m uniforms
 void oracle_dist1d(const float *f, uint32_t n, uint32_t m, const float *u, float *x, uint32_t *seg, float *pdf,
                    float *cdf_out, float *cdf_f32_out) {
@@ -847,6 +851,19 @@ void oracle_steerable_root_weight(void *sp, float *w9) {
     const Scene &S = *static_cast<Scene *>(sp);
     const SH9 w = S.steer.root->leaf ? S.steer.TriWeight(S.steer.root->tri) : SteerableSampler::NodeWeight(*S.steer.root);
     for (int i = 0; i < 9; ++i) w9[i] = w[i];
+}
+// Parts of the steerable sampler's pdf for the clamped-cosine lobe of `normal`: GetBasePdf of dir
+// and of -dir and ComputeWholeIntegral (em_steerable_sampler.hxx:2387-2432), per direction.
+void oracle_steerable_pdf_parts(void *sp, const float *normal, uint32_t m, const float *dirs, float *out3) {
+    const Scene &S = *static_cast<Scene *>(sp);
+    const SH9 c = ClampedCosCoeffs(V3(normal[0], normal[1], normal[2]), true);
+    const float whole = S.steer.WholeIntegral(c);
+    for (uint32_t i = 0; i < m; ++i) {
+        const V3 d(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]);
+        out3[i * 3] = S.steer.BasePdf(d, c);
+        out3[i * 3 + 1] = S.steer.BasePdf(-d, c);
+        out3[i * 3 + 2] = whole;
+    }
 }
 // Samples triangle leaves via PickTriangle for a normal; returns leaf index per uniform (for the
 // picking-frequency test, em_steerable_sampler.hxx:4778-4925)

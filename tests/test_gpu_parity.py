@@ -25,6 +25,12 @@ def ctx():
     c.close()
 
 
+def tier1_report(what, e):
+    """Prints the maximum relative error and how many samples exceed the 1e-5 bound."""
+    e = np.asarray(e, np.float64).ravel()
+    print(f"{what}: max {e.max():.2e}, q999 {np.quantile(e, 0.999):.2e}, above 1e-5: {int(np.sum(e > REL))} of {e.size}")
+
+
 def rel_err(a, b, floor):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
@@ -66,6 +72,8 @@ def test_bsdf_eval_parity(ctx, name, mat, both):
     ef = rel_err(fg, fo, scale_f)
     ep = rel_err(pg, po, max(1e-3, float(np.abs(po).max()) * 1e-6))
     assert np.all(cg == co)
+    tier1_report(f"{name} eval f", ef)
+    tier1_report(f"{name} eval pdf", ep)
     assert np.quantile(ef, 0.999) <= REL and np.quantile(ep, 0.999) <= REL, (ef.max(), ep.max())
     assert ef.max() <= 1e-3 and ep.max() <= 1e-3
 
@@ -94,6 +102,7 @@ def test_bsdf_sample_parity(ctx, name, mat, both):
     flips = ew > 1e-3
     q_warp = float(np.quantile(ew[~flips], 0.999))
     stats = f"{name}: flips {flips.mean():.2e}, warp q999 {q_warp:.2e}"
+    tier1_report(f"{name} sample direction (no branch flip)", ew[~flips])
     assert flips.mean() < 5e-3, stats
     assert q_warp <= REL, stats
     fe, pe, ce = O.bsdf_eval(mat, wg, wol)
@@ -102,6 +111,8 @@ def test_bsdf_sample_parity(ctx, name, mat, both):
         # value at the GPU-sampled direction
         ef = rel_err(fg[live], fe[live], 1e-30).max(axis=1)
         ep = rel_err(pg[live], pe[live], 1e-30)
+        tier1_report(f"{name} sampled value at the sampled direction", ef)
+        tier1_report(f"{name} sampled pdf at the sampled direction", ep)
         stats += f", value-at-direction q999 f {np.quantile(ef, 0.999):.2e} pdf {np.quantile(ep, 0.999):.2e}"
         if mat.type in (_ffi.MAT_LAMBERT, _ffi.MAT_PHONG):
             # SampleBsdf returns EvalBsdf at the sampled direction (materials.hxx:285-292, 494-503)
@@ -152,6 +163,7 @@ def test_em_parity(ctx, sampler):
     # evaluation
     d = dirs(rng, n, True)
     eg, eo = ctx.em_eval(d), orc.em_eval(d)
+    tier1_report(f"sampler {sampler} EM eval", rel_err(eg, eo, 1e-3))
     assert np.quantile(rel_err(eg, eo, 1e-3), 0.999) <= REL
     # sampling
     dg, pg, rg, okg = ctx.em_sample(normals, sides, u)
@@ -161,6 +173,8 @@ def test_em_parity(ctx, sampler):
     ed = np.abs(dg - do).max(axis=1)[good]
     ep = rel_err(pg, po, 1e-6)[good]
     er = rel_err(rg, ro, 1e-3).max(axis=1)[good]
+    for nm, e in (("direction", ed), ("pdf", ep), ("radiance*cos", er)):
+        tier1_report(f"sampler {sampler} EM sample {nm}", e)
     for e in (ed, ep, er):
         assert np.quantile(e, 0.995) <= REL * 2, e.max()
         assert np.mean(e > 1e-3) < 2e-3
@@ -168,6 +182,7 @@ def test_em_parity(ctx, sampler):
     pg2 = ctx.em_pdf(d, normals, sides)
     po2 = orc.em_pdf(d, normals, sides)
     e = rel_err(pg2, po2, 1e-6)
+    tier1_report(f"sampler {sampler} EM pdf", e)
     assert np.quantile(e, 0.999) <= REL * 2 and np.mean(e > 1e-3) < 1e-3
 
 

@@ -551,3 +551,325 @@ def test_oracle_occluded_against_fp64_ray_casting():
     assert clear.mean() > 0.99
     assert np.array_equal(occ[clear], np.isfinite(t64)[clear])
     assert 0.1 < occ.mean() < 0.8
+
+
+# ---------------------------------------------------------------------------------------------
+# Pins of the functions no reference vector covers: the triangle sampler's conditional part, the
+# steerable pdf, the adaptive refinement, the EM pole wrap, splitting and clipping.
+
+def test_triangle_g_function_solves_its_quadratic():
+    """SampleTriangleGFunction (em_steerable_sampler.hxx:3728-3747) returns the root in [0, 1] of
+    gamma t^2 + rho t = u with gamma = s (c - b) / n, rho = 2 ((1 - s) a + s b) / n and the
+    reference's normaliser n = s (c - b) + 2 (1 - s) a + s b (restated here from the reference in
+    FP64).  A wrong sign, a dropped factor 2 or the other root fails."""
+    rng = np.random.default_rng(21)
+    worst = 0.0
+    for a, b, c in ((0.3, 1.0, 2.5), (2.0, 0.4, 0.1), (1.0, 1.0, 1.0), (0.05, 3.0, 0.7)):
+        for _ in range(400):
+            ux, uy = rng.random(2)
+            s, t = (float(v) for v in O.sample_tri_st(float(ux), float(uy), a, b, c))
+            n = s * (c - b) + 2 * (1 - s) * a + s * b
+            gamma, rho = s * (c - b) / n, 2 * ((1 - s) * a + s * b) / n
+            res = gamma * t * t + rho * t - float(np.float32(uy))
+            worst = max(worst, abs(res))
+            assert -1e-6 <= t <= 1.5
+    print(f"max |gamma t^2 + rho t - u| = {worst:.2e}")
+    assert worst < 2e-6
+
+
+def test_triangle_g_function_closed_forms():
+    """Two closed forms of SampleTriangleGFunction: with a = b = c the quadratic is linear and
+    t = u (2 - s) / 2 (the reference normaliser's missing s*b term makes this differ from the
+    uniform t = u); with b = 0 the normaliser is exact, so t follows the conditional density
+    proportional to (1 - s) a + s t c on [0, 1] (KS statistic against its CDF)."""
+    rng = np.random.default_rng(22)
+    for _ in range(500):
+        ux, uy = rng.random(2)
+        s, t = (float(v) for v in O.sample_tri_st(float(ux), float(uy), 0.7, 0.7, 0.7))
+        assert abs(t - float(np.float32(uy)) * (2 - s) / 2) < 2e-6
+    a, c = 0.4, 2.0
+    for ux in (0.2, 0.5, 0.9):
+        uys = rng.random(20000)
+        st = np.array([O.sample_tri_st(float(ux), float(q), a, 0.0, c) for q in uys])
+        s = float(st[0, 0])
+        assert np.allclose(st[:, 0], s)
+        t = np.sort(st[:, 1])
+        A, B = (1 - s) * a, s * c
+        cdf = (A * t + B * t * t / 2) / (A + B / 2)
+        ks = np.max(np.abs(cdf - np.arange(1, len(t) + 1) / len(t)))
+        assert ks < 0.015, (ux, ks)
+
+
+def _steer_scene(img, levels=(3, 3), max_error=5.0):
+    return CustomScene(materials=[material(_ffi.MAT_LAMBERT, diffuse=(0.5, 0.5, 0.5))],
+                       lights=[light(_ffi.LIGHT_BACKGROUND)], em=img, em_sampler=_ffi.EM_STEERABLE,
+                       steerable_levels=levels, steerable_max_error=max_error, camera={"res": (8, 8)})
+
+
+def _clamped_cos_values(dirs, normal):
+    """Dot(GenerateSphHarm(d), GenerateForClampedCos(n, compensate)) from the KAT-pinned pieces."""
+    cc = O.clamped_cos(normal, True)
+    return np.array([float(O.sh_basis(d) @ cc) for d in dirs])
+
+
+def _sphere_dirs(rng, n):
+    u = rng.random((n, 2))
+    z = 1 - 2 * u[:, 1]
+    r = np.sqrt(np.maximum(0.0, 1 - z * z))
+    return np.stack([r * np.cos(2 * np.pi * u[:, 0]), r * np.sin(2 * np.pi * u[:, 0]), z], 1).astype(np.float32)
+
+
+def _whole_integral_from_leaves(o, normal, em_lum):
+    """ComputeWholeIntegral recomputed from the leaf triangles: sum over leaves of planar area x
+    mean over its vertices of EM luminance x clamped cosine (TriangleNode::ComputeWeight,
+    em_steerable_sampler.hxx:1282-1301, with vertex weights GenerateSphHarm(dir, luminance))."""
+    leaves, _, _ = o.steerable()
+    area = 0.5 * np.linalg.norm(np.cross(leaves[:, 1] - leaves[:, 0], leaves[:, 2] - leaves[:, 0]), axis=1)
+    verts = leaves.reshape(-1, 3)
+    vals = (em_lum(verts) * _clamped_cos_values(verts, normal)).reshape(-1, 3)
+    return float(np.sum(area * vals.mean(axis=1)))
+
+
+def test_steerable_pdf_constant_em_closed_form():
+    """For a constant EM (luminance L) GetBasePdf(d) = L cc(d) / W, so the hemisphere pdf
+    (PdfWHemisphere, em_steerable_sampler.hxx:2281-2299, with the flipped direction's base pdf,
+    :2379-2413) is L (cc(d) + cc(-d)) / W above the horizon and 0 below, and integrates over the
+    hemisphere to L c0 Y00 4 pi / W (only the constant SH band survives the sphere integral); with
+    both sides sampled each side gets half.  W recomputed from the leaf triangles."""
+    img = np.ones((16, 32, 3), np.float32)
+    lum = 0.212671 + 0.715160 + 0.072169
+    o = O.OracleScene(_steer_scene(img))
+    rng = np.random.default_rng(23)
+    d = _sphere_dirs(rng, 4000)
+    for normal in (np.array([0, 0, 1], np.float32), direction(1.1, 0.4), direction(2.5, -1.0)):
+        W = _whole_integral_from_leaves(o, normal, lambda v: np.full(len(v), lum))
+        ns = np.tile(normal, (len(d), 1))
+        pdf = o.em_pdf(d, ns, np.ones(len(d), np.int32))
+        up = d @ normal >= 0
+        want = lum * (_clamped_cos_values(d, normal) + _clamped_cos_values(-d, normal)) / W
+        err = np.abs(pdf[up] - want[up]) / np.abs(want[up]).max()
+        print(f"normal {np.round(normal, 3)}: max rel error {err.max():.2e}")
+        assert err.max() < 2e-5
+        assert np.all(pdf[~up] == 0)
+        cc0 = float(O.clamped_cos(normal, True)[0])
+        total = lum * cc0 * 0.282095 * 4 * math.pi / W
+        big = _sphere_dirs(rng, 200000)
+        p_up = o.em_pdf(big, np.tile(normal, (len(big), 1)), np.ones(len(big), np.int32))
+        assert abs(p_up.mean() * 4 * math.pi - total) < 0.01 * total
+        assert abs(total - 1.0) < 0.02           # the planar triangulation of the sphere is close
+        p_both = o.em_pdf(big, np.tile(normal, (len(big), 1)), np.full(len(big), 3, np.int32))
+        assert abs(p_both.mean() * 4 * math.pi - total) < 0.01 * total
+
+
+def test_steerable_pdf_parts_on_a_textured_em():
+    """A textured EM: ComputeWholeIntegral equals the sum recomputed from the leaves, GetBasePdf(d)
+    = lum(EM(d)) cc(d) / W with the EM evaluated by the (pinned) tent filter, and the sampler pdf is
+    GetBasePdf(d) + GetBasePdf(-d) above the horizon, weighted by the side's integral share when
+    both sides are sampled (PdfW, em_steerable_sampler.hxx:2220-2278)."""
+    h, w = 16, 32
+    yy, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    img = 0.2 + 0.6 * (1 - yy / h)[..., None] * np.array([0.6, 0.8, 1.0])
+    img = (img + 30.0 * np.exp(-((xx - 9) ** 2 + (yy - 5) ** 2) / 3.0)[..., None]).astype(np.float32)
+    o = O.OracleScene(_steer_scene(img))
+    em_lum = lambda v: o.em_eval(v) @ np.array([0.212671, 0.715160, 0.072169])   # noqa: E731
+    rng = np.random.default_rng(24)
+    d = _sphere_dirs(rng, 3000)
+    for normal in (direction(0.4, 0.3), direction(2.0, 2.2)):
+        parts = o.steerable_pdf_parts(normal, d)
+        W = _whole_integral_from_leaves(o, normal, em_lum)
+        assert abs(parts[0, 2] - W) < 1e-4 * W
+        b1 = em_lum(d) * _clamped_cos_values(d, normal) / parts[0, 2]
+        b2 = em_lum(-d) * _clamped_cos_values(-d, normal) / parts[0, 2]
+        scale = max(np.abs(b1).max(), 1e-12)
+        assert np.max(np.abs(parts[:, 0] - b1)) < 1e-5 * scale
+        assert np.max(np.abs(parts[:, 1] - b2)) < 1e-5 * scale
+        ns = np.tile(normal, (len(d), 1))
+        up = d @ normal >= 0
+        front = o.em_pdf(d, ns, np.ones(len(d), np.int32))
+        assert np.allclose(front[up], (parts[:, 0] + parts[:, 1])[up], rtol=1e-5, atol=1e-7 * scale)
+        assert np.all(front[~up] == 0)
+        lower = o.steerable_pdf_parts(-normal, d)
+        ui, li = parts[0, 2], lower[0, 2]
+        both = o.em_pdf(d, ns, np.full(len(d), 3, np.int32))
+        want = np.where(up, (parts[:, 0] + parts[:, 1]) * ui / (ui + li), (lower[:, 0] + lower[:, 1]) * li / (ui + li))
+        assert np.allclose(both, want, rtol=1e-5, atol=1e-7 * scale)
+
+
+def _leaf_levels(leaves):
+    area = 0.5 * np.linalg.norm(np.cross(leaves[:, 1] - leaves[:, 0], leaves[:, 2] - leaves[:, 0]), axis=1)
+    a0 = 4 * math.pi / 20
+    return np.rint(np.log(a0 / area) / np.log(4)).astype(int), area
+
+
+def _leaf_set(leaves):
+    cent = leaves.mean(axis=1)
+    return {tuple(np.round(c, 4)) for c in cent}
+
+
+def test_steerable_refinement_thresholds():
+    """The refinement test (TriangleHasToBeSubdivided / IsEstimationErrorTooLarge,
+    em_steerable_sampler.hxx:3160-3433) splits a triangle when a test sample of the EM differs from
+    its linear approximation by more than a factor 1 + maxError, or, whatever the error, when the
+    triangle would need more than 50 test samples per dimension (:3327-3331; near the poles an EM
+    pixel is narrow in azimuth).  The decision is monotone in the threshold and a split produces the
+    same children whatever the threshold, so: the leaf count never grows as the error grows; an
+    error of ~0 on a noisy map refines every triangle to the maximum level; and with a huge error
+    only the geometric rule acts, so the leaves do not depend on the map's content and every leaf
+    finer than the minimum level lies near a pole."""
+    rng = np.random.default_rng(25)
+    img = (0.2 + rng.random((16, 32, 3)) ** 2 * 4).astype(np.float32)
+    counts = []
+    for err in (1e-6, 0.05, 0.3, 1.0, 5.0, 1e9):
+        leaves, _, _ = O.OracleScene(_steer_scene(img, levels=(1, 4), max_error=err)).steerable()
+        counts.append(len(leaves))
+    print("leaf counts for max error 1e-6 .. 1e9:", counts)
+    assert all(a >= b for a, b in zip(counts, counts[1:]))
+    assert counts[0] == 20 * 4 ** 4
+    assert counts[2] > counts[-1]
+    huge, _, _ = O.OracleScene(_steer_scene(img, levels=(1, 4), max_error=1e9)).steerable()
+    const, _, _ = O.OracleScene(_steer_scene(np.ones_like(img), levels=(1, 4), max_error=1e9)).steerable()
+    assert _leaf_set(huge) == _leaf_set(const)
+    levels, _ = _leaf_levels(huge)
+    fine = levels > 1
+    cent = huge.mean(axis=1)
+    cent /= np.linalg.norm(cent, axis=1, keepdims=True)
+    assert fine.any() and np.all(np.abs(cent[fine, 2]) > 0.8)
+    assert np.all(levels[np.abs(cent[:, 2]) < 0.5] == 1)
+
+
+def test_steerable_refinement_stays_near_a_bright_texel():
+    """One bright texel on a constant map: the linear approximation is exact wherever neither a
+    triangle's vertices nor its test samples see the texel's tent footprint, so every leaf the
+    bright map refines beyond the constant map's triangulation (same parameters) lies near the
+    texel, and there are such leaves."""
+    h, w = 16, 32
+    img = np.ones((h, w, 3), np.float32)
+    tx, ty = 20, 6
+    img[ty, tx] = 50.0
+    leaves, _, _ = O.OracleScene(_steer_scene(img, levels=(2, 5), max_error=0.5)).steerable()
+    base, _, _ = O.OracleScene(_steer_scene(np.ones_like(img), levels=(2, 5), max_error=0.5)).steerable()
+    base_set = _leaf_set(base)
+    extra = np.array([tuple(np.round(c, 4)) not in base_set for c in leaves.mean(axis=1)])
+    centre = O.latlong2dir((tx + 0.5) / w, (ty + 0.5) / h)
+    cent = leaves.mean(axis=1)
+    cent /= np.linalg.norm(cent, axis=1, keepdims=True)
+    ang = np.degrees(np.arccos(np.clip(cent @ centre, -1, 1)))
+    # level-2 triangles span ~16 degrees, a texel's tent footprint ~11 x 22 degrees
+    print(f"{extra.sum()} leaves refined for the texel, farthest {ang[extra].max():.1f} degrees from it")
+    assert extra.sum() >= 4
+    assert ang[extra].max() < 45.0
+
+
+def _tent_reference(img, u, v):
+    """EnvironmentMapImage::Evaluate with the tent filter (em_image.hxx:95-142) and its coordinate
+    wrap (NormalizeImgCoords, :245-278) restated from the reference: a row beyond a pole wraps to
+    the pole row shifted by half the width; the four texels are taken as (x0, y0), (x1, y0),
+    (x0, y1), (x1, y1) of the two normalised corners, and bilinearly interpolated (Math::Bilerp)."""
+    h, w, _ = img.shape
+    xf = np.float32(np.float32(u) * np.float32(w))
+    yf = np.float32(np.float32(v) * np.float32(h))
+    cx, cy = int(np.float32(xf + np.float32(0.5))), int(np.float32(yf + np.float32(0.5)))
+    x0, y0 = cx - 1, cy - 1
+    xl = np.float32(xf - np.float32(x0 + 0.5))
+    yl = np.float32(yf - np.float32(y0 + 0.5))
+
+    def norm(x, y):
+        if y >= h:
+            x, y = x + w // 2, h - 1
+        elif y < 0:
+            x, y = x + w // 2, 0
+        return x % w, y
+    (ax, ay), (bx, by) = norm(x0, y0), norm(cx, cy)
+    lerp = lambda c, p, q: (np.float32(1) - c) * p + c * q   # noqa: E731
+    return lerp(yl, lerp(xl, img[ay, ax], img[ay, bx]), lerp(xl, img[by, ax], img[by, bx]))
+
+
+def test_em_tent_filter_pole_and_meridian_wrap():
+    """Directions within half a texel of a pole and of the prime meridian, where the tent filter
+    reaches across the image edge: the oracle equals the restatement above (a version that clamps
+    the row, or shifts both corners by half the width, fails)."""
+    rng = np.random.default_rng(26)
+    h, w = 16, 32
+    img = (rng.random((h, w, 3)) * np.array([1.0, 2.0, 3.0])).astype(np.float32)
+    o = O.OracleScene(_em_scene(img))
+    n = 600
+    u = rng.random(n).astype(np.float32)
+    v = np.concatenate([rng.random(n // 3) * 0.5 / h, 1 - rng.random(n // 3) * 0.5 / h,
+                        rng.random(n - 2 * (n // 3))]).astype(np.float32)
+    u[-(n // 3):] = np.where(rng.random(n // 3) < 0.5, rng.random(n // 3) * 0.5 / w,
+                             1 - rng.random(n // 3) * 0.5 / w).astype(np.float32)
+    got = o.em_eval_uv(np.stack([u, v], 1))
+    want = np.array([_tent_reference(img, a, b) for a, b in zip(u, v)])
+    err = np.abs(got - want).max()
+    print(f"max abs error {err:.2e} over {n} uv near the poles and the meridian")
+    assert err < 1e-5
+
+
+# ---- splitting and clipping (path_tracer.hxx:240-454) ------------------------------------------
+def _split_box(**params):
+    sb = SceneBuild.workload(1, seed=41, res=24)
+    sb.set_params(algorithm="pt", **params)
+    return sb
+
+
+def test_clipping_threshold_no_term_reaches_changes_nothing():
+    """ClipProportionally scales an indirect term only when its maximum exceeds -iic
+    (path_tracer.hxx:400-402, math.hxx:538-549): a threshold no term reaches renders the same
+    floats as no clipping, with splitting and Russian roulette."""
+    a, _ = O.OracleScene(_split_box(splitting_budget=4.0, indirect_clipping=0.0, max_path_length=0)).render(0, 4, threads=8)
+    b, _ = O.OracleScene(_split_box(splitting_budget=4.0, indirect_clipping=1e30, max_path_length=0)).render(0, 4, threads=8)
+    assert np.array_equal(a, b)
+
+
+def test_tiny_clipping_keeps_only_the_first_vertex_terms():
+    """-iic 1e-7 clips every indirect term of the first vertex to at most 1e-7 per channel, while the
+    first vertex's light samples and the emission its BSDF samples find are not clipped: the image
+    equals the -maxpl 2 image (which stops at the second vertex) of the same samples to within
+    spp x nB x 1e-7 per pixel and channel."""
+    spp, nB = 4, 4
+    a, _ = O.OracleScene(_split_box(splitting_budget=4.0, indirect_clipping=1e-7, max_path_length=0)).render(0, spp, threads=8)
+    b, _ = O.OracleScene(_split_box(splitting_budget=4.0, indirect_clipping=0.0, max_path_length=2)).render(0, spp, threads=8)
+    assert np.allclose(a, b, rtol=2e-6, atol=1.5 * spp * nB * 1e-7)
+    c, _ = O.OracleScene(_split_box(splitting_budget=4.0, indirect_clipping=0.0, max_path_length=0)).render(0, spp, threads=8)
+    assert c.sum() > 1.2 * a.sum()    # the indirect terms that were clipped carry light
+
+
+@pytest.mark.parametrize("params", [dict(splitting_budget=4.0, splitting_level=0.5, light_to_bsdf_ratio=2.0),
+                                    dict(splitting_budget=8.0, light_to_bsdf_ratio=0.5),
+                                    dict(splitting_budget=3.0, indirect_clipping=16.0)])
+def test_white_furnace_with_splitting(params):
+    """The convex Lambert sphere (albedo 0.5) under constant unit radiance reflects exactly 0.5
+    under any splitting budget, level and light-to-BSDF ratio (ComputeSplittingCounts and the MIS
+    sample counts, path_tracer.hxx:252-454): a missing 1/nB, a wrong light count or MIS count
+    changes the sphere's brightness.  Clipping above every term (all radiance <= 1) changes nothing."""
+    sc = CustomScene(spheres=[[0, 0, 0, 1]], sphere_mat=[0],
+                     materials=[material(_ffi.MAT_LAMBERT, diffuse=(0.5, 0.5, 0.5))],
+                     lights=[light(_ffi.LIGHT_BACKGROUND, power=(1, 1, 1))],
+                     camera={"position": (0, -4, 0), "forward": (0, 1, 0), "up": (0, 0, 1), "hfov": 40,
+                             "res": (16, 16)},
+                     params=dict(max_path_length=0, indirect_clipping=params.get("indirect_clipping", 0.0),
+                                 **{k: v for k, v in params.items() if k != "indirect_clipping"}))
+    img, _ = O.OracleScene(sc).render(0, 256, threads=8)
+    img /= 256
+    centre = img[5:11, 5:11]
+    print(f"{params}: sphere centre mean {centre.mean():.5f}")
+    assert abs(centre.mean() - 0.5) < 0.006
+    assert np.allclose(img[0, 0], 1.0)
+
+
+def test_splitting_configurations_converge_to_one_image():
+    """Splitting changes the estimator's variance, not its expectation (without clipping): the box
+    rendered with -sb 4 -sl 0.5 -slbr 2 and with -sb 1 agree block by block within noise."""
+    ref = _split_box(splitting_budget=1.0, indirect_clipping=0.0, max_path_length=4)
+    spl = _split_box(splitting_budget=4.0, splitting_level=0.5, light_to_bsdf_ratio=2.0, indirect_clipping=0.0,
+                     max_path_length=4, seed=77)
+    a, _ = O.OracleScene(ref).render(0, 1024, threads=8)
+    b, _ = O.OracleScene(spl).render(0, 512, threads=8)
+    a /= 1024
+    b /= 512
+    ba = a.reshape(4, 6, 4, 6, 3).mean(axis=(1, 3))
+    bb = b.reshape(4, 6, 4, 6, 3).mean(axis=(1, 3))
+    rel = np.abs(ba - bb) / np.maximum(ba, 1e-3)
+    print(f"block means: median relative difference {np.median(rel):.3f}, image {b.mean() / a.mean() - 1:+.4f}")
+    assert np.median(rel) < 0.03
+    assert abs(a.mean() - b.mean()) / a.mean() < 0.015
