@@ -58,12 +58,24 @@ struct DevEm {
     float rootW[9];
 };
 
+// Material classes of the wavefront pipeline's shading stage, one kernel each: diffuse (Lambert,
+// Phong), GGX conductor, GGX dielectric, Dirac (smooth conductor and dielectric), and escaped
+// paths.  kClsAny: a kernel that shades every class.
+enum : int { kClsAny = -1, kClsDiffuse = 0, kClsGgxCond = 1, kClsGgxDiel = 2, kClsDirac = 3, kClsMiss = 4, kNumCls = 5 };
+__host__ __device__ constexpr uint32_t ClassOfType(int32_t t) {
+    return (t == kMatLambert || t == kMatPhong) ? kClsDiffuse
+           : (t == kMatGgxCond)                 ? kClsGgxCond
+           : (t == kMatGgxDiel)                 ? kClsGgxDiel
+                                                : kClsDirac;
+}
+
 struct SceneView {
     const DevTri *tris;
     const DevSphere *sphs;
     const DevMaterial *mats;
     const DevLight *lights;
     uint32_t numTris, numSphs, numLights;
+    uint32_t matClasses;       // bit c: some material of class c (ClassOfType); kClsMiss always set
     int32_t bgLight;
     DevEm em;
     float r2w[16];             // raster->world (row-major), camera.hxx:47-55

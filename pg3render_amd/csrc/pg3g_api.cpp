@@ -389,6 +389,8 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
     v.numTris = d->num_triangles;
     v.numSphs = d->num_spheres;
     v.numLights = d->num_lights;
+    v.matClasses = 1u << kClsMiss;
+    for (const DevMaterial &m : mats) v.matClasses |= 1u << ClassOfType(m.type);
     PG3_HIP(Upload(S.gradBase, S.gradBaseHost.data(), S.gradBaseHost.size(), st));
     v.evalMats = v.mats;
     v.gradBase = S.gradBase.as<int32_t>();

@@ -108,6 +108,16 @@ PG3_DEV int32_t PickLight(const LightSpec &spec, const LightCache &lc, float u, 
     return min((int32_t)(u * n), (int32_t)(n - 1));
 }
 
+// Tells the compiler which material types a class kernel meets: the type becomes one of the
+// class's own constants, so the BSDF code of every other type is dead code in that kernel.
+template <int kCls>
+PG3_DEV void NarrowMaterial(DevMaterial &m) {
+    if constexpr (kCls == kClsDiffuse) m.type = (m.type == kMatPhong) ? kMatPhong : kMatLambert;
+    else if constexpr (kCls == kClsGgxCond) m.type = kMatGgxCond;
+    else if constexpr (kCls == kClsGgxDiel) m.type = kMatGgxDiel;
+    else if constexpr (kCls == kClsDirac) m.type = (m.type == kMatSmoothCond) ? kMatSmoothCond : kMatSmoothDiel;
+}
+
 // State of one path between two vertices of PathTracer::EstimateIncomingRadiancePT (-sb 1).
 struct PathState {
     F3 org, dir;
@@ -251,6 +261,7 @@ PG3_DEV bool ContinuePath(const SceneView &S, PathState &ps, F3 sp, const Frame 
 // (:2387-2413) of that direction and its antipode.  Lanes of both kinds run these lookups in one
 // pass, so a wavefront no longer issues them once for its escaped lanes and again for its
 // shading lanes.
+template <int kCls = kClsAny>
 PG3_DEV bool ShadeVertexSteerable(const SceneView &S, const LightSpec &spec, const RngKey &key, PathState &ps,
                                   bool hit, const Hit &h, ShadowReq &sr, uint32_t &segments) {
     const DevEm &E = S.em;
@@ -271,6 +282,7 @@ PG3_DEV bool ShadeVertexSteerable(const SceneView &S, const LightSpec &spec, con
         fr = FrameFromZ(h.n);
         wol = ToLocal(fr, -ps.dir);
         mat = LoadGlobal(S.mats + h.mat);
+        NarrowMaterial<kCls>(mat);
         shade = HitEmissionAndChecks(S, spec, key, ps, sp, wol, mat, segments);
         if (shade) {
             RngVertex(key, len, vr);
@@ -376,11 +388,12 @@ PG3_DEV bool ShadeVertexSteerable(const SceneView &S, const LightSpec &spec, con
 // light-sample request and samples the BSDF.  Returns true if the path continues.
 // kMergeEm: kernels whose only light is a steerable EM take ShadeVertexSteerable (with several
 // lights its extra live state costs more register spill than it saves).
-template <bool kMergeEm = false>
+// kCls: the material class of every hit this call shades (kClsAny: any).
+template <bool kMergeEm = false, int kCls = kClsAny>
 PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey &key, PathState &ps, bool hit, const Hit &h,
                          ShadowReq &sr, uint32_t &segments) {
     sr.valid = false;
-    if constexpr (kMergeEm) return ShadeVertexSteerable(S, spec, key, ps, hit, h, sr, segments);
+    if constexpr (kMergeEm) return ShadeVertexSteerable<kCls>(S, spec, key, ps, hit, h, sr, segments);
     const uint32_t len = ps.len;
     const bool hasPrev = ps.flags & kHasPrev, prevFinite = ps.flags & kPrevFinite;
     if (!hit) {
@@ -407,7 +420,8 @@ PG3_DEV bool ShadeVertex(const SceneView &S, const LightSpec &spec, const RngKey
     const F3 sp = ps.org + ps.dir * h.dist;
     const Frame fr = FrameFromZ(h.n);
     const F3 wol = ToLocal(fr, -ps.dir);
-    const DevMaterial mat = LoadGlobal(S.mats + h.mat);
+    DevMaterial mat = LoadGlobal(S.mats + h.mat);
+    NarrowMaterial<kCls>(mat);
     if (mat.lightId >= 0 && len >= S.minPL) {                                         // :198-210
         const DevLight Lt = LoadGlobal(S.lights + mat.lightId);
         F3 Le = mk(0.f);

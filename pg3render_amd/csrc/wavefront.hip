@@ -95,6 +95,41 @@ __device__ __forceinline__ void BlockAppend(const bool (&p)[kN], uint32_t *const
     __syncthreads();   // the next call rewrites the counts
 }
 
+// Orders a workgroup's chunk of live paths by material class (the hit's material, kClsMiss for an
+// escaped path; inactive lanes last) so that its wavefronts shade one material each, except where
+// a class's run ends inside a wavefront: returns the live-path position this lane takes after
+// the reordering (0xFFFFFFFF: none).  A counting sort over the workgroup's lanes: a ballot per
+// class and wavefront, class offsets scanned in LDS.  Every lane of the workgroup calls it.
+template <int kWaves>
+__device__ __forceinline__ uint32_t SortChunkByClass(uint32_t j, uint32_t cls) {
+    __shared__ uint32_t cnt[kNumCls + 1][kWaves];
+    __shared__ uint32_t perm[kWaves * 64];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t c = 0; c <= (uint32_t)kNumCls; ++c) {
+        const uint64_t m = __ballot(cls == c);
+        if (cls == c) mine = m;
+        if (lane == 0) cnt[c][wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t off = 0;
+        for (uint32_t c = 0; c <= (uint32_t)kNumCls; ++c)
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t n = cnt[c][w];
+                cnt[c][w] = off;
+                off += n;
+            }
+    }
+    __syncthreads();
+    perm[cnt[cls][wave] + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull))] = j;
+    __syncthreads();
+    const uint32_t r = perm[threadIdx.x];
+    __syncthreads();   // the next call rewrites the table
+    return r;
+}
+
 __device__ __forceinline__ void StorePath(const WfPaths &P, uint32_t i, const PathState &ps, uint32_t pixel,
                                           uint32_t iteration) {
     P.A[i] = make_float4(ps.org.x, ps.org.y, ps.org.z, ps.tmin);
@@ -244,8 +279,16 @@ WfShade(const SceneView *__restrict__ Sp, WfQueues Q, uint32_t qi, uint32_t qo, 
     const WfPaths Pi = Q.P[qi], Po = Q.P[qo];
     uint32_t segments = 0;
     for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < count; j0 += gridDim.x * blockDim.x) {
-        const uint32_t j = j0 + threadIdx.x;
-        const bool active = j < count;
+        // material-coherent wavefronts: the chunk's paths reordered by material class
+        uint32_t jl = j0 + threadIdx.x, cls = kNumCls;
+        if (jl < count) {
+            const int32_t m = Q.hMat[jl];
+            cls = (m < 0) ? (uint32_t)kClsMiss : ClassOfType(AsGlobal(&S.mats[m].type)[0]);
+        } else {
+            jl = 0xFFFFFFFFu;
+        }
+        const uint32_t j = SortChunkByClass<kWfBlock / 64>(jl, cls);
+        const bool active = j != 0xFFFFFFFFu;
         bool cont = false;
         uint32_t pixel = 0xFFFFFFFFu, it = 0;
         PathState ps;
