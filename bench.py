@@ -22,10 +22,13 @@ TRI_FLOP, SPH_FLOP, SHADE_FLOP = 62, 35, 400
 BYTES_PER_SAMPLE, BYTES_PER_SEGMENT = 112, 304
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 KiB, factors from a 512 MiB copy): profiles/render_pt_box_steerable_em_512px_64spp.md
-MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 8375358 + 6484874) * 1024)}
+MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 6987434 + 5213501) * 1024)}
 # HBM bytes per frame of the wavefront pipeline on the default workload (all Wf* launches of a
 # frame, same calibration): profiles/wavefront_box_steerable_em_512px_64spp.md
-MEASURED_WF_TRAFFIC = int((2.0 * 17421712 + 18143837) * 1024)
+MEASURED_WF_TRAFFIC = int((2.0 * 16699838 + 17767364) * 1024)
+# HBM bytes per launch of the split kernel at the reference's defaults (same calibration and file)
+MEASURED_SPLIT_TRAFFIC = {"box_area_light_256px_16spp": int((2.0 * 2122068 + 3529365) * 1024),
+                          "box_steerable_em_512px_64spp": int((2.0 * 75420273 + 65573372) * 1024)}
 
 
 def parse():
@@ -86,7 +89,9 @@ def defaults_measure(ctx):
         out[name] = {"ms_per_frame": round(t, 3), "msamples_per_s": round(smp / 3 / t / 1e3, 1),
                      "msegments_per_s": round(seg / 3 / t / 1e3, 1),
                      "rays_per_s": round((ext + shadow) / 3 / t * 1e3, 1),
-                     "roofline": {"bound": bound, "frac": legs[bound]["frac"], "legs": legs}}
+                     "roofline": {"bound": bound, "frac": legs[bound]["frac"], "legs": legs,
+                                  "traffic": MEASURED_SPLIT_TRAFFIC[name],
+                                  "hbm_measured_gbs": round(MEASURED_SPLIT_TRAFFIC[name] / t / 1e6, 1)}}
     return out
 
 
