@@ -69,25 +69,39 @@ __global__ void __launch_bounds__(256) RenderMegakernel(SceneView S, uint32_t fi
 // nodes, 60 KiB): the tree descent of every light sample then gathers from LDS instead of from the
 // L2.  With the simple sampler the marginal CDF
 // is staged instead (up to kMargLdsMax entries, an EM of up to 4095 rows).
-// The split path keeps more state per lane: 3 waves per SIMD (168 VGPRs) measured faster than 4
-// (C2 frame at the reference's defaults: 62.4 against 64.7 ms; 2 waves: 90 ms).
+// The split path keeps more state per lane.  Its generic kernel (light samples in a loop with their
+// shadow rays) runs at 3 waves per SIMD (168 VGPRs; the box + area light frame at the reference's
+// defaults: 2.6 ms against 2.85 at 4 waves).  The kernel for a steerable EM as the only light
+// (SplitPath<true>: shared EM lookups, the shadow ray after the BSDF sample) needs 143 VGPRs and
+// runs faster at 4 waves per SIMD (128 VGPRs) despite the spill: the box + steerable EM frame
+// at the reference's defaults takes 45.5 against 56.0 ms.
 #ifndef PG3_SPLIT_WAVES_PER_EU
 #define PG3_SPLIT_WAVES_PER_EU 3
 #endif
-// Waves per SIMD of the chain path and of the split path (the split path keeps more state)
-template <bool kSplit>
-constexpr uint32_t PTWavesPerEU() { return kSplit ? PG3_SPLIT_WAVES_PER_EU : PG3_PT_WAVES_PER_EU; }
-template <bool kSplit>
-constexpr uint32_t PTWavesPerCU() { return 4u * PTWavesPerEU<kSplit>(); }
-template <int kEmKind, bool kSplit>
-constexpr uint32_t PTBlock() { return kEmKind == kEmSteerable ? 64u * PTWavesPerCU<kSplit>() : 256u; }
-static_assert(64u * PTWavesPerCU<false>() <= 1024u && 64u * PTWavesPerCU<true>() <= 1024u,
+#ifndef PG3_SPLIT_EM_WAVES_PER_EU
+#define PG3_SPLIT_EM_WAVES_PER_EU 4
+#endif
+// Waves per SIMD of the chain path, of the generic split path and of the split path with shared
+// EM lookups (kMerge)
+template <bool kSplit, bool kMerge>
+constexpr uint32_t PTWavesPerEU() {
+    return !kSplit ? PG3_PT_WAVES_PER_EU : (kMerge ? PG3_SPLIT_EM_WAVES_PER_EU : PG3_SPLIT_WAVES_PER_EU);
+}
+template <bool kSplit, bool kMerge>
+constexpr uint32_t PTWavesPerCU() { return 4u * PTWavesPerEU<kSplit, kMerge>(); }
+template <int kEmKind, bool kOneLight, bool kSplit>
+constexpr uint32_t PTBlock() {
+    return kEmKind == kEmSteerable ? 64u * PTWavesPerCU<kSplit, kOneLight && kEmKind == kEmSteerable>() : 256u;
+}
+static_assert(64u * PTWavesPerCU<false, false>() <= 1024u && 64u * PTWavesPerCU<false, true>() <= 1024u &&
+                  64u * PTWavesPerCU<true, false>() <= 1024u && 64u * PTWavesPerCU<true, true>() <= 1024u,
               "one steerable workgroup per CU holds every resident wavefront");
 
 // kSplit: the path tracer with splitting at the first vertex and indirect clipping (SplitPath),
 // else the -sb 1 -iic 0 chain (ChainPath).
 template <int kEmKind, bool kOneLight, bool kSplit>
-__global__ void __launch_bounds__((PTBlock<kEmKind, kSplit>())) __attribute__((amdgpu_waves_per_eu(PTWavesPerEU<kSplit>())))
+__global__ void __launch_bounds__((PTBlock<kEmKind, kOneLight, kSplit>()))
+__attribute__((amdgpu_waves_per_eu(PTWavesPerEU<kSplit, kOneLight && kEmKind == kEmSteerable>())))
 RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nIter, uint32_t itersPerThread,
                uint32_t numChunks, double *__restrict__ fb, uint32_t *__restrict__ work,
                unsigned long long *__restrict__ counters) {
@@ -111,7 +125,7 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
                                                          nIter,     itersPerThread, numTiles * numChunks, numTiles,
                                                          tilesX,    0u,       0u};
     src.SetTail(numChunks);
-    if constexpr (kSplit) TraceTasks<SplitPath, false>(src, tally);
+    if constexpr (kSplit) TraceTasks<SplitPath<kOneLight && kEmKind == kEmSteerable>, false>(src, tally);
     else TraceTasks<ChainPath<kOneLight && kEmKind == kEmSteerable>, false>(src, tally);
     FlushTally(tally, counters);
 }
@@ -352,21 +366,40 @@ __global__ void ResetWorkCounter(uint32_t *work) {
     if (threadIdx.x == 0) work[0] = 0u;
 }
 
-// Specialised path-tracer kernel for a scene (`pt` only).
-static PTKernelFn PickPTKernel(const SceneView &S, bool split) {
-    const bool one = (S.numLights == 1);
+// Whether every lane of the split path takes at most one light sample per step: nL <= nB light
+// samples at the first vertex (ComputeSplittingCounts, path_tracer.hxx:440-450) and one at the others.
+static bool OneLightSamplePerStep(const SceneView &S) {
+    const float nB = (float)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * S.splitBudget);
+    const long nL0 = lroundf(std::max(S.lightRatio * nB, 1.f)), nL1 = lroundf(std::max(S.lightRatio, 1.f));
+    return nL0 <= (long)nB && nL1 == 1;
+}
+
+// Specialised path-tracer kernel for a scene (`pt` only), its workgroup size and its resident
+// wavefronts per CU.  The split path's one-light steerable kernel (SplitPath<true>) takes one light
+// sample per lane and step; other light-sample counts run the generic split kernel.
+struct PTLaunch {
+    PTKernelFn fn;
+    uint32_t block, wavesPerCU;
+};
+template <int kEmKind, bool kOneLight, bool kSplit>
+static PTLaunch PTLaunchOf() {
+    return {RenderPTKernel<kEmKind, kOneLight, kSplit>, PTBlock<kEmKind, kOneLight, kSplit>(),
+            PTWavesPerCU<kSplit, kOneLight && kEmKind == kEmSteerable>()};
+}
+static PTLaunch PickPTKernel(const SceneView &S, bool split) {
     const int em = SceneLightSpec(S).em;
-#define PG3_PT_CASE(K)                                                                        \
-    if (em == (K)) {                                                                          \
-        if (split) return one ? RenderPTKernel<(K), true, true> : RenderPTKernel<(K), false, true>; \
-        return one ? RenderPTKernel<(K), true, false> : RenderPTKernel<(K), false, false>;    \
+    const bool one = (S.numLights == 1) && !(split && em == kEmSteerable && !OneLightSamplePerStep(S));
+#define PG3_PT_CASE(K)                                                                                    \
+    if (em == (K)) {                                                                                      \
+        if (split) return one ? PTLaunchOf<(K), true, true>() : PTLaunchOf<(K), false, true>();           \
+        return one ? PTLaunchOf<(K), true, false>() : PTLaunchOf<(K), false, false>();                    \
     }
     PG3_PT_CASE(kEmAbsent)
     PG3_PT_CASE(kEmCosine)
     PG3_PT_CASE(kEmSimple)
     PG3_PT_CASE(kEmSteerable)
 #undef PG3_PT_CASE
-    return nullptr;
+    return {nullptr, 0u, 0u};
 }
 
 hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
@@ -399,11 +432,9 @@ hipError_t LaunchMegakernel(const SceneView &S, const SceneView *Sdev, uint32_t 
     // every later vertex has one BSDF sample (SplitPath); other budgets take the general tree
     const uint32_t nB0 = (uint32_t)lroundf((1.f - S.splitLevel) * 1.f + S.splitLevel * S.splitBudget);
     const bool rootSplit = (S.algorithm == 7) && !chain && nB0 >= 1u && (1 + ((S.splitBudget - nB0) / (float)nB0)) == 1.f;
-    const PTKernelFn pt = (chain || rootSplit) ? PickPTKernel(S, rootSplit) : nullptr;
-    const bool steer = SceneLightSpec(S).em == kEmSteerable;
-    const uint32_t ptBlock = rootSplit ? (steer ? PTBlock<kEmSteerable, true>() : PTBlock<kEmSimple, true>())
-                                       : (steer ? PTBlock<kEmSteerable, false>() : PTBlock<kEmSimple, false>());
-    const uint32_t wavesPerCU = rootSplit ? PTWavesPerCU<true>() : PTWavesPerCU<false>();
+    const PTLaunch pl = (chain || rootSplit) ? PickPTKernel(S, rootSplit) : PTLaunch{nullptr, 0u, 0u};
+    const PTKernelFn pt = pl.fn;
+    const uint32_t ptBlock = pl.block, wavesPerCU = pl.wavesPerCU;
     if (pt) {
         if (!work) return hipErrorInvalidValue;
         // items of a few iterations: the wave overlaps items, so small items cost no tail, and

@@ -252,19 +252,81 @@ PG3_DEV bool ContinuePath(const SceneView &S, PathState &ps, F3 sp, const Frame 
     return true;
 }
 
-// Vertex of the path tracer with a steerable EM; the same results as the generic vertex below,
-// in another order.  An escaped path (emission with MIS, path_tracer.hxx:408-422: the EM pdf of
-// its direction at the previous vertex) and a light sample of the EM (SampleImpl and
-// SampleHemisphereImpl, em_steerable_sampler.hxx:2084-2217) need the same lookups: the
-// hemisphere's clamped-cosine coefficients and integrals, the EM radiance at one direction and
-// the luminance at its antipode (one lat-long mapping), and the two GetBasePdf terms
-// (:2387-2413) of that direction and its antipode.  Lanes of both kinds run these lookups in one
-// pass, so a wavefront no longer issues them once for its escaped lanes and again for its
-// shading lanes.
+// The EM lookups of a vertex in a scene with a steerable EM.  An escaped path (emission
+// with MIS, path_tracer.hxx:408-422: the EM pdf of its direction at the previous vertex) and a light
+// sample of the EM (SampleImpl and SampleHemisphereImpl, em_steerable_sampler.hxx:2084-2217) need
+// the same lookups: the hemisphere's clamped-cosine coefficients and integrals, the EM radiance at
+// one direction and the luminance at its antipode (one lat-long mapping), and the two GetBasePdf
+// terms (:2387-2413) of that direction and its antipode.  Lanes of both kinds run these lookups in
+// one pass, so a wavefront does not issue them once for its escaped lanes and again for its
+// shading lanes.  A light-sample lane (nee) samples around the normal qn of its vertex (sides
+// qsides, random numbers uSide, u0, u1, pick probability prob) and gets ls / lsOk; an escaped lane
+// gets the radiance Le along qdir and its pdf emPdf under the sampling of the previous vertex
+// (normal qn, sides qsides).
+struct EmQuery {
+    F3 Le;
+    float emPdf;
+    LightSample ls;
+    bool lsOk;
+};
+PG3_DEV void SteerableEmQuery(const DevEm &E, const LightSpec &spec, bool nee, F3 qn, F3 qdir, uint32_t qsides,
+                              float uSide, float u0, float u1, float prob, EmQuery &q) {
+    const bool front = qsides & kSideFront, back = qsides & kSideBack;
+    bool ok = front || back;
+    float ui = 0.f, li = 0.f, wi = 0.f;
+    bool up = front;
+    if (front && back) {
+        ui = WholeIntegral(E, ClampedCos(qn));
+        li = WholeIntegral(E, ClampedCos(-qn));
+        wi = ui + li;
+        if (IsTiny(wi)) ok = false;
+        up = nee ? (uSide * wi < ui) : (Dot(qdir, qn) >= 0.f);
+    }
+    const F3 hn = up ? qn : -qn;
+    const SH9 c = ClampedCos(hn);
+    const float whole = WholeIntegral(E, c);
+    float triPdf = 0.f, cs = 0.f;
+    bool flip = false;
+    if (nee && ok) {
+        float val;
+        const F3 sdir = SteerTriangleSample(E, c, u0, u1, spec.tree, val);
+        triPdf = IsTiny(whole) ? 0.f : FastDiv(val, whole);
+        cs = Dot(sdir, hn);
+        flip = cs < 0.f;
+        qdir = sdir;
+        if (flip) { qdir = -sdir; cs *= -1.f; }
+    }
+    float u, v, uo, vo;
+    Dir2LatLong(qdir, u, v);
+    q.Le = EmEvalUV(E, u, v);
+    Antipode(u, v, uo, vo);
+    const float lumOpp = EmLumUV(E, uo, vo);
+    // GetBasePdf of qdir and of -qdir; a light sample needs the one of -(sampled direction)
+    const float t1 = BasePdfLum(E, qdir, Luminance(q.Le), c);
+    const float t2 = BasePdfLum(E, -qdir, lumOpp, c);
+    q.emPdf = 0.f;
+    q.lsOk = false;
+    if (nee) {
+        q.lsOk = ok;
+        float pdf = triPdf;
+        pdf += flip ? t1 : t2;
+        if (front && back) pdf *= FastDiv(up ? ui : li, wi);
+        q.ls.wig = qdir;
+        q.ls.pdfW = pdf;
+        q.ls.sample = q.Le * cs;
+        q.ls.prob = prob;
+        q.ls.dist = 3.402823466e+38f;   // em_sampler.hxx:57
+    } else if (ok) {
+        const float p = (Dot(qdir, hn) < 0.f) ? 0.f : t1 + t2;
+        q.emPdf = (front && back) ? p * FastDiv(up ? ui : li, wi) : p;
+    }
+}
+
+// Vertex of the path tracer with a steerable EM; the same results as the generic vertex below, in
+// another order (the EM lookups of SteerableEmQuery in one pass).
 template <int kCls = kClsAny>
 PG3_DEV bool ShadeVertexSteerable(const SceneView &S, const LightSpec &spec, const RngKey &key, PathState &ps,
                                   bool hit, const Hit &h, ShadowReq &sr, uint32_t &segments) {
-    const DevEm &E = S.em;
     const uint32_t len = ps.len;
     F3 sp = mk(0.f), wol = mk(0.f);
     Frame fr{};
@@ -274,7 +336,7 @@ PG3_DEV bool ShadeVertexSteerable(const SceneView &S, const LightSpec &spec, con
     lc.sum = 0.f;
     bool shade = false;            // hit lane that shades this vertex
     bool emq = false, nee = false;  // lane runs the EM lookups; they serve a light sample
-    F3 qn = ps.prevN, qdir = ps.dir;
+    F3 qn = ps.prevN;
     uint32_t qsides = ps.flags >> 4;
     float uSide = 0.f, u0 = 0.f, u1 = 0.f, prob = 1.f;
     if (hit) {
@@ -310,76 +372,28 @@ PG3_DEV bool ShadeVertexSteerable(const SceneView &S, const LightSpec &spec, con
     } else {
         emq = len >= S.minPL && S.bgLight >= 0;
     }
-    F3 Le = mk(0.f);
-    float emPdf = 0.f;   // escaped lane: pdf of its direction under the previous vertex's EM sampling
-    LightSample ls;
-    bool lsOk = false;
-    if (emq) {
-        const bool front = qsides & kSideFront, back = qsides & kSideBack;
-        bool ok = front || back;
-        float ui = 0.f, li = 0.f, wi = 0.f;
-        bool up = front;
-        if (front && back) {
-            ui = WholeIntegral(E, ClampedCos(qn));
-            li = WholeIntegral(E, ClampedCos(-qn));
-            wi = ui + li;
-            if (IsTiny(wi)) ok = false;
-            up = nee ? (uSide * wi < ui) : (Dot(qdir, qn) >= 0.f);
-        }
-        const F3 hn = up ? qn : -qn;
-        const SH9 c = ClampedCos(hn);
-        const float whole = WholeIntegral(E, c);
-        float triPdf = 0.f, cs = 0.f;
-        bool flip = false;
-        if (nee && ok) {
-            float val;
-            const F3 sdir = SteerTriangleSample(E, c, u0, u1, spec.tree, val);
-            triPdf = IsTiny(whole) ? 0.f : FastDiv(val, whole);
-            cs = Dot(sdir, hn);
-            flip = cs < 0.f;
-            qdir = sdir;
-            if (flip) { qdir = -sdir; cs *= -1.f; }
-        }
-        float u, v, uo, vo;
-        Dir2LatLong(qdir, u, v);
-        Le = EmEvalUV(E, u, v);
-        Antipode(u, v, uo, vo);
-        const float lumOpp = EmLumUV(E, uo, vo);
-        // GetBasePdf of qdir and of -qdir; a light sample needs the one of -(sampled direction)
-        const float t1 = BasePdfLum(E, qdir, Luminance(Le), c);
-        const float t2 = BasePdfLum(E, -qdir, lumOpp, c);
-        if (nee) {
-            lsOk = ok;
-            float pdf = triPdf;
-            pdf += flip ? t1 : t2;
-            if (front && back) pdf *= FastDiv(up ? ui : li, wi);
-            ls.wig = qdir;
-            ls.pdfW = pdf;
-            ls.sample = Le * cs;
-            ls.prob = prob;
-            ls.dist = 3.402823466e+38f;   // em_sampler.hxx:57
-        } else if (ok) {
-            const float p = (Dot(qdir, hn) < 0.f) ? 0.f : t1 + t2;
-            emPdf = (front && back) ? p * FastDiv(up ? ui : li, wi) : p;
-        }
-    }
+    EmQuery q;
+    q.Le = mk(0.f);
+    q.emPdf = 0.f;
+    q.lsOk = false;
+    if (emq) SteerableEmQuery(S.em, spec, nee, qn, ps.dir, qsides, uSide, u0, u1, prob, q);
     if (!hit) {                                                                       // :408-422
         if (emq) {
             if (!(ps.flags & kHasPrev)) {
-                ps.L += Le;
-            } else if (!IsZero(Le)) {
+                ps.L += q.Le;
+            } else if (!IsZero(q.Le)) {
                 if (ps.flags & kPrevFinite) {
-                    const float lp = emPdf * PrevPickProb(S, spec, key, ps, (uint32_t)S.bgLight);
-                    ps.L += ps.eThr * Le * MisPower(ps.prevBsdfPdf, lp);
+                    const float lp = q.emPdf * PrevPickProb(S, spec, key, ps, (uint32_t)S.bgLight);
+                    ps.L += ps.eThr * q.Le * MisPower(ps.prevBsdfPdf, lp);
                 } else {
-                    ps.L += ps.eThr * Le;
+                    ps.L += ps.eThr * q.Le;
                 }
             }
         }
         return false;
     }
     if (!shade) return false;
-    if (lsOk) LightSampleRequest(ps, mat, fr, wol, sp, ls, sr);
+    if (q.lsOk) LightSampleRequest(ps, mat, fr, wol, sp, q.ls, sr);
     return ContinuePath(S, ps, sp, fr, wol, mat, vr, lc.sum);
 }
 
@@ -1151,8 +1165,11 @@ struct TileItem {
 //   src.Next(TileItem&) -> bool   fetches the next item (false: none left; wave-uniform)
 //   src.View(scene) -> const SceneView *, src.Spec(scene) -> LightSpec
 //   Path: a lane's task.  Path::Subs(S) tasks per camera sample (sub-samples whose radiances add
-//   up to the sample's); Start(S, pixel, iteration, sub); Step(S, spec, tally, shadowed, step) ->
-//   true when the task is done, with `step` the wavefront's step number (the same on every lane);
+//   up to the sample's); Start(S, pixel, iteration, sub); Step(S, spec, tally, shadowed, step, ring)
+//   -> true when the task is done, with `step` the wavefront's step number (the same on every lane)
+//   and `ring` the lane's Path::Ring, an array kept apart from the path so that the path's own
+//   fields stay in registers (an array indexed at run time lives in scratch memory, and a struct
+//   holding one would be kept there whole);
 //   Result() -> its radiance; CountsSample() -> whether the task counts as a camera sample.  Step
 //   traces exactly one extension ray (counted here) and counts its segments; a path with at most
 //   one shadow ray per step sets `shadowed` when it traced one (counted here), others count
@@ -1169,6 +1186,7 @@ PG3_DEV void TraceTasks(Src &src, RayTally &tally) {
     double *fb = nullptr;              // and its framebuffer
     uint32_t step = 0;                 // steps taken by the wavefront (wave-uniform)
     Path path;
+    typename Path::Ring ring;
     for (;;) {
         const uint64_t idle = __ballot(!busy);
         if (idle != 0 && more && next >= T.total) {
@@ -1207,7 +1225,7 @@ PG3_DEV void TraceTasks(Src &src, RayTally &tally) {
         }
         tally.wExt += (uint32_t)__popcll(stepping);        // one Intersect per Step
         bool shadowed = false, counted = false;
-        if (busy && path.Step(S, spec, tally, shadowed, step)) {
+        if (busy && path.Step(S, spec, tally, shadowed, step, ring)) {
             const F3 L = path.Result();
             double *f = fb + (size_t)px * 3;
             atomicAdd(f + 0, (double)L.x);
@@ -1228,6 +1246,7 @@ template <bool kMergeEm>
 struct ChainPath {
     RngKey key;
     PathState ps;
+    struct Ring {};
     PG3_DEV static constexpr uint32_t Subs(const SceneView &) { return 1u; }
     PG3_DEV static constexpr bool CountsSample() { return true; }
     PG3_DEV void Start(const SceneView &S, uint32_t pixel, uint32_t iteration, uint32_t) {
@@ -1235,7 +1254,7 @@ struct ChainPath {
         CameraSample(S, pixel, iteration, key, org, dir);
         InitPath(ps, org, dir);
     }
-    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &t, bool &shadowed, uint32_t) {
+    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &t, bool &shadowed, uint32_t, Ring &) {
         Hit h;
         h.dist = 1e36f;
         const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
@@ -1269,7 +1288,19 @@ struct ChainPath {
 // replays it exactly) and folds the next part onto the folded suffix, and so on to the first
 // vertex: the same operations in the same order as one fold of the whole chain.  RNG streams as
 // RadiancePTTree (the chain of BSDF sample s is branch s + 1 when nB > 1).
-constexpr uint32_t kChainRing = 32;
+// kMergeEm: the scene's only light is a steerable EM and every lane takes at most one light
+// sample per step (LightCount(nB) <= nB at the first vertex and 1 at the others: the reference's
+// light-to-BSDF ratio 1).  The escaped lanes' emission and the shading lanes' light samples then
+// share one pass of EM lookups (SteerableEmQuery), and the light sample's shadow ray is traced at
+// the end of the step, after the BSDF sample, when little else is live; otherwise the light
+// samples of a vertex run in a loop with their shadow rays (LightSampleSum).
+#ifndef PG3_CHAIN_RING
+#define PG3_CHAIN_RING 32
+#endif
+constexpr uint32_t kChainRing = PG3_CHAIN_RING;
+static_assert(kChainRing >= 2 && (kChainRing & (kChainRing - 1)) == 0,
+              "ring slots follow the step counter modulo a power of two, across its wrap-around");
+template <bool kMergeEm>
 struct SplitPath {
     // ray in flight: it reaches vertex `len` (1: the camera ray's hit) of sub-sample `sub`
     F3 org, dir;
@@ -1290,7 +1321,9 @@ struct SplitPath {
     uint32_t nPush;
     int32_t target;
     F3 rSuffix;
-    float ring[kChainRing][7];   // A.xyz, attCos.xyz, den
+    struct Ring {
+        float r[kChainRing][7];  // A.xyz, attCos.xyz, den
+    };
 
     // BSDF samples of the first vertex (ComputeSplittingCounts, :440-446): the sub-samples per sample
     PG3_DEV static uint32_t Subs(const SceneView &S) {
@@ -1341,13 +1374,13 @@ struct SplitPath {
     // holds the older records, the newest of them pushed at step t - jStart.  Folds them back to
     // front; if the oldest records were overwritten, sets up the re-trace up to the oldest folded
     // depth and returns false, else adds the first vertex's estimate to L and returns true.
-    PG3_DEV bool Fold(const SceneView &S, uint32_t t, uint32_t jStart, F3 rChild) {
+    PG3_DEV bool Fold(const SceneView &S, uint32_t t, uint32_t jStart, F3 rChild, const Ring &ring) {
         F3 R = tA;
         if (tDen > 0.f && !IsZero(rChild)) R += ClippedIndirect(tAttCos, rChild, tDen, S.iic);
         const uint32_t n = min(nPush, kChainRing);
 #pragma unroll 1
         for (uint32_t i = 0; i < n; ++i) {
-            const float *r = ring[(t - jStart - i) % kChainRing];
+            const float *r = ring.r[(t - jStart - i) % kChainRing];
             F3 Rk = mk(r[0], r[1], r[2]);
             const float den = r[6];
             if (den > 0.f && !IsZero(R)) Rk += ClippedIndirect(mk(r[3], r[4], r[5]), R, den, S.iic);
@@ -1368,7 +1401,153 @@ struct SplitPath {
         return true;
     }
     // One vertex per step (t: the wavefront's step number, the same on every lane).
-    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &tally, bool &, uint32_t t) {
+    PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &tally, bool &shadowed, uint32_t t,
+                      Ring &ring) {
+        if constexpr (kMergeEm) return StepMerged(S, spec, tally, shadowed, t, ring);
+        else return StepGeneric(S, spec, tally, t, ring);
+    }
+    // Ring record of the parent vertex (its A is complete once the emission its BSDF sample finds
+    // is added), pushed at step t.
+    PG3_DEV void Push(uint32_t t, Ring &ring) {
+        float *r = ring.r[t % kChainRing];
+        r[0] = tA.x;
+        r[1] = tA.y;
+        r[2] = tA.z;
+        r[3] = tAttCos.x;
+        r[4] = tAttCos.y;
+        r[5] = tAttCos.z;
+        r[6] = tDen;
+        ++nPush;
+    }
+    // BSDF sample si of vertex len (:281-404) with nBv samples at the vertex: the indirect factors
+    // of the new record, what the MIS weight of the emission it finds needs, and the next ray.
+    // Returns false for a blocker sample (no indirect term).
+    PG3_DEV bool SampleNext(const SceneView &S, const RngKey &k, uint32_t si, uint32_t nBv, F3 sp, const Frame &fr,
+                            F3 wol, const DevMaterial &mat) {
+        const float rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
+        const bool cut = (S.maxPL == 0) && (RngGet(k, len, si, kDimRR) > rr);
+        BsdfRec r;
+        SampleBsdf(mat, wol, RngGet(k, len, si, kDimBsdf0), RngGet(k, len, si, kDimBsdf1), RngGet(k, len, si, kDimBsdf2),
+                   r);
+        if (IsBlocker(r)) {
+            tDen = -1.f;
+            return false;
+        }
+        const bool finite = (r.pdfW != kInf);
+        const float cosAbs = fabsf(r.wil.z);
+        tAttCos = finite ? r.att * cosAbs : r.att;
+        tDen = cut ? -1.f : (finite ? (r.pdfW * nBv * rr * r.compProb) : (nBv * rr * r.compProb));
+        pPdf = finite ? r.pdfW * r.compProb : r.compProb;
+        pN = fr.z;
+        pFlags = (finite ? 1u : 0u) | (mat.sides << 4);
+        org = sp;
+        dir = ToWorld(fr, r.wil);
+        tmin = EpsRayCos(cosAbs);
+        len = len + 1;
+        refl = !cut;
+        return true;
+    }
+    PG3_DEV bool StepMerged(const SceneView &S, const LightSpec &spec, RayTally &tally, bool &shadowed, uint32_t t,
+                            Ring &ring) {
+        const uint32_t nB = Subs(S);
+        const RngKey k = Branch(S, BranchOf(len, nB));
+        Hit h;
+        h.dist = 1e36f;
+        const bool hit = Intersect(S, org, dir, tmin, h);
+        const bool first = (len == 1);
+        const bool atTarget = (target >= 0) && ((int32_t)len - 1 == target);
+        F3 em = mk(0.f);                 // emission found at vertex len (area light or EM)
+        float lpdf = 0.f;
+        int32_t lid = -1;
+        F3 sp = mk(0.f), wol = mk(0.f);
+        Frame fr{};
+        DevMaterial mat{};
+        bool shade = false, emq = false, nee = false;
+        F3 qn = pN;
+        uint32_t qsides = pFlags >> 4;
+        float uSide = 0.f, u0 = 0.f, u1 = 0.f, prob = 1.f;
+        const uint32_t nBv = first ? nB : 1u, nLv = LightCount(S, nBv), l = first ? sub : 0u;
+        if (hit) {                                                                    // :176-238
+            sp = org + dir * h.dist;
+            fr = FrameFromZ(h.n);
+            wol = ToLocal(fr, -dir);
+            mat = LoadGlobal(S.mats + h.mat);
+            if (mat.lightId >= 0 && len >= S.minPL) {
+                const DevLight Lt = LoadGlobal(S.lights + mat.lightId);
+                if (Lt.type == kLightArea) {
+                    if (len > 1) lpdf = AreaEmissionPdf(Lt, sp, org);
+                    if (wol.z > 0.f) em = Lt.radiance;
+                } else if (Lt.type == kLightPoint) {
+                    lpdf = kInf;
+                }
+                lid = mat.lightId;
+            }
+            shade = !atTarget && refl && !ReflectanceZero(mat) && !(S.maxPL > 0 && len >= S.maxPL) && len < 1200u;
+            if (shade && (len + 1) >= S.minPL && l < nLv) {                          // :252-265
+                LightCache lc;
+                lc.sum = 0.f;
+                const int32_t li = PickLight(spec, lc, RngGet(k, len, l, kDimLightPick), prob);
+                if (li >= 0) {
+                    emq = nee = true;
+                    qn = fr.z;
+                    qsides = mat.sides;
+                    u0 = RngGet(k, len, l, kDimLight0);
+                    u1 = RngGet(k, len, l, kDimLight1);
+                    uSide = RngGet(k, len, l, kDimLight2);
+                }
+            }
+        } else {
+            emq = len >= S.minPL && S.bgLight >= 0;                                   // :408-422
+        }
+        EmQuery q;
+        q.Le = mk(0.f);
+        q.emPdf = 0.f;
+        q.lsOk = false;
+        if (emq) SteerableEmQuery(S.em, spec, nee, qn, dir, qsides, uSide, u0, u1, prob, q);
+        if (!hit && emq) {
+            em = q.Le;
+            lpdf = q.emPdf;
+            lid = S.bgLight;
+        }
+        if (first) {
+            if (sub == 0u && target < 0) L += em;      // the camera hit's emission, once per sample
+        } else {
+            tA += ParentEmission(S, spec, em, lpdf, lid, nB);
+        }
+        if (!shade) {
+            if (first) return true;
+            return Fold(S, t, 1u, atTarget ? rSuffix : mk(0.f), ring);
+        }
+        if (target < 0 && (!first || sub == 0u)) ++tally.segments;   // the shared first vertex once
+        if (!first) Push(t, ring);                      // the parent's record is complete
+        // the light sample's contribution (path_tracer_base.hxx:350-425); its shadow ray is traced
+        // after the BSDF sample
+        F3 contrib = mk(0.f), sdir = mk(0.f);
+        float smax = 0.f;
+        bool sv = false;
+        if (q.lsOk && MaxC(q.ls.sample) > 0.f) {
+            BsdfRec r;
+            EvalBsdf(mat, ToLocal(fr, q.ls.wig), wol, r);
+            if (q.ls.pdfW != kInf) {
+                const float bsdfPdf = r.pdfW * r.compProb;
+                const float lightPdf = q.ls.pdfW * q.ls.prob;
+                contrib = FastDiv(q.ls.sample * r.att * MisPowerN(lightPdf, nLv, bsdfPdf, nBv), lightPdf);
+            } else {
+                contrib = FastDiv(q.ls.sample * r.att, q.ls.prob * nLv);
+            }
+            sdir = q.ls.wig;
+            smax = q.ls.dist;
+            sv = true;
+        }
+        tA = mk(0.f);
+        pLcSum = 0.f;
+        const bool more = SampleNext(S, k, first ? sub : 0u, nBv, sp, fr, wol, mat);
+        shadowed = sv;
+        if (sv && !Occluded(S, sp, sdir, smax)) tA = contrib;
+        if (!more) return Fold(S, t, 0u, mk(0.f), ring);
+        return false;
+    }
+    PG3_DEV bool StepGeneric(const SceneView &S, const LightSpec &spec, RayTally &tally, uint32_t t, Ring &ring) {
         const uint32_t nB = Subs(S);
         const RngKey k = Branch(S, BranchOf(len, nB));
         Hit h;
@@ -1410,50 +1589,19 @@ struct SplitPath {
                           len >= 1200u;
         if (stop) {
             if (len == 1) return true;
-            return Fold(S, t, 1u, atTarget ? rSuffix : mk(0.f));
+            return Fold(S, t, 1u, atTarget ? rSuffix : mk(0.f), ring);
         }
         const bool first = (len == 1);
         if (target < 0 && (!first || sub == 0u)) ++tally.segments;   // the shared first vertex once
-        if (!first) {                                   // the parent's record is complete: push it
-            float *r = ring[t % kChainRing];
-            r[0] = tA.x;
-            r[1] = tA.y;
-            r[2] = tA.z;
-            r[3] = tAttCos.x;
-            r[4] = tAttCos.y;
-            r[5] = tAttCos.z;
-            r[6] = tDen;
-            ++nPush;
-        }
+        if (!first) Push(t, ring);                      // the parent's record is complete
         LightCache lc;
         lc.sum = 0.f;
         if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
         const uint32_t nBv = first ? nB : 1u;
         tA = LightSampleSum(S, spec, k, len, sp, fr, wol, mat, first ? sub : 0u, nBv, LightCount(S, nBv), nBv, lc,
                             tally.shadow);
-        const float rr = (S.maxPL == 0) ? Clampf(RRProb(mat, wol), 0.0f, 0.997f) : 1.0f;
-        const uint32_t si = first ? sub : 0u;           // :281-404, BSDF sample si of this vertex
-        const bool cut = (S.maxPL == 0) && (RngGet(k, len, si, kDimRR) > rr);
-        BsdfRec r;
-        SampleBsdf(mat, wol, RngGet(k, len, si, kDimBsdf0), RngGet(k, len, si, kDimBsdf1), RngGet(k, len, si, kDimBsdf2),
-                   r);
-        if (IsBlocker(r)) {
-            tDen = -1.f;
-            return Fold(S, t, 0u, mk(0.f));
-        }
-        const bool finite = (r.pdfW != kInf);
-        const float cosAbs = fabsf(r.wil.z);
-        tAttCos = finite ? r.att * cosAbs : r.att;
-        tDen = cut ? -1.f : (finite ? (r.pdfW * nBv * rr * r.compProb) : (nBv * rr * r.compProb));
-        pPdf = finite ? r.pdfW * r.compProb : r.compProb;
-        pN = fr.z;
         pLcSum = lc.sum;
-        pFlags = (finite ? 1u : 0u) | (mat.sides << 4);
-        org = sp;
-        dir = ToWorld(fr, r.wil);
-        tmin = EpsRayCos(cosAbs);
-        len = len + 1;
-        refl = !cut;
+        if (!SampleNext(S, k, first ? sub : 0u, nBv, sp, fr, wol, mat)) return Fold(S, t, 0u, mk(0.f), ring);
         return false;
     }
     PG3_DEV F3 Result() const { return L; }

@@ -398,6 +398,31 @@ def test_splitting_and_clipping_parity(ctx, params):
     assert_image_parity(g, o, frac=0.03)
 
 
+STEERABLE_SPLIT_CASES = [
+    pytest.param(dict(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0), id="defaults_sb4_iic16_rr"),
+    pytest.param(dict(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=8), id="sb4_iic16_maxpl8"),
+    # two light samples over four sub-samples: sub-samples 2 and 3 take none
+    pytest.param(dict(splitting_budget=4.0, light_to_bsdf_ratio=0.5, indirect_clipping=4.0, max_path_length=0),
+                 id="sb4_ratio0.5_rr"),
+    # two light samples per sub-sample: the generic split kernel
+    pytest.param(dict(splitting_budget=3.0, light_to_bsdf_ratio=2.0, indirect_clipping=16.0, max_path_length=0),
+                 id="sb3_ratio2_rr"),
+]
+
+
+@pytest.mark.parametrize("params", STEERABLE_SPLIT_CASES)
+def test_split_path_with_steerable_em_parity(ctx, params):
+    """First-vertex splitting on the box with a steerable EM as its only light, the workload of the
+    reference's defaults: the split kernel whose lanes share the EM lookups of escaped paths and
+    light samples (one light sample per lane and vertex), and the generic split kernel for light
+    sample counts above the BSDF sample count."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(2, seed=23, res=32)
+    sb.set_params(**params)
+    g, o = render_pair(ctx, sb, 4)
+    assert_image_parity(g, o, frac=0.03)
+
+
 @pytest.mark.parametrize("scene_id", [33, 35, 36])
 def test_tree_walker_keeps_the_1200_vertex_limit(ctx, scene_id):
     """Splitting below the first vertex (-sb 4 -sl 0.5: three BSDF samples at the first vertex,
