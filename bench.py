@@ -27,8 +27,8 @@ MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 7223703 + 4919513) * 1024)}
 # frame, same calibration): profiles/wavefront_box_steerable_em_512px_64spp.md
 MEASURED_WF_TRAFFIC = int((2.0 * 16699838 + 17767364) * 1024)
 # HBM bytes per launch of the split kernel at the reference's defaults (same calibration and file)
-MEASURED_SPLIT_TRAFFIC = {"box_area_light_256px_16spp": int((2.0 * 1046262 + 1148485) * 1024),
-                          "box_steerable_em_512px_64spp": int((2.0 * 59333204 + 37199015) * 1024)}
+MEASURED_SPLIT_TRAFFIC = {"box_area_light_256px_16spp": int((2.0 * 1048337 + 1148503) * 1024),
+                          "box_steerable_em_512px_64spp": int((2.0 * 51761217 + 20670494) * 1024)}
 
 
 def parse():

@@ -1294,6 +1294,26 @@ struct ChainPath {
 // share one pass of EM lookups (SteerableEmQuery), and the light sample's shadow ray is traced at
 // the end of the step, after the BSDF sample, when little else is live; otherwise the light
 // samples of a vertex run in a loop with their shadow rays (LightSampleSum).
+// Per-lane fields of a task.  kLds: in LDS, one slot per lane of a workgroup of at most
+// kLaneFieldLanes lanes (field i of lane x at i * kLaneFieldLanes + x: consecutive lanes,
+// consecutive banks); else in registers.
+constexpr uint32_t kLaneFieldLanes = 1024;
+template <uint32_t N, bool kLds>
+struct LaneFields {
+    uint32_t v[N];
+    PG3_DEV uint32_t Get(uint32_t i) const { return v[i]; }
+    PG3_DEV void Set(uint32_t i, uint32_t x) { v[i] = x; }
+};
+template <uint32_t N>
+struct LaneFields<N, true> {
+    PG3_DEV static uint32_t *Base() {
+        __shared__ uint32_t fields[N * kLaneFieldLanes];
+        return fields;
+    }
+    PG3_DEV uint32_t Get(uint32_t i) const { return Base()[i * kLaneFieldLanes + threadIdx.x]; }
+    PG3_DEV void Set(uint32_t i, uint32_t x) { Base()[i * kLaneFieldLanes + threadIdx.x] = x; }
+};
+
 #ifndef PG3_CHAIN_RING
 #define PG3_CHAIN_RING 32
 #endif
@@ -1305,10 +1325,8 @@ struct SplitPath {
     // ray in flight: it reaches vertex `len` (1: the camera ray's hit) of sub-sample `sub`
     F3 org, dir;
     float tmin;
-    uint32_t len, sub;
+    uint32_t len;
     bool refl;                   // the vertex it reaches is shaded (not cut by Russian roulette)
-    uint32_t pixel, iteration;   // RNG key of the sample (with the scene's key and id)
-    F3 L;                        // the sub-sample's radiance (sub-sample 0 adds the camera hit's emission)
     // newest record (vertex len - 1): A, the indirect factors of its BSDF sample (attCos, den;
     // den < 0: no indirect term -- cut by Russian roulette or no sample), and what the MIS weight
     // of the emission found at vertex len needs of that vertex (its position is `org`): its normal,
@@ -1316,11 +1334,27 @@ struct SplitPath {
     F3 tA, tAttCos, pN;
     float tDen, pPdf, pLcSum;
     uint32_t pFlags;             // bit 0: finite BSDF sample; sides << 4
-    // records of depths [0, nPush) (vertex = depth + 1) pushed at consecutive steps.  Re-tracing a
-    // deep chain: the pass stops at depth `target` (-1: first pass), whose estimate is rSuffix.
-    uint32_t nPush;
-    int32_t target;
-    F3 rSuffix;
+    // Fields a step reads a few times at most: the sample's RNG key (pixel, iteration), the
+    // sub-sample index, the count nPush of records of depths [0, nPush) (vertex = depth + 1) pushed
+    // at consecutive steps, the depth `target` a re-trace of a deep chain stops at (-1: first pass)
+    // and that depth's estimate rSuffix, and the sub-sample's radiance L (sub-sample 0 adds the
+    // camera hit's emission).  The split kernel of a steerable EM keeps them in LDS, which frees
+    // the registers it would otherwise spill.
+    enum : uint32_t { kPixel, kIteration, kSub, kPush, kTarget, kSuffix, kL = kSuffix + 3, kFields = kL + 3 };
+    LaneFields<kFields, kMergeEm> cold;
+    PG3_DEV uint32_t pixel() const { return cold.Get(kPixel); }
+    PG3_DEV uint32_t iteration() const { return cold.Get(kIteration); }
+    PG3_DEV uint32_t sub() const { return cold.Get(kSub); }
+    PG3_DEV uint32_t nPush() const { return cold.Get(kPush); }
+    PG3_DEV int32_t target() const { return (int32_t)cold.Get(kTarget); }
+    PG3_DEV F3 Vec(uint32_t f) const {
+        return mk(__uint_as_float(cold.Get(f)), __uint_as_float(cold.Get(f + 1)), __uint_as_float(cold.Get(f + 2)));
+    }
+    PG3_DEV void SetVec(uint32_t f, F3 v) {
+        cold.Set(f, __float_as_uint(v.x));
+        cold.Set(f + 1, __float_as_uint(v.y));
+        cold.Set(f + 2, __float_as_uint(v.z));
+    }
     struct Ring {
         float r[kChainRing][7];  // A.xyz, attCos.xyz, den
     };
@@ -1332,30 +1366,30 @@ struct SplitPath {
     PG3_DEV static uint32_t LightCount(const SceneView &S, uint32_t nBv) {
         return (uint32_t)lroundf(fmaxf(S.lightRatio * nBv, 1.f));
     }
-    PG3_DEV bool CountsSample() const { return sub == 0u; }
+    PG3_DEV bool CountsSample() const { return sub() == 0u; }
     PG3_DEV void Start(const SceneView &S, uint32_t px, uint32_t it, uint32_t s) {
         RngKey key;
         CameraSample(S, px, it, key, org, dir);
-        pixel = px;
-        iteration = it;
-        sub = s;
+        cold.Set(kPixel, px);
+        cold.Set(kIteration, it);
+        cold.Set(kSub, s);
         tmin = 0.f;
         len = 1;
         refl = true;
-        L = mk(0.f);
-        nPush = 0;
-        target = -1;
+        SetVec(kL, mk(0.f));
+        cold.Set(kPush, 0u);
+        cold.Set(kTarget, (uint32_t)-1);
     }
     PG3_DEV RngKey Branch(const SceneView &S, uint32_t b) const {
-        return RngKey{S.k0, S.k1, pixel, iteration, (S.sceneId << 16) | b};
+        return RngKey{S.k0, S.k1, pixel(), iteration(), (S.sceneId << 16) | b};
     }
-    // RNG branch of vertex v of this sub-sample
-    PG3_DEV uint32_t BranchOf(uint32_t v, uint32_t nB) const {
-        return (v == 1u || nB <= 1u) ? 0u : ((sub + 1u) & 0xFFFFu);
+    // RNG branch of vertex v of sub-sample s
+    PG3_DEV static uint32_t BranchOf(uint32_t v, uint32_t nB, uint32_t s) {
+        return (v == 1u || nB <= 1u) ? 0u : ((s + 1u) & 0xFFFFu);
     }
     // MIS-weighted emission that the BSDF sample of vertex len - 1 finds at vertex len
     PG3_DEV F3 ParentEmission(const SceneView &S, const LightSpec &spec, F3 em, float lpdf, int32_t lid,
-                              uint32_t nB) const {
+                              uint32_t nB, uint32_t sb) const {
         EmissionParent P;
         P.sp = org;
         P.n = pN;
@@ -1366,7 +1400,7 @@ struct SplitPath {
         P.nB = (len == 2u) ? nB : 1u;
         P.nL = LightCount(S, P.nB);
         P.len = len - 1u;
-        P.key = Branch(S, BranchOf(len - 1u, nB));
+        P.key = Branch(S, BranchOf(len - 1u, nB, sb));
         return EmissionMis(S, spec, em, lpdf, lid, P);
     }
     // End of a chain pass at wavefront step t.  The newest record is in registers and rChild is the
@@ -1377,7 +1411,7 @@ struct SplitPath {
     PG3_DEV bool Fold(const SceneView &S, uint32_t t, uint32_t jStart, F3 rChild, const Ring &ring) {
         F3 R = tA;
         if (tDen > 0.f && !IsZero(rChild)) R += ClippedIndirect(tAttCos, rChild, tDen, S.iic);
-        const uint32_t n = min(nPush, kChainRing);
+        const uint32_t np = nPush(), n = min(np, kChainRing);
 #pragma unroll 1
         for (uint32_t i = 0; i < n; ++i) {
             const float *r = ring.r[(t - jStart - i) % kChainRing];
@@ -1386,18 +1420,18 @@ struct SplitPath {
             if (den > 0.f && !IsZero(R)) Rk += ClippedIndirect(mk(r[3], r[4], r[5]), R, den, S.iic);
             R = Rk;
         }
-        if (nPush > kChainRing) {
-            target = (int32_t)(nPush - kChainRing);
-            rSuffix = R;
+        if (np > kChainRing) {
+            cold.Set(kTarget, np - kChainRing);
+            SetVec(kSuffix, R);
             RngKey key;
-            CameraSample(S, pixel, iteration, key, org, dir);
+            CameraSample(S, pixel(), iteration(), key, org, dir);
             tmin = 0.f;
             len = 1;
             refl = true;
-            nPush = 0;
+            cold.Set(kPush, 0u);
             return false;
         }
-        L += R;
+        SetVec(kL, Vec(kL) + R);
         return true;
     }
     // One vertex per step (t: the wavefront's step number, the same on every lane).
@@ -1417,7 +1451,7 @@ struct SplitPath {
         r[4] = tAttCos.y;
         r[5] = tAttCos.z;
         r[6] = tDen;
-        ++nPush;
+        cold.Set(kPush, nPush() + 1u);
     }
     // BSDF sample si of vertex len (:281-404) with nBv samples at the vertex: the indirect factors
     // of the new record, what the MIS weight of the emission it finds needs, and the next ray.
@@ -1449,13 +1483,14 @@ struct SplitPath {
     }
     PG3_DEV bool StepMerged(const SceneView &S, const LightSpec &spec, RayTally &tally, bool &shadowed, uint32_t t,
                             Ring &ring) {
-        const uint32_t nB = Subs(S);
-        const RngKey k = Branch(S, BranchOf(len, nB));
+        const uint32_t nB = Subs(S), sb = sub();
+        const int32_t tgt = target();
+        const RngKey k = Branch(S, BranchOf(len, nB, sb));
         Hit h;
         h.dist = 1e36f;
         const bool hit = Intersect(S, org, dir, tmin, h);
         const bool first = (len == 1);
-        const bool atTarget = (target >= 0) && ((int32_t)len - 1 == target);
+        const bool atTarget = (tgt >= 0) && ((int32_t)len - 1 == tgt);
         F3 em = mk(0.f);                 // emission found at vertex len (area light or EM)
         float lpdf = 0.f;
         int32_t lid = -1;
@@ -1466,7 +1501,7 @@ struct SplitPath {
         F3 qn = pN;
         uint32_t qsides = pFlags >> 4;
         float uSide = 0.f, u0 = 0.f, u1 = 0.f, prob = 1.f;
-        const uint32_t nBv = first ? nB : 1u, nLv = LightCount(S, nBv), l = first ? sub : 0u;
+        const uint32_t nBv = first ? nB : 1u, nLv = LightCount(S, nBv), l = first ? sb : 0u;
         if (hit) {                                                                    // :176-238
             sp = org + dir * h.dist;
             fr = FrameFromZ(h.n);
@@ -1510,15 +1545,15 @@ struct SplitPath {
             lid = S.bgLight;
         }
         if (first) {
-            if (sub == 0u && target < 0) L += em;      // the camera hit's emission, once per sample
+            if (sb == 0u && tgt < 0) SetVec(kL, Vec(kL) + em);   // the camera hit's emission, once per sample
         } else {
-            tA += ParentEmission(S, spec, em, lpdf, lid, nB);
+            tA += ParentEmission(S, spec, em, lpdf, lid, nB, sb);
         }
         if (!shade) {
             if (first) return true;
-            return Fold(S, t, 1u, atTarget ? rSuffix : mk(0.f), ring);
+            return Fold(S, t, 1u, atTarget ? Vec(kSuffix) : mk(0.f), ring);
         }
-        if (target < 0 && (!first || sub == 0u)) ++tally.segments;   // the shared first vertex once
+        if (tgt < 0 && (!first || sb == 0u)) ++tally.segments;   // the shared first vertex once
         if (!first) Push(t, ring);                      // the parent's record is complete
         // the light sample's contribution (path_tracer_base.hxx:350-425); its shadow ray is traced
         // after the BSDF sample
@@ -1541,15 +1576,16 @@ struct SplitPath {
         }
         tA = mk(0.f);
         pLcSum = 0.f;
-        const bool more = SampleNext(S, k, first ? sub : 0u, nBv, sp, fr, wol, mat);
+        const bool more = SampleNext(S, k, first ? sb : 0u, nBv, sp, fr, wol, mat);
         shadowed = sv;
         if (sv && !Occluded(S, sp, sdir, smax)) tA = contrib;
         if (!more) return Fold(S, t, 0u, mk(0.f), ring);
         return false;
     }
     PG3_DEV bool StepGeneric(const SceneView &S, const LightSpec &spec, RayTally &tally, uint32_t t, Ring &ring) {
-        const uint32_t nB = Subs(S);
-        const RngKey k = Branch(S, BranchOf(len, nB));
+        const uint32_t nB = Subs(S), sb = sub();
+        const int32_t tgt = target();
+        const RngKey k = Branch(S, BranchOf(len, nB, sb));
         Hit h;
         h.dist = 1e36f;
         const bool hit = Intersect(S, org, dir, tmin, h);
@@ -1580,31 +1616,31 @@ struct SplitPath {
             lid = S.bgLight;
         }
         if (len == 1) {
-            if (sub == 0u && target < 0) L += em;      // the camera hit's emission, once per sample
+            if (sb == 0u && tgt < 0) SetVec(kL, Vec(kL) + em);   // the camera hit's emission, once per sample
         } else {
-            tA += ParentEmission(S, spec, em, lpdf, lid, nB);
+            tA += ParentEmission(S, spec, em, lpdf, lid, nB, sb);
         }
-        const bool atTarget = (target >= 0) && ((int32_t)len - 1 == target);
+        const bool atTarget = (tgt >= 0) && ((int32_t)len - 1 == tgt);
         const bool stop = atTarget || !hit || !refl || ReflectanceZero(mat) || (S.maxPL > 0 && len >= S.maxPL) ||
                           len >= 1200u;
         if (stop) {
             if (len == 1) return true;
-            return Fold(S, t, 1u, atTarget ? rSuffix : mk(0.f), ring);
+            return Fold(S, t, 1u, atTarget ? Vec(kSuffix) : mk(0.f), ring);
         }
         const bool first = (len == 1);
-        if (target < 0 && (!first || sub == 0u)) ++tally.segments;   // the shared first vertex once
+        if (tgt < 0 && (!first || sb == 0u)) ++tally.segments;   // the shared first vertex once
         if (!first) Push(t, ring);                      // the parent's record is complete
         LightCache lc;
         lc.sum = 0.f;
         if (spec.numLights > 1) FillLightCache(S, spec, sp, fr, mat.sides, k, len, lc);
         const uint32_t nBv = first ? nB : 1u;
-        tA = LightSampleSum(S, spec, k, len, sp, fr, wol, mat, first ? sub : 0u, nBv, LightCount(S, nBv), nBv, lc,
+        tA = LightSampleSum(S, spec, k, len, sp, fr, wol, mat, first ? sb : 0u, nBv, LightCount(S, nBv), nBv, lc,
                             tally.shadow);
         pLcSum = lc.sum;
-        if (!SampleNext(S, k, first ? sub : 0u, nBv, sp, fr, wol, mat)) return Fold(S, t, 0u, mk(0.f), ring);
+        if (!SampleNext(S, k, first ? sb : 0u, nBv, sp, fr, wol, mat)) return Fold(S, t, 0u, mk(0.f), ring);
         return false;
     }
-    PG3_DEV F3 Result() const { return L; }
+    PG3_DEV F3 Result() const { return Vec(kL); }
 };
 
 // Item source of one scene: items chunk * numTiles + tile from the launch's work counter.
