@@ -126,7 +126,7 @@ RenderPTKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nI
                                                          tilesX,    0u,       0u};
     src.SetTail(numChunks);
     if constexpr (kSplit) TraceTasks<SplitPath<kOneLight && kEmKind == kEmSteerable>, false>(src, tally);
-    else TraceTasks<ChainPath<kOneLight && kEmKind == kEmSteerable>, false>(src, tally);
+    else TraceTasks<ChainPath<kOneLight && kEmKind == kEmSteerable, kOneLight && kEmKind == kEmSteerable>, false>(src, tally);
     FlushTally(tally, counters);
 }
 

@@ -10,6 +10,7 @@
 // recursion walker RadiancePTTree.
 #pragma once
 
+#include <type_traits>
 #include "dev_scene.hpp"
 
 namespace pg3 {
@@ -1240,21 +1241,67 @@ PG3_DEV void TraceTasks(Src &src, RayTally &tally) {
     }
 }
 
+// Per-lane fields of a task.  kLds: in LDS, one slot per lane of a workgroup of at most
+// kLaneFieldLanes lanes (field i of lane x at i * kLaneFieldLanes + x: consecutive lanes,
+// consecutive banks); else in registers.
+constexpr uint32_t kLaneFieldLanes = 1024;
+template <uint32_t N, bool kLds>
+struct LaneFields {
+    uint32_t v[N];
+    PG3_DEV uint32_t Get(uint32_t i) const { return v[i]; }
+    PG3_DEV void Set(uint32_t i, uint32_t x) { v[i] = x; }
+};
+template <uint32_t N>
+struct LaneFields<N, true> {
+    PG3_DEV static uint32_t *Base() {
+        __shared__ uint32_t fields[N * kLaneFieldLanes];
+        return fields;
+    }
+    PG3_DEV uint32_t Get(uint32_t i) const { return Base()[i * kLaneFieldLanes + threadIdx.x]; }
+    PG3_DEV void Set(uint32_t i, uint32_t x) { Base()[i * kLaneFieldLanes + threadIdx.x] = x; }
+};
+
 // A lane's sample of the path tracer with -sb 1 -iic 0: the chain of ShadeVertex, one vertex
 // (extension ray, light sample and its shadow ray, BSDF sample) per step.
-template <bool kMergeEm>
+// kLds: the sample's pixel and iteration and its radiance so far live in LDS (LaneFields); a step
+// rebuilds the RNG key from them and the scene, and adds the radiance it finds to the LDS sum.
+template <bool kMergeEm, bool kLds = false>
 struct ChainPath {
-    RngKey key;
+    struct Keyed {                      // kLds = false: the RNG key in registers
+        RngKey key;
+    };
+    struct Unkeyed {};
+    using KeyStore = typename std::conditional<kLds, Unkeyed, Keyed>::type;
+    enum : uint32_t { kPixel, kIteration, kL, kFields = kL + 3 };
+    KeyStore ks;
+    LaneFields<kLds ? kFields : 1u, kLds> cold;
     PathState ps;
     struct Ring {};
     PG3_DEV static constexpr uint32_t Subs(const SceneView &) { return 1u; }
     PG3_DEV static constexpr bool CountsSample() { return true; }
     PG3_DEV void Start(const SceneView &S, uint32_t pixel, uint32_t iteration, uint32_t) {
         F3 org, dir;
+        RngKey key;
         CameraSample(S, pixel, iteration, key, org, dir);
         InitPath(ps, org, dir);
+        if constexpr (kLds) {
+            cold.Set(kPixel, pixel);
+            cold.Set(kIteration, iteration);
+            cold.Set(kL, 0u);
+            cold.Set(kL + 1, 0u);
+            cold.Set(kL + 2, 0u);
+        } else {
+            ks.key = key;
+        }
     }
     PG3_DEV bool Step(const SceneView &S, const LightSpec &spec, RayTally &t, bool &shadowed, uint32_t, Ring &) {
+        RngKey key;
+        if constexpr (kLds) {
+            key = RngKey{S.k0, S.k1, cold.Get(kPixel), cold.Get(kIteration), S.sceneId << 16};
+            ps.L = mk(0.f);
+        } else {
+            key = ks.key;
+        }
         Hit h;
         h.dist = 1e36f;
         const bool hit = Intersect(S, ps.org, ps.dir, ps.tmin, h);
@@ -1262,9 +1309,21 @@ struct ChainPath {
         const bool cont = ShadeVertex<kMergeEm>(S, spec, key, ps, hit, h, sr, t.segments);
         shadowed = sr.valid;
         if (sr.valid && !Occluded(S, sr.p, sr.dir, sr.tmax)) ps.L += sr.contrib;
+        if constexpr (kLds) {
+            if (!IsZero(ps.L)) {
+                cold.Set(kL, __float_as_uint(__uint_as_float(cold.Get(kL)) + ps.L.x));
+                cold.Set(kL + 1, __float_as_uint(__uint_as_float(cold.Get(kL + 1)) + ps.L.y));
+                cold.Set(kL + 2, __float_as_uint(__uint_as_float(cold.Get(kL + 2)) + ps.L.z));
+            }
+        }
         return !cont;
     }
-    PG3_DEV F3 Result() const { return ps.L; }
+    PG3_DEV F3 Result() const {
+        if constexpr (kLds)
+            return mk(__uint_as_float(cold.Get(kL)), __uint_as_float(cold.Get(kL + 1)), __uint_as_float(cold.Get(kL + 2)));
+        else
+            return ps.L;
+    }
 };
 
 // A lane's share of one sample of EstimateIncomingRadiancePT (path_tracer.hxx:167-430) with
@@ -1294,26 +1353,6 @@ struct ChainPath {
 // share one pass of EM lookups (SteerableEmQuery), and the light sample's shadow ray is traced at
 // the end of the step, after the BSDF sample, when little else is live; otherwise the light
 // samples of a vertex run in a loop with their shadow rays (LightSampleSum).
-// Per-lane fields of a task.  kLds: in LDS, one slot per lane of a workgroup of at most
-// kLaneFieldLanes lanes (field i of lane x at i * kLaneFieldLanes + x: consecutive lanes,
-// consecutive banks); else in registers.
-constexpr uint32_t kLaneFieldLanes = 1024;
-template <uint32_t N, bool kLds>
-struct LaneFields {
-    uint32_t v[N];
-    PG3_DEV uint32_t Get(uint32_t i) const { return v[i]; }
-    PG3_DEV void Set(uint32_t i, uint32_t x) { v[i] = x; }
-};
-template <uint32_t N>
-struct LaneFields<N, true> {
-    PG3_DEV static uint32_t *Base() {
-        __shared__ uint32_t fields[N * kLaneFieldLanes];
-        return fields;
-    }
-    PG3_DEV uint32_t Get(uint32_t i) const { return Base()[i * kLaneFieldLanes + threadIdx.x]; }
-    PG3_DEV void Set(uint32_t i, uint32_t x) { Base()[i * kLaneFieldLanes + threadIdx.x] = x; }
-};
-
 #ifndef PG3_CHAIN_RING
 #define PG3_CHAIN_RING 32
 #endif
