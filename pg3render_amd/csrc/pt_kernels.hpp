@@ -394,6 +394,11 @@ PG3_DEV bool ShadeVertexSteerable(const SceneView &S, const LightSpec &spec, con
         return false;
     }
     if (!shade) return false;
+    // The material is loaded again rather than kept in registers through the EM lookups (the
+    // barrier keeps the compiler from reusing the first load).
+    __asm__ volatile("" ::: "memory");
+    mat = LoadGlobal(S.mats + h.mat);
+    NarrowMaterial<kCls>(mat);
     if (q.lsOk) LightSampleRequest(ps, mat, fr, wol, sp, q.ls, sr);
     return ContinuePath(S, ps, sp, fr, wol, mat, vr, lc.sum);
 }
