@@ -40,6 +40,7 @@ def load():
         "oracle_em_eval": (None, [C.c_void_p, C.c_uint32, _FP, _FP]),
         "oracle_em_eval_uv": (None, [C.c_void_p, C.c_uint32, _FP, _FP]),
         "oracle_em_avg_pixel_lum": (C.c_float, [C.c_void_p, C.c_uint32, C.c_uint32]),
+        "oracle_estimate_contribution": (None, [C.c_void_p, C.c_int32, C.c_uint32, _FP, _FP, _IP, C.c_uint32, _FP]),
         "oracle_camera_ray": (None, [C.c_void_p, C.c_float, C.c_float, _FP, _FP]),
         "oracle_intersect_batch": (None, [C.c_void_p, C.c_uint32, _FP, _FP, _FP, _FP, _FP, _IP]),
         "oracle_occluded_batch": (None, [C.c_void_p, C.c_uint32, _FP, _FP, _FP, _IP]),
@@ -137,6 +138,17 @@ class OracleScene:
         rgb = np.zeros((len(uv), 3), np.float32)
         self.lib.oracle_em_eval_uv(self.ptr, len(uv), fp(uv), fp(rgb))
         return rgb
+
+    def estimate_contribution(self, light_index, points, normals, sides, iteration=0):
+        """Light-picking estimate of one light at surface points (AreaLight::EstimateContribution,
+        lights.hxx:133-167; EnvironmentMap::EstimateIrradiance, em.hxx:123-170)."""
+        points, normals = f32(points), f32(normals)
+        n = len(points)
+        sides = np.ascontiguousarray(np.broadcast_to(np.asarray(sides, np.int32), (n,)))
+        out = np.zeros(n, np.float32)
+        self.lib.oracle_estimate_contribution(self.ptr, light_index, n, fp(points), fp(normals),
+                                              sides.ctypes.data_as(_IP), iteration, fp(out))
+        return out
 
     def camera_ray(self, x, y):
         o = np.zeros(3, np.float32); d = np.zeros(3, np.float32)

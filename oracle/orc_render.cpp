@@ -770,6 +770,21 @@ void oracle_em_eval_uv(void *sp, uint32_t n, const float *uv, float *rgb) {
         rgb[i * 3] = v.x; rgb[i * 3 + 1] = v.y; rgb[i * 3 + 2] = v.z;
     }
 }
+// The light-picking estimate of light `li` (AreaLight::EstimateContribution, lights.hxx:133-167;
+// BackgroundLight / EnvironmentMap::EstimateIrradiance, lights.hxx:451-467, em.hxx:123-170) at n
+// surface points with normals and side flags (bit 0 front, bit 1 back).  Point i draws its random
+// numbers from pixel i, iteration `iteration`, vertex 1 of the scene's seed.
+void oracle_estimate_contribution(void *sp, int32_t li, uint32_t n, const float *pts, const float *normals,
+                                  const int32_t *flags, uint32_t iteration, float *out) {
+    const Scene &S = *static_cast<Scene *>(sp);
+    Material m;
+    for (uint32_t i = 0; i < n; ++i) {
+        Frame f; f.SetFromZ(V3(normals[i * 3], normals[i * 3 + 1], normals[i * 3 + 2]));
+        m.front = flags[i] & 1; m.back = (flags[i] & 2) != 0;
+        PathRng rng{S.params.seed, i, iteration, 0, 0};
+        out[i] = S.EstimateContribution(li, V3(pts[i * 3], pts[i * 3 + 1], pts[i * 3 + 2]), f, m, rng, 1);
+    }
+}
 float oracle_em_avg_pixel_lum(void *sp, uint32_t x, uint32_t y) { return static_cast<Scene *>(sp)->em.AvgPixelLum(x, y); }
 void oracle_camera_ray(void *sp, float x, float y, float *org, float *dir) {
     V3 o, d; static_cast<Scene *>(sp)->GenerateRay(x, y, o, d);

@@ -74,8 +74,8 @@ def test_bsdf_eval_parity(ctx, name, mat, both):
     assert np.all(cg == co)
     tier1_report(f"{name} eval f", ef)
     tier1_report(f"{name} eval pdf", ep)
-    assert np.quantile(ef, 0.999) <= REL and np.quantile(ep, 0.999) <= REL, (ef.max(), ep.max())
-    assert ef.max() <= 1e-3 and ep.max() <= 1e-3
+    # every sample within the tier-1 bound (measured: at most 3e-7)
+    assert ef.max() <= REL and ep.max() <= REL, (ef.max(), ep.max())
 
 
 @pytest.mark.parametrize("name,mat,both", MATERIALS, ids=[m[0] for m in MATERIALS])
@@ -175,7 +175,10 @@ def test_em_parity(ctx, sampler):
     er = rel_err(rg, ro, 1e-3).max(axis=1)[good]
     for nm, e in (("direction", ed), ("pdf", ep), ("radiance*cos", er)):
         tier1_report(f"sampler {sampler} EM sample {nm}", e)
-    for e in (ed, ep, er):
+    # sampled directions: every sample (measured: at most 5e-7); pdf and radiance*cos inherit the
+    # one-ulp differences of the lat-long mapping's atan2/acos next to the sun's steep gradient
+    assert ed.max() <= REL, ed.max()
+    for e in (ep, er):
         assert np.quantile(e, 0.995) <= REL * 2, e.max()
         assert np.mean(e > 1e-3) < 2e-3
     # pdf of arbitrary directions

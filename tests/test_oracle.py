@@ -873,3 +873,147 @@ def test_splitting_configurations_converge_to_one_image():
     print(f"block means: median relative difference {np.median(rel):.3f}, image {b.mean() / a.mean() - 1:+.4f}")
     assert np.median(rel) < 0.03
     assert abs(a.mean() - b.mean()) / a.mean() < 0.015
+
+
+# ---------------------------------------------------------------------------------------------
+# Light-picking estimates (PathTracerBase::PickSingleLight weights): they set probabilities only,
+# so the expectation of a render does not depend on them, and every check here is on the values.
+LUM = np.array([0.212671, 0.715160, 0.072169])
+
+
+def _area_light_scene(p0, p1, p2, power):
+    return CustomScene(materials=[material(_ffi.MAT_LAMBERT, diffuse=(0.5, 0.5, 0.5))],
+                       lights=[light(_ffi.LIGHT_AREA, p0, p1, p2, power)], camera={"res": (8, 8)})
+
+
+def test_area_light_contribution_small_triangle_limit():
+    """AreaLight::EstimateContribution (lights.hxx:133-167, ComputeSample :171-219) averages
+    Lum(Le) cos_in / pdf_W over four points of the triangle, pdf_W = d^2 / (A cos_out).  For a
+    triangle that is small against its distance every point gives the same value, so the estimate
+    is the irradiance of a small source, Lum(Le) A cos_in cos_out / d^2 with Le = power / (A pi).
+    A dropped cosine, a distance instead of its square or a missing area fails."""
+    rng = np.random.default_rng(41)
+    power = np.array([3.0, 5.0, 2.0])
+    s = 2e-3
+    p0 = np.array([0.1, -0.2, 0.3])
+    e1, e2 = np.array([s, 0.0, 0.0]), np.array([0.3 * s, s, 0.0])   # normal +z
+    area = 0.5 * np.linalg.norm(np.cross(e1, e2))
+    o = O.OracleScene(_area_light_scene(p0, p0 + e1, p0 + e2, power))
+    le_lum = float(power @ LUM) / (area * math.pi)
+    n = 2000
+    centre = p0 + (e1 + e2) / 3
+    up = _sphere_dirs(rng, n).astype(np.float64)
+    up[:, 2] = np.abs(up[:, 2]) + 0.05                      # points above the light (cos_out > 0)
+    up /= np.linalg.norm(up, axis=1, keepdims=True)
+    dist = rng.uniform(0.5, 2.0, n)
+    pts = centre + up * dist[:, None]
+    nrm = -up + 0.6 * _sphere_dirs(rng, n)                  # surfaces facing the light, tilted
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    cos_in = np.einsum("ij,ij->i", nrm, -up)
+    keep = cos_in > 0.05
+    pts, nrm, up, dist, cos_in = pts[keep], nrm[keep], up[keep], dist[keep], cos_in[keep]
+    got = o.estimate_contribution(0, pts, nrm, 1)
+    want = le_lum * area * cos_in * up[:, 2] / dist ** 2
+    err = np.abs(got - want) / want
+    print(f"small triangle: max rel error {err.max():.2e} over {len(want)} points")
+    assert err.max() < 5e-3
+    # a surface that faces away sees nothing with front-side sampling; both sides use |cos_in|
+    # and back-side sampling alone the flipped normal (ComputeSample's side rules, :186-202)
+    assert np.all(o.estimate_contribution(0, pts, -nrm, 1) == 0)
+    np.testing.assert_allclose(o.estimate_contribution(0, pts, -nrm, 3), got, rtol=1e-6)
+    np.testing.assert_allclose(o.estimate_contribution(0, pts, -nrm, 2), got, rtol=1e-6)
+    # points behind the emitting side (cos_out <= 0) get nothing
+    behind = centre - up * dist[:, None]
+    assert np.all(o.estimate_contribution(0, behind, up, 3) == 0)
+
+
+def test_area_light_contribution_points_of_the_rule():
+    """The four points of the rule are the three vertices and p0 + 0.33 (e1 + e2): a surface point
+    just above one of them (height h, facing it) receives Lum(Le) A / h^2 from that point alone,
+    so the estimate is a quarter of it.  Just above the true centroid (p0 + (e1 + e2) / 3) the
+    estimate is far smaller: the rule's fourth point is 0.0047 away, not at h = 1e-3."""
+    power = np.array([1.0, 1.0, 1.0])
+    p0, e1, e2 = np.zeros(3), np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0])
+    o = O.OracleScene(_area_light_scene(p0, p0 + e1, p0 + e2, power))
+    area = 0.5
+    le_lum = float(power @ LUM) / (area * math.pi)
+    h = 1e-3
+    down = np.array([[0.0, 0.0, -1.0]])
+    for q in (p0, p0 + e1, p0 + e2, p0 + 0.33 * (e1 + e2)):
+        got = float(o.estimate_contribution(0, (q + [0, 0, h])[None], down, 1)[0])
+        want = le_lum * area / h ** 2 / 4
+        assert abs(got - want) < 1e-3 * want, (q, got, want)
+    c = p0 + (e1 + e2) / 3
+    got = float(o.estimate_contribution(0, (c + [0, 0, h])[None], down, 1)[0])
+    assert got < 0.2 * le_lum * area / h ** 2 / 4
+
+
+def _irradiance_quadrature(o, normal, both, n_theta=400, n_phi=800):
+    """Integral of Lum(EM(w)) |cos| over the sampled hemisphere(s), midpoint rule in float64 with
+    the EM evaluated by the (pinned) tent-filtered lookup."""
+    th = (np.arange(n_theta) + 0.5) * math.pi / n_theta
+    ph = (np.arange(n_phi) + 0.5) * 2 * math.pi / n_phi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    d = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    dw = (np.sin(T) * (math.pi / n_theta) * (2 * math.pi / n_phi)).reshape(-1)
+    c = d @ normal
+    w = np.abs(c) if both else np.maximum(c, 0.0)
+    lum = o.em_eval(d.astype(np.float32)).astype(np.float64) @ LUM
+    return float(np.sum(lum * w * dw))
+
+
+def _gradient_em(h=32, w=64):
+    yy, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    img = 0.3 + 1.2 * (1 - yy / h)[..., None] * np.array([0.6, 0.8, 1.0])
+    img = img + 0.5 * np.cos(2 * np.pi * xx / w)[..., None] ** 2
+    return img.astype(np.float32)
+
+
+@pytest.mark.parametrize("sampler", [_ffi.EM_SIMPLE, _ffi.EM_COSINE])
+def test_em_irradiance_estimate_matches_quadrature(sampler):
+    """EnvironmentMap::EstimateIrradiance without a sampler of its own (em.hxx:123-170): ten rounds
+    of a cosine sample and an EM-proportional sample combined by the balance heuristic, an
+    estimate of the integral of Lum(L_e) |cos| over the sampled side(s).  Its mean over
+    independent draws equals a quadrature of that integral (constant EM: pi Lum per side).  A
+    dropped cosine, a one-sided pdf or a missing MIS term biases it."""
+    rng = np.random.default_rng(43)
+    for em, tol_name in ((np.full((64, 128, 3), 0.7, np.float32), "constant"), (_gradient_em(64, 128), "gradient")):
+        sc = CustomScene(materials=[material(_ffi.MAT_LAMBERT, diffuse=(0.5, 0.5, 0.5))],
+                         lights=[light(_ffi.LIGHT_BACKGROUND)], em=em, em_sampler=sampler, camera={"res": (8, 8)})
+        o = O.OracleScene(sc)
+        for normal in (np.array([0.0, 0.0, 1.0]), direction(1.2, 0.7), direction(2.6, -2.0)):
+            for sides in (1, 3):
+                n = 20000
+                est = o.estimate_contribution(0, np.zeros((n, 3)), np.tile(normal, (n, 1)), sides).astype(np.float64)
+                mean, se = est.mean(), est.std() / math.sqrt(n)
+                want = _irradiance_quadrature(o, normal.astype(np.float64), sides == 3)
+                if tol_name == "constant":
+                    assert abs(want - 0.7 * LUM.sum() * math.pi * (2 if sides == 3 else 1)) < 1e-3 * want
+                print(f"{tol_name} normal {np.round(normal, 2)} sides {sides}: mean {mean:.5f} +- {se:.5f}, "
+                      f"quadrature {want:.5f}")
+                # the EM-proportional sample's pdf is the texel's (its sin(theta) taken at the
+                # texel centre), so the estimator carries a bias that shrinks with the texel size:
+                # 0.1-0.16 % at 32x16 texels, 0.02 % at 128x64
+                assert abs(mean - want) < 4 * se + 1e-3 * want
+
+
+def test_steerable_irradiance_estimate_is_the_whole_integral():
+    """With the steerable sampler EstimateIrradiance is the sampler's own (:2304-2358): the SH
+    whole integral of the clamped cosine around the normal, for the back side around the flipped
+    normal, summed when both sides are sampled.  It equals the integral recomputed from the leaf
+    triangles, and for a constant EM the sphere integral of the lobe's constant band."""
+    for img, const in ((np.full((16, 32, 3), 0.7, np.float32), True), (_gradient_em(16, 32), False)):
+        o = O.OracleScene(_steer_scene(img))
+        em_lum = (lambda v: np.full(len(v), 0.7 * LUM.sum())) if const else \
+            (lambda v: o.em_eval(v) @ LUM)
+        for normal in (np.array([0, 0, 1], np.float32), direction(1.1, 0.4), direction(2.5, -1.0)):
+            w_up = _whole_integral_from_leaves(o, normal, em_lum)
+            w_dn = _whole_integral_from_leaves(o, -normal, em_lum)
+            for sides, want in ((1, w_up), (2, w_dn), (3, w_up + w_dn)):
+                got = float(o.estimate_contribution(0, np.zeros((1, 3)), normal[None], sides)[0])
+                assert abs(got - want) < 1e-4 * want, (sides, got, want)
+            if const:
+                # only the constant SH band survives the sphere integral of the (compensated)
+                # clamped-cosine lobe: Lum c0 Y00 4 pi, up to the planar triangulation
+                cc0 = float(O.clamped_cos(normal, True)[0])
+                assert abs(w_up - 0.7 * LUM.sum() * cc0 * 0.282095 * 4 * math.pi) < 0.02 * w_up
