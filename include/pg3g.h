@@ -191,6 +191,13 @@ int pg3g_read_counters(pg3g_context *ctx, uint64_t *segments, uint64_t *camera_s
  * The differentiable kernels (pg3g_render_diff, pg3g_adjoint) count segments and samples only. */
 int pg3g_read_ray_counters(pg3g_context *ctx, uint64_t *extension_rays, uint64_t *shadow_rays);
 int pg3g_reset_counters(pg3g_context *ctx);
+/* Primitives of an uploaded slot as the intersection kernels see them: axis-aligned rectangles
+ * (pairs of the scene's triangles merged at upload: the reference's box walls, src/scene.hxx:
+ * 459-488, tested as one primitive with the triangles' own hit distance, normal and material),
+ * the remaining triangles and the spheres (GeometryList, src/scene_graph.hxx:33-82).  Any output
+ * pointer may be null.  Introspection of the upload; no counterpart in the reference. */
+int pg3g_scene_primitives(pg3g_context *ctx, uint32_t slot, uint32_t *rectangles, uint32_t *triangles,
+                          uint32_t *spheres);
 
 /* ---- function-level entry points (tier-1 parity; host arrays, n records) ----------------- */
 /* AbstractMaterial::EvalBsdf with kOptSamplingProbs (materials.hxx:228-229): wil/wol local

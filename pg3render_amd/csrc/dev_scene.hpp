@@ -20,6 +20,20 @@ struct alignas(16) DevTri {
     int32_t mat;
     int32_t pad[3];
 };
+// Axis-aligned rectangle made of two triangles of the scene (the walls of the reference's boxes,
+// scene.hxx:459-488): both triangles lie in the plane x_A = const, have bit-identical normals
+// (exactly zero off axis A) and plane distances, the same material, and together cover
+// [lo0, hi0] x [lo1, hi1] along the axes (A + 1) % 3 and (A + 2) % 3.  The plane distance of a hit
+// takes the triangles' own float operations, so hit distance, normal and material are the
+// triangles' bit for bit; one bounds test replaces the two triangles' edge tests, which differ from
+// it only for rays within a few ulp of the rectangle's border.  48 B, read with scalar loads.
+struct alignas(16) DevQuad {
+    float n[3];
+    float np0;
+    float lo0, hi0, lo1, hi1;
+    int32_t mat;
+    int32_t pad[3];
+};
 struct alignas(16) DevSphere {
     float c[3], r;
     int32_t mat;
@@ -70,7 +84,9 @@ __host__ __device__ constexpr uint32_t ClassOfType(int32_t t) {
 }
 
 struct SceneView {
-    const DevTri *tris;
+    const DevQuad *quads;      // rectangles normal to x, then y, then z: [0, quadEnd[0]), ...
+    uint32_t quadEnd[3];
+    const DevTri *tris;        // the triangles not merged into rectangles, in scene order
     const DevSphere *sphs;
     const DevMaterial *mats;
     const DevLight *lights;
@@ -209,6 +225,36 @@ PG3_DEV DevTri LoadTri(const DevTri *base, uint32_t i) {
     t.mat = __float_as_int(f[22]);
     return t;
 }
+PG3_DEV DevQuad LoadQuad(const DevQuad *base, uint32_t i) {
+    const ConstPtr<F4v> q = (ConstPtr<F4v>)AsConst(base + i);
+    const F4v a = q[0], b = q[1], c = q[2];
+    DevQuad r;
+    r.n[0] = a.x; r.n[1] = a.y; r.n[2] = a.z; r.np0 = a.w;
+    r.lo0 = b.x; r.hi0 = b.y; r.lo1 = b.z; r.hi1 = b.w;
+    r.mat = __float_as_int(c.x);
+    return r;
+}
+template <int A> PG3_DEV float Comp(F3 v) { return A == 0 ? v.x : A == 1 ? v.y : v.z; }
+// Hit of the rectangle normal to axis A.  The distance is FastDiv(np0 - Dot(n, org), Dot(n, dir))
+// as in TriHit: with n zero off axis A each dot product is the single rounded product of the A
+// components (a product with zero adds nothing), so it is computed as that product, without
+// contraction (np0 - n_A o_A is rounded after the product, as the dot product is).
+template <int A>
+PG3_DEV bool QuadHit(const DevQuad &q, F3 org, F3 dir, float tmin, Hit &h) {
+    const float na = q.n[A];
+    const float d = FastDiv(q.np0 - na * Comp<A>(org), na * Comp<A>(dir));
+    if ((d > tmin) & (d < h.dist)) {
+        const float pu = Comp<(A + 1) % 3>(org) + d * Comp<(A + 1) % 3>(dir);
+        const float pv = Comp<(A + 2) % 3>(org) + d * Comp<(A + 2) % 3>(dir);
+        if ((pu >= q.lo0) & (pu <= q.hi0) & (pv >= q.lo1) & (pv <= q.hi1)) {
+            h.dist = d;
+            h.mat = q.mat;
+            h.n = mk(q.n[0], q.n[1], q.n[2]);
+            return true;
+        }
+    }
+    return false;
+}
 PG3_DEV DevSphere LoadSphere(const DevSphere *base, uint32_t i) {
     const ConstPtr<F4v> q = (ConstPtr<F4v>)AsConst(base + i);
     const F4v a = q[0], b = q[1];
@@ -224,6 +270,12 @@ PG3_DEV F3 RayMoment(F3 org, F3 dir) {
 }
 PG3_DEV bool Intersect(const SceneView &S, F3 org, F3 dir, float tmin, Hit &h) {  // scene.hxx:42-59
     bool any = false;
+    for (uint32_t i = 0; i < S.quadEnd[2]; ++i) {
+        const DevQuad q = LoadQuad(S.quads, i);
+        if (i < S.quadEnd[0]) any |= QuadHit<0>(q, org, dir, tmin, h);
+        else if (i < S.quadEnd[1]) any |= QuadHit<1>(q, org, dir, tmin, h);
+        else any |= QuadHit<2>(q, org, dir, tmin, h);
+    }
     const F3 m = RayMoment(org, dir);
     for (uint32_t i = 0; i < S.numTris; ++i) {
         const DevTri t = LoadTri(S.tris, i);
@@ -239,6 +291,12 @@ PG3_DEV bool Occluded(const SceneView &S, F3 p, F3 dir, float tmax) {           
     const F3 org = p + dir * kEpsRay;
     Hit h;
     h.dist = tmax - 2 * kEpsRay;
+    for (uint32_t i = 0; i < S.quadEnd[2]; ++i) {
+        const DevQuad q = LoadQuad(S.quads, i);
+        if (i < S.quadEnd[0] ? QuadHit<0>(q, org, dir, 0.f, h)
+            : i < S.quadEnd[1] ? QuadHit<1>(q, org, dir, 0.f, h) : QuadHit<2>(q, org, dir, 0.f, h))
+            return true;
+    }
     const F3 m = RayMoment(org, dir);
     for (uint32_t i = 0; i < S.numTris; ++i) {
         const DevTri t = LoadTri(S.tris, i);

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -61,7 +62,7 @@ struct SceneSlot {
     bool valid = false;
     SceneView view{};
     pg3g_render_params params{};
-    DevBuf tris, sphs, mats, lights, texels, lum, margCdf, condCdf, rowInt, nodeW, leafVerts, fb;
+    DevBuf quads, tris, sphs, mats, lights, texels, lum, margCdf, condCdf, rowInt, nodeW, leafVerts, fb;
     DevBuf viewDev;      // device copy of `view`, refreshed on upload and parameter changes
     // differentiable rendering
     std::vector<pg3g_material> hostMats;      // sampling materials as given (ctor arguments)
@@ -106,6 +107,7 @@ struct pg3g_batch {
     std::vector<uint32_t> slots, spp;
     std::vector<BatchScene> info;
     DevBuf views, infoDev, work;
+    uint32_t gridBlocks = 0, blockLanes = 0;
     pg3g_batch_config cfg{8, 16, 1};
     uint32_t numItems = 0;
     uint32_t passes = 0;       // passes rendered since creation / the last autotune
@@ -230,6 +232,67 @@ hipError_t Upload(DevBuf &buf, const T *src, size_t count, hipStream_t s) {
     return hipMemcpyAsync(buf.p, src, sizeof(T) * count, hipMemcpyHostToDevice, s);
 }
 
+// Rectangles: triangles i and i + 1 of the scene (the two halves of a quad, as the reference's
+// boxes list their walls) merge into one DevQuad when both lie in a plane x_A = const, their
+// normals are bit-identical and exactly zero off axis A, their plane distances and materials are
+// equal, they share a diagonal, and their four corners are the corners of an axis-aligned
+// rectangle.  PG3G_TRIANGLES_ONLY=1 in the environment keeps every triangle (for comparisons).
+bool MergeRectangles() {
+    const char *e = std::getenv("PG3G_TRIANGLES_ONLY");
+    return !(e && e[0] == '1');
+}
+bool RectangleOf(const pg3g_scene_desc *d, const std::vector<DevTri> &tris, uint32_t i, DevQuad &q, int &axis) {
+    const DevTri &a = tris[i], &b = tris[i + 1];
+    if (a.mat != b.mat || std::memcmp(a.n, b.n, sizeof(a.n)) != 0 || std::memcmp(&a.np0, &b.np0, 4) != 0) return false;
+    axis = -1;
+    for (int k = 0; k < 3; ++k) {
+        if (a.n[k] == 0.f) continue;
+        if (axis >= 0) return false;
+        axis = k;
+    }
+    if (axis < 0) return false;
+    const int u = (axis + 1) % 3, w = (axis + 2) % 3;
+    float P[6][3];
+    std::memcpy(P, d->triangles + (size_t)i * 9, sizeof(P));
+    for (int j = 1; j < 6; ++j)
+        if (P[j][axis] != P[0][axis]) return false;
+    // corners: the distinct points of the two triangles
+    float C[6][3];
+    int nc = 0, shared = 0;
+    for (int j = 0; j < 6; ++j) {
+        bool seen = false;
+        for (int c = 0; c < nc; ++c) seen |= std::memcmp(C[c], P[j], sizeof(C[c])) == 0;
+        if (seen) { ++shared; continue; }
+        std::memcpy(C[nc++], P[j], sizeof(C[0]));
+    }
+    if (nc != 4 || shared != 2) return false;
+    const float lo0 = std::min(std::min(C[0][u], C[1][u]), std::min(C[2][u], C[3][u]));
+    const float hi0 = std::max(std::max(C[0][u], C[1][u]), std::max(C[2][u], C[3][u]));
+    const float lo1 = std::min(std::min(C[0][w], C[1][w]), std::min(C[2][w], C[3][w]));
+    const float hi1 = std::max(std::max(C[0][w], C[1][w]), std::max(C[2][w], C[3][w]));
+    if (!(lo0 < hi0 && lo1 < hi1)) return false;
+    // each corner of [lo0, hi0] x [lo1, hi1] appears once
+    int mask = 0;
+    for (int c = 0; c < 4; ++c) {
+        const bool uLo = C[c][u] == lo0, uHi = C[c][u] == hi0, wLo = C[c][w] == lo1, wHi = C[c][w] == hi1;
+        if (!(uLo || uHi) || !(wLo || wHi)) return false;
+        mask |= 1 << ((uHi ? 1 : 0) + (wHi ? 2 : 0));
+    }
+    if (mask != 15) return false;
+    // the shared edge is a diagonal: its two points differ in both coordinates
+    int sh[2], ns = 0;
+    for (int j = 3; j < 6 && ns < 2; ++j)
+        for (int k = 0; k < 3; ++k)
+            if (std::memcmp(P[j], P[k], sizeof(P[0])) == 0) { sh[ns++] = j; break; }
+    if (ns != 2 || P[sh[0]][u] == P[sh[1]][u] || P[sh[0]][w] == P[sh[1]][w]) return false;
+    std::memset(&q, 0, sizeof(q));
+    std::memcpy(q.n, a.n, sizeof(q.n));
+    q.np0 = a.np0;
+    q.lo0 = lo0; q.hi0 = hi0; q.lo1 = lo1; q.hi1 = hi1;
+    q.mat = a.mat;
+    return true;
+}
+
 int ValidateDesc(const pg3g_scene_desc *d) {
     if (!d) return Fail(PG3G_ERR_INVALID_ARG, "scene desc is null");
     if (d->num_triangles && (!d->triangles || !d->triangle_mat)) return Fail(PG3G_ERR_INVALID_ARG, "triangle arrays are null");
@@ -339,6 +402,27 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
         t.np0 = Dot(n, P[0]);
         t.mat = d->triangle_mat[i];
     }
+    std::vector<DevQuad> quads[3];
+    std::vector<DevTri> rest;
+    if (MergeRectangles()) {
+        for (uint32_t i = 0; i < d->num_triangles; ++i) {
+            DevQuad q;
+            int axis;
+            if (i + 1 < d->num_triangles && RectangleOf(d, tris, i, q, axis)) {
+                quads[axis].push_back(q);
+                ++i;
+            } else {
+                rest.push_back(tris[i]);
+            }
+        }
+    } else {
+        rest = tris;
+    }
+    std::vector<DevQuad> allQuads;
+    for (int a = 0; a < 3; ++a) {
+        allQuads.insert(allQuads.end(), quads[a].begin(), quads[a].end());
+        v.quadEnd[a] = (uint32_t)allQuads.size();
+    }
     std::vector<DevSphere> sphs(d->num_spheres);
     for (uint32_t i = 0; i < d->num_spheres; ++i) {
         std::memset(&sphs[i], 0, sizeof(DevSphere));
@@ -378,15 +462,17 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
             v.bgLight = (int32_t)i;
         }
     }
-    PG3_HIP(Upload(S.tris, tris.data(), tris.size(), st));
+    PG3_HIP(Upload(S.quads, allQuads.data(), allQuads.size(), st));
+    PG3_HIP(Upload(S.tris, rest.data(), rest.size(), st));
     PG3_HIP(Upload(S.sphs, sphs.data(), sphs.size(), st));
     PG3_HIP(Upload(S.mats, mats.data(), mats.size(), st));
     PG3_HIP(Upload(S.lights, lights.data(), lights.size(), st));
+    v.quads = S.quads.as<DevQuad>();
     v.tris = S.tris.as<DevTri>();
     v.sphs = S.sphs.as<DevSphere>();
     v.mats = S.mats.as<DevMaterial>();
     v.lights = S.lights.as<DevLight>();
-    v.numTris = d->num_triangles;
+    v.numTris = (uint32_t)rest.size();
     v.numSphs = d->num_spheres;
     v.numLights = d->num_lights;
     v.matClasses = 1u << kClsMiss;
@@ -527,7 +613,8 @@ BatchLaunch MakeBatchLaunch(pg3g_batch *b) {
     L.numItems = b->numItems;
     L.work = b->work.as<uint32_t>();
     L.counters = b->ctx->counters.as<unsigned long long>();
-    L.gridBlocks = (uint32_t)std::max(1, b->ctx->numCUs) * (b->cfg.waves_per_cu / 4u);
+    L.gridBlocks = b->gridBlocks;
+    L.blockLanes = b->blockLanes;
     L.emKind = b->emKind;
     L.oneLight = b->oneLight;
     return L;
@@ -551,7 +638,40 @@ int BuildBatchItems(pg3g_batch *b) {
         if (items > 0xFFFFFFF0ull) return Fail(PG3G_ERR_INVALID_ARG, "batch has too many work items");
     }
     b->numItems = (uint32_t)items;
+    // One persistent workgroup of up to 1024 lanes per compute unit (more workgroups when more
+    // wavefronts per CU are asked for).
+    const uint32_t waves = b->cfg.waves_per_cu;
+    b->blockLanes = 64u * std::min(waves, 16u);
+    b->gridBlocks = (uint32_t)std::max(1, ctx->numCUs) * std::max(1u, waves / 16u);
+    // kBatchParts contiguous parts of the item sequence with about equal sample counts (the items
+    // of one scene hold about equal sample counts)
+    const uint32_t nS = (uint32_t)b->info.size();
+    std::vector<double> w(nS);
+    double total = 0.0;
+    for (uint32_t i = 0; i < nS; ++i) {
+        const uint32_t n = (i + 1 < nS ? b->info[i + 1].itemBegin : b->numItems) - b->info[i].itemBegin;
+        w[i] = n ? 64.0 * b->info[i].numTiles * b->info[i].spp / n : 0.0;
+        total += w[i] * n;
+    }
+    uint32_t parts[kBatchParts + 1];
+    parts[0] = 0;
+    for (uint32_t p = 1; p <= kBatchParts; ++p) parts[p] = b->numItems;
+    {
+        uint32_t p = 1;
+        double acc = 0.0;
+        for (uint32_t i = 0; i < nS && p < kBatchParts; ++i) {
+            const uint32_t b0 = b->info[i].itemBegin, n = (i + 1 < nS ? b->info[i + 1].itemBegin : b->numItems) - b0;
+            while (p < kBatchParts && w[i] > 0.0 && acc + w[i] * n >= total * p / kBatchParts) {
+                const double need = total * p / kBatchParts - acc;
+                parts[p] = std::max(parts[p - 1], b0 + std::min(n, (uint32_t)std::ceil(need / w[i])));
+                ++p;
+            }
+            acc += w[i] * n;
+        }
+    }
     PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(hipMemcpyAsync(b->work.as<uint32_t>() + kBatchParts + 1, parts, sizeof(parts), hipMemcpyHostToDevice,
+                           ctx->stream));
     PG3_HIP(b->infoDev.Alloc(sizeof(BatchScene) * b->info.size()));
     PG3_HIP(hipMemcpyAsync(b->infoDev.p, b->info.data(), sizeof(BatchScene) * b->info.size(), hipMemcpyHostToDevice,
                            ctx->stream));
@@ -619,7 +739,7 @@ int BatchRestart(pg3g_batch *b) {
         PG3_HIP(hipMemsetAsync(S.fb.p, 0, S.fb.bytes, ctx->stream));
         S.iterations = 0;
     }
-    PG3_HIP(hipMemsetAsync(b->work.p, 0, sizeof(uint32_t) * 2, ctx->stream));
+    PG3_HIP(hipMemsetAsync(b->work.p, 0, sizeof(uint32_t) * (kBatchParts + 1), ctx->stream));   // counters, pass
     PG3_HIP(hipStreamSynchronize(ctx->stream));
     b->passes = 0;
     return PG3G_OK;
@@ -805,6 +925,17 @@ int pg3g_read_ray_counters(pg3g_context *ctx, uint64_t *extension_rays, uint64_t
     PG3_HIP(hipStreamSynchronize(ctx->stream));
     if (extension_rays) *extension_rays = h[2];
     if (shadow_rays) *shadow_rays = h[3];
+    return PG3G_OK;
+}
+
+int pg3g_scene_primitives(pg3g_context *ctx, uint32_t slot, uint32_t *rectangles, uint32_t *triangles,
+                          uint32_t *spheres) {
+    if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    SceneSlot *S = GetSlot(ctx, slot);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene uploaded in this slot");
+    if (rectangles) *rectangles = S->view.quadEnd[2];
+    if (triangles) *triangles = S->view.numTris;
+    if (spheres) *spheres = S->view.numSphs;
     return PG3G_OK;
 }
 
@@ -1142,8 +1273,8 @@ int pg3g_batch_create(pg3g_context *ctx, uint32_t num_scenes, const uint32_t *sl
     PG3_HIP(b->views.Alloc(sizeof(SceneView) * num_scenes));
     PG3_HIP(hipMemcpyAsync(b->views.p, views.data(), sizeof(SceneView) * num_scenes, hipMemcpyHostToDevice,
                            ctx->stream));
-    PG3_HIP(b->work.Alloc(sizeof(uint32_t) * 2));
-    PG3_HIP(hipMemsetAsync(b->work.p, 0, sizeof(uint32_t) * 2, ctx->stream));
+    PG3_HIP(b->work.Alloc(sizeof(uint32_t) * (2 * kBatchParts + 2)));
+    PG3_HIP(hipMemsetAsync(b->work.p, 0, sizeof(uint32_t) * (kBatchParts + 1), ctx->stream));
     if (int rc = BuildBatchItems(b.get())) return rc;
     *batch = b.release();
     return PG3G_OK;

@@ -34,13 +34,16 @@ struct BatchScene {
     uint32_t itersPerItem;      // iterations of one tile per work item
     uint32_t pad;
 };
+constexpr uint32_t kBatchParts = 8;   // MI355X: 8 XCDs, each with its own L2
 struct BatchLaunch {
     const SceneView *views;     // device, numScenes
     const BatchScene *info;     // device, numScenes
     uint32_t numScenes, numItems;
-    uint32_t *work;             // device: [0] next item, [1] pass index
+    uint32_t *work;             // device: [p] next item of part p, [kBatchParts] pass index,
+                                // [kBatchParts + 1 + p] first item of part p (kBatchParts + 1 entries)
     unsigned long long *counters;
-    uint32_t gridBlocks;        // persistent workgroups of 256 lanes
+    uint32_t gridBlocks;        // persistent workgroups
+    uint32_t blockLanes;        // lanes per workgroup (multiple of 64, at most 1024)
     int emKind;                 // EM sampler shared by every scene, or ignored when oneLight < 0
     int oneLight;               // 1 / 0 when shared by every scene, -1 = read per scene
 };

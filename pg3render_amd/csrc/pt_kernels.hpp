@@ -1121,19 +1121,28 @@ PG3_DEV void CameraSample(const SceneView &S, uint32_t pixel, uint32_t iteration
 // contiguous at the end of nodeW), cooperatively by the workgroup; the caller synchronises the
 // workgroup before the returned cache is read.
 constexpr uint32_t kTreeLdsNodes = 1280;
+// TreeLevels: which levels StageTree copies (minLevel == E.numLevels: none fits) and how many nodes.
 template <uint32_t kMaxNodes = kTreeLdsNodes>
-PG3_DEV TreeCache StageTree(const DevEm &E, F4v *lds) {
-    TreeCache tc;
-    uint32_t minLevel = E.numLevels, count = 0;
+PG3_DEV uint32_t TreeLevels(const DevEm &E, uint32_t &count) {
+    uint32_t minLevel = E.numLevels;
+    count = 0;
     for (int32_t l = (int32_t)E.numLevels - 1; l >= 1; --l) {
         if (count + E.levelCount[l] > kMaxNodes) break;
         count += E.levelCount[l];
         minLevel = (uint32_t)l;
     }
+    return minLevel;
+}
+// The copy is shared by the lanes first, first + stride, ... (the workgroup by default).
+template <uint32_t kMaxNodes = kTreeLdsNodes>
+PG3_DEV TreeCache StageTree(const DevEm &E, F4v *lds, uint32_t first = threadIdx.x, uint32_t stride = blockDim.x) {
+    TreeCache tc;
+    uint32_t count;
+    const uint32_t minLevel = TreeLevels<kMaxNodes>(E, count);
     if (minLevel < E.numLevels) {
         const uint32_t base = E.levelBase[minLevel];
         const GlobalPtr<F4v> src = (GlobalPtr<F4v>)AsGlobal(E.nodeW + (size_t)base * 12u);
-        for (uint32_t i = threadIdx.x; i < count * 3u; i += blockDim.x) lds[i] = src[i];
+        for (uint32_t i = first; i < count * 3u; i += stride) lds[i] = src[i];
         tc.nodes = reinterpret_cast<const float *>(lds);
         tc.minLevel = minLevel;
         tc.baseNode = base;
@@ -1145,10 +1154,11 @@ PG3_DEV TreeCache StageTree(const DevEm &E, F4v *lds) {
 // `lds` when it fits in kMargLdsMax entries (an EM of up to 4095 rows), cooperatively by the
 // workgroup; the caller synchronises the workgroup before the staged table is read.
 constexpr uint32_t kMargLdsMax = 4096;
-PG3_DEV void StageMarginal(const DevEm &E, float *lds, TreeCache &tc) {
+PG3_DEV void StageMarginal(const DevEm &E, float *lds, TreeCache &tc, uint32_t first = threadIdx.x,
+                           uint32_t stride = blockDim.x) {
     if (E.h + 1u > kMargLdsMax) return;
     const GlobalPtr<float> src = AsGlobal(E.margCdf);
-    for (uint32_t i = threadIdx.x; i <= E.h; i += blockDim.x) lds[i] = src[i];
+    for (uint32_t i = first; i <= E.h; i += stride) lds[i] = src[i];
     tc.marg = lds;
 }
 
@@ -1170,6 +1180,8 @@ struct TileItem {
 // task changes nothing: a sample's path depends only on its (pixel, iteration) RNG stream.
 //   src.Next(TileItem&) -> bool   fetches the next item (false: none left; wave-uniform)
 //   src.View(scene) -> const SceneView *, src.Spec(scene) -> LightSpec
+//   kMultiScene: src.Enter(scene) when the wavefront starts items of a scene (all lanes idle),
+//   src.Entered(scene) -> whether it did for the current one
 //   Path: a lane's task.  Path::Subs(S) tasks per camera sample (sub-samples whose radiances add
 //   up to the sample's); Start(S, pixel, iteration, sub); Step(S, spec, tally, shadowed, step, ring)
 //   -> true when the task is done, with `step` the wavefront's step number (the same on every lane)
@@ -1203,7 +1215,12 @@ PG3_DEV void TraceTasks(Src &src, RayTally &tally) {
                 more = false;
             }
         }
-        if (kMultiScene && T.scene != scene && idle == ~0ull) scene = T.scene;   // drained: switch
+        if constexpr (kMultiScene) {   // drained: switch
+            if (idle == ~0ull && (T.scene != scene || !src.Entered(T.scene))) {
+                scene = T.scene;
+                src.Enter(scene);
+            }
+        }
         // the scene is looked up by index at each use: a pointer variable carried around the
         // loop would lose its address space (flat loads in place of scalar and LDS loads)
         const SceneView &S = *src.View(scene);

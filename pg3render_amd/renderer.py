@@ -220,6 +220,13 @@ class Context:
         check(self._lib.pg3g_read_ray_counters(self.ptr, C.byref(e), C.byref(sh)), "pg3g_read_ray_counters")
         return int(e.value), int(sh.value)
 
+    def primitives(self, slot=0):
+        """(rectangles, triangles, spheres) of a slot as the intersection kernels test them."""
+        r, t, s = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(self._lib.pg3g_scene_primitives(self.ptr, slot, C.byref(r), C.byref(t), C.byref(s)),
+              "pg3g_scene_primitives")
+        return int(r.value), int(t.value), int(s.value)
+
     def reset_counters(self):
         check(self._lib.pg3g_reset_counters(self.ptr), "pg3g_reset_counters")
 

@@ -685,6 +685,43 @@ def test_occluded_parity(ctx, build):
     assert 0.05 < oo.mean() < 0.95
 
 
+@pytest.mark.parametrize("build", ISECT_SCENES)
+def test_rectangles_equal_their_triangles(ctx, build, monkeypatch, request):
+    """The walls merged into axis-aligned rectangles at upload (pairs of triangles tested as one
+    primitive) hit exactly as their two triangles: on 100k rays the same hit/miss and material and
+    bit-identical distances and normals, except on rays within 1e-5 of an edge, a tangent or a
+    second surface (independent FP64 caster); Scene::Occluded likewise."""
+    sb = build()
+    rng, org, dir, tmin = _isect_rays(63)
+    tmax = rng.uniform(0.05, 1.5, len(org)).astype(np.float32)
+    ctx.upload(sb)
+    rect, tris, sphs = ctx.primitives()
+    dq, nq, mq = ctx.intersect(org, dir, tmin)
+    oq = ctx.occluded(org, dir, tmax)
+    monkeypatch.setenv("PG3G_TRIANGLES_ONLY", "1")
+    ctx.upload(sb)
+    monkeypatch.delenv("PG3G_TRIANGLES_ONLY")
+    assert ctx.primitives() == (0, tris + 2 * rect, sphs)
+    dt, nt, mt = ctx.intersect(org, dir, tmin)
+    ot = ctx.occluded(org, dir, tmax)
+    ctx.upload(sb)
+    print(f"  {rect} rectangles, {tris} triangles, {sphs} spheres")
+    if "workload" in request.node.name:
+        assert rect == 5   # the unit box's five walls; the reference boxes' halves differ in the last ulp of n
+    _, _, _, margin = cast64(sb, org, dir, tmin)
+    clear = margin > 1e-5
+    assert clear.mean() > 0.99
+    assert np.array_equal(mq[clear], mt[clear])
+    assert np.array_equal(dq[clear].view(np.uint32), dt[clear].view(np.uint32))
+    assert np.array_equal(nq[clear].view(np.uint32), nt[clear].view(np.uint32))
+    assert np.mean(mq != mt) < 1e-3
+    p64 = org.astype(np.float64) + 1e-3 * dir.astype(np.float64)
+    _, _, _, margin_o = cast64(sb, p64, dir, 0.0, tmax.astype(np.float64) - 2e-3)
+    clear_o = margin_o > 1e-5
+    assert np.array_equal(oq[clear_o], ot[clear_o])
+    assert np.mean(oq != ot) < 1e-3
+
+
 ALL_SCENE_IDS = [i for i in range(38) if i != 27]   # 27: the SRA layered material, outside the GPU path
 
 
