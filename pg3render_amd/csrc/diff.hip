@@ -7,8 +7,14 @@
 namespace pg3 {
 
 // Differentiable forward: same work decomposition as the plain path-tracing kernel.
+// The forward runs at 3 waves per SIMD (168 VGPRs, 40 B of spill): at 4 (128 VGPRs, 208 B of
+// spill) the 256^2 x 32 spp forward of the box with two GGX conductors took 1.71-1.75 ms against
+// 1.44-1.51 ms.
+#ifndef PG3_DIFF_WAVES_PER_EU
+#define PG3_DIFF_WAVES_PER_EU 3
+#endif
 template <int kEmKind, bool kOneLight>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PG3_DIFF_WAVES_PER_EU)))
 DiffKernel(const SceneView *__restrict__ Sp, uint32_t firstIter, uint32_t nIter, uint32_t itersPerThread,
            uint32_t numChunks, double *__restrict__ fb, float *__restrict__ sampleL,
            unsigned long long *__restrict__ counters) {
