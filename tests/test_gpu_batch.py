@@ -117,6 +117,26 @@ def test_full_64_scene_batch_equals_per_scene_and_oracle(ctx):
         oracle_band_parity(scenes[i], img)
 
 
+@pytest.mark.parametrize("waves", [4, 8, 32])
+def test_batch_launch_shapes_equal_per_scene_renders(ctx, waves):
+    """Every launch shape the autotuner can pick: workgroups of 256 and 512 lanes (4 and 8
+    wavefronts per CU, one workgroup per CU) and two 1024-lane workgroups per CU (32, the second
+    starting when the first has drained the work), with the per-XCD item parts, the stealing of
+    items from other parts and the LDS cache of the scenes' steerable tables: the same images as
+    per-scene renders."""
+    scenes = c3_scenes(5)
+    for i, sb in enumerate(scenes):
+        ctx.upload(sb, i)
+    batch = Batch(ctx, range(len(scenes)), [sb.spp for sb in scenes])
+    batch.set_config(iters_per_item=2, waves_per_cu=waves, use_graph=True)
+    batch.render(2)
+    got = [ctx.framebuffer(i, sb.resolution) for i, sb in enumerate(scenes)]
+    batch.close()
+    want = plain_images(ctx, scenes, 100, passes=2)
+    for i in range(len(scenes)):
+        assert_same(got[i], want[i], f"scene {i}")
+
+
 def test_batch_graph_and_direct_launch_agree(ctx):
     scenes = c3_scenes(3)
     for i, sb in enumerate(scenes):
