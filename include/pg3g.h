@@ -349,6 +349,24 @@ int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
 int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
                  const float *dloss_dimage, double *grad_scalars, double *grad_em, float *elapsed_ms);
 
+/* Multi-view differentiable rendering (no counterpart in the reference; the inverse-rendering
+ * workload renders one scene from several cameras at every optimisation step): the slot's scene,
+ * parameters and sampling tables seen from num_views cameras (each with the slot's resolution;
+ * num_views = 0 removes them).  View v's samples use the RNG streams of scene id scene_ids[v],
+ * so they equal the samples of the same scene uploaded with that camera and scene id.  One launch
+ * renders every view; pg3g_render_diff_views returns the FP64 sums of iterations [first_iter,
+ * first_iter + n_iter) of each view (num_views x res_y x res_x x 3, view-major; the image is sum /
+ * n_iter), and pg3g_adjoint_views takes dLoss/dImage of every view (the same layout, float) and
+ * returns the gradients summed over the views, with pg3g_adjoint's conventions (it reuses the
+ * per-sample radiance of a pg3g_render_diff_views of the same iterations). */
+#define PG3G_MAX_DIFF_VIEWS 64
+int pg3g_diff_set_views(pg3g_context *ctx, uint32_t slot, uint32_t num_views, const pg3g_camera_desc *cameras,
+                        const uint32_t *scene_ids);
+int pg3g_render_diff_views(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, double *sums,
+                           float *elapsed_ms);
+int pg3g_adjoint_views(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
+                       const float *dloss_dimages, double *grad_scalars, double *grad_em, float *elapsed_ms);
+
 /* Allocation-free variant of pg3g_adjoint.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory -- code under HIP graph capture, tight optimisation loops,
  * harnesses that forbid hipMalloc.  It calls no hipMalloc / hipFree and does not synchronise:

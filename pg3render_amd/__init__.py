@@ -5,9 +5,9 @@ only binds it.  There is no CPU fallback: if the library is not built, importing
 raises.
 """
 from . import _ffi
-from .diff import DiffScene, render_function
+from .diff import DiffScene, DiffViews, render_function
 from .renderer import (Batch, Context, GpuPathTracer, SceneBuild, compare_images, load_image, save_image,
                        steerable_build, steerable_cache_dir)
 
-__all__ = ["Batch", "Context", "DiffScene", "GpuPathTracer", "SceneBuild", "_ffi", "compare_images", "load_image",
+__all__ = ["Batch", "Context", "DiffScene", "DiffViews", "GpuPathTracer", "SceneBuild", "_ffi", "compare_images", "load_image",
            "render_function", "save_image", "steerable_build", "steerable_cache_dir"]

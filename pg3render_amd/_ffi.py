@@ -142,6 +142,9 @@ _SIGNATURES = {
     "pg3g_diff_set_params": (C.c_int, [_P, C.c_uint32, _FP, _FP, C.c_int]),
     "pg3g_render_diff": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _FP]),
     "pg3g_adjoint": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _FP, _DP, _DP, _FP]),
+    "pg3g_diff_set_views": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(CameraDesc), _U32P]),
+    "pg3g_render_diff_views": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _DP, _FP]),
+    "pg3g_adjoint_views": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _FP, _DP, _DP, _FP]),
     "pg3g_build_workload_scene": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "pg3g_build_reference_scene": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "pg3g_build_reference_scene_em": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_char_p, C.POINTER(_P)]),

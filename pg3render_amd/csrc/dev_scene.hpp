@@ -310,8 +310,14 @@ PG3_DEV bool Occluded(const SceneView &S, F3 p, F3 dir, float tmax) {           
 }
 
 // Camera::GenerateRay, camera.hxx:91-100 (TransformPoint, math.hxx:734-752)
-PG3_DEV void GenerateRay(const SceneView &S, float x, float y, F3 &org, F3 &dir) {
-    const float *m = S.r2w;
+// One more camera of a scene (multi-view differentiable rendering): raster -> world matrix,
+// position, and the scene id keying the RNG streams of its samples.
+struct DevView {
+    float r2w[16];
+    F3 camPos;
+    uint32_t sceneId;
+};
+PG3_DEV void GenerateRay(const float *m, F3 camPos, float x, float y, F3 &org, F3 &dir) {
     float w = m[15];
     w += m[12] * x;
     w += m[13] * y;
@@ -321,8 +327,11 @@ PG3_DEV void GenerateRay(const SceneView &S, float x, float y, F3 &org, F3 &dir)
     p.x = (((m[3] + x * m[0]) + y * m[1]) + 0.f * m[2]) * invW;
     p.y = (((m[7] + x * m[4]) + y * m[5]) + 0.f * m[6]) * invW;
     p.z = (((m[11] + x * m[8]) + y * m[9]) + 0.f * m[10]) * invW;
-    org = S.camPos;
-    dir = Normalize(p - S.camPos);
+    org = camPos;
+    dir = Normalize(p - camPos);
+}
+PG3_DEV void GenerateRay(const SceneView &S, float x, float y, F3 &org, F3 &dir) {
+    GenerateRay(S.r2w, S.camPos, x, y, org, dir);
 }
 
 // ---------------------------------------------------------------------------------------------

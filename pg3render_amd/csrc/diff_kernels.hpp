@@ -234,14 +234,17 @@ PG3_DEV bool DiffVertex(const SceneView &S, const LightSpec &spec, const RngKey 
 // A work item (8x8 tile x iterations) of the differentiable forward (kAdjoint false: per-pixel
 // sums into fb) or of the adjoint (kAdjoint true: each sample is traced twice, the primal replay
 // giving its radiance and the second replay the gradients).  Samples are handed out in order, one
-// to each lane whose path has ended.  sampleL (may be null) holds one radiance per sample of the launch, index
-// (iteration - firstIter) * pixels + pixel: the forward writes it, and the adjoint then reads the
-// radiance instead of running the primal replay.
+// to each lane whose path has ended.  sampleL (may be null) holds one radiance per sample of the
+// launch, index (iteration - firstIter) * sampleStride + pixel: the forward writes it, and the
+// adjoint then reads the radiance instead of running the primal replay.  view (may be null): the
+// camera of the tile's view in a multi-view scene; fb, adjImg and sampleL then point at the view's
+// part of buffers that hold every view (sampleStride = views x pixels).
 template <bool kAdjoint>
 PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x0, uint32_t y0, uint32_t itBegin,
                            uint32_t itEnd, double *acc, double *fb, const float *adjImg, float invN, double *gradEm,
                            double *gradScal, float *sampleL, uint32_t firstIter, int32_t *cacheKeys,
-                           double *cacheVals, uint32_t &segments, uint32_t &samples) {
+                           double *cacheVals, uint32_t &segments, uint32_t &samples, const DevView *view,
+                           size_t sampleStride) {
     const uint32_t lane = __lane_id();
     if (!kAdjoint) {
         acc[lane * 3 + 0] = 0.0;
@@ -275,7 +278,7 @@ PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x
                     pixel = y * S.resX + x;
                     iter = itBegin + (q >> 6);
                     F3 org, dir;
-                    CameraSample(S, pixel, iter, key, org, dir);
+                    CameraSample(S, view, pixel, iter, key, org, dir);
                     InitPath(ps, org, dir);
                     busy = true;
                     second = false;
@@ -283,7 +286,7 @@ PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x
                         const GlobalPtr<float> a = AsGlobal(adjImg + (size_t)pixel * 3);
                         g.adj = mk(a[0], a[1], a[2]) * invN;
                         if (sampleL) {
-                            const size_t si = ((size_t)(iter - firstIter) * S.resX * S.resY + pixel) * 3;
+                            const size_t si = ((size_t)(iter - firstIter) * sampleStride + pixel) * 3;
                             const GlobalPtr<float> l = AsGlobal(sampleL + si);
                             g.total = mk(l[0], l[1], l[2]);
                             second = true;
@@ -307,7 +310,7 @@ PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x
             if (!cont) {
                 if (!kAdjoint) {
                     if (sampleL) {
-                        float *l = sampleL + ((size_t)(iter - firstIter) * S.resX * S.resY + pixel) * 3;
+                        float *l = sampleL + ((size_t)(iter - firstIter) * sampleStride + pixel) * 3;
                         l[0] = ps.L.x;
                         l[1] = ps.L.y;
                         l[2] = ps.L.z;
@@ -320,7 +323,7 @@ PG3_DEV void TraceTileDiff(const SceneView &S, const LightSpec &spec, uint32_t x
                 } else if (!second) {
                     g.total = ps.L;              // primal replay done: replay the same path again
                     F3 org, dir;
-                    CameraSample(S, pixel, iter, key, org, dir);
+                    CameraSample(S, view, pixel, iter, key, org, dir);
                     InitPath(ps, org, dir);
                     second = true;
                 } else {

@@ -73,6 +73,9 @@ struct SceneSlot {
     pg3g_em_desc emDesc{};
     DevBuf evalMats, evalTexels, gradBase, gradScal, gradEm, adjImg, adjWork;
     DevBuf sampleL;                           // per-sample radiance of the last differentiable forward
+    uint32_t sampleViews = 1;                 // views of that forward
+    uint32_t numViews = 0;                    // cameras of pg3g_diff_set_views (0: none)
+    DevBuf viewCams, fbViews;                 // DevView per view; numViews FP64 images
     bool sampleValid = false;
     uint32_t sampleFirst = 0, sampleCount = 0;
     uint64_t sampleEpoch = 0;
@@ -435,6 +438,7 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
     S.evalHostMats = S.hostMats;
     S.diffMats.clear();
     S.numScalars = 0;
+    S.numViews = 0;
     S.gradBaseHost.assign(d->num_materials, -1);
     std::vector<DevLight> lights(d->num_lights);
     v.bgLight = -1;
@@ -1449,9 +1453,16 @@ int pg3g_diff_set_params(pg3g_context *ctx, uint32_t slot, const float *scalars,
     return SyncView(ctx, *S);
 }
 
-int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, float *elapsed_ms) {
-    SceneSlot *S = nullptr;
-    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+}  // extern "C"
+
+namespace {
+
+// Differentiable forward of iterations [first, first + n) of one view (views == nullptr: the
+// slot's camera, into its framebuffer) or of every view of pg3g_diff_set_views (into fbViews,
+// cleared first).  Keeps the per-sample radiance for an adjoint of the same samples (up to 2^26
+// samples).
+int RenderDiffImpl(pg3g_context *ctx, SceneSlot *S, uint32_t first_iter, uint32_t n_iter, const DevView *views,
+                   uint32_t nv, double *fb, float *elapsed_ms) {
     if ((uint64_t)first_iter + n_iter > 0xFFFFFFFFull) return Fail(PG3G_ERR_INVALID_ARG, "iteration range overflows");
     PG3_HIP(hipSetDevice(ctx->device));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1460,17 +1471,16 @@ int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
         PG3_HIP(hipEventCreate(&e1));
         PG3_HIP(hipEventRecord(e0, ctx->stream));
     }
-    // keep the per-sample radiance for the adjoint of the same samples (up to 2^26 samples)
-    const uint64_t ns = (uint64_t)n_iter * S->view.resX * S->view.resY;
+    const uint64_t ns = (uint64_t)n_iter * nv * S->view.resX * S->view.resY;
     float *sl = nullptr;
     S->sampleValid = false;
     if (ns <= (1ull << 26)) {
         PG3_HIP(S->sampleL.Alloc(sizeof(float) * 3 * ns));
         sl = S->sampleL.as<float>();
     }
-    hipError_t e = LaunchDiff(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(), nullptr,
-                              nullptr, nullptr, 0, sl, nullptr, ctx->numCUs, ctx->counters.as<unsigned long long>(),
-                              ctx->stream);
+    hipError_t e = LaunchDiff(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, fb, nullptr, nullptr, nullptr,
+                              0, sl, nullptr, ctx->numCUs, ctx->counters.as<unsigned long long>(), ctx->stream, views,
+                              nv);
     if (elapsed_ms) {
         if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -1481,31 +1491,31 @@ int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint
         *elapsed_ms = ms;
     }
     PG3_HIP(e);
-    S->iterations += n_iter;
     if (sl) {
         S->sampleValid = true;
         S->sampleFirst = first_iter;
         S->sampleCount = n_iter;
         S->sampleEpoch = S->epoch;
+        S->sampleViews = nv;
     }
     return PG3G_OK;
 }
 
-int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, const float *dloss_dimage,
-                 double *grad_scalars, double *grad_em, float *elapsed_ms) {
-    SceneSlot *S = nullptr;
-    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
-    if (!dloss_dimage) return Fail(PG3G_ERR_INVALID_ARG, "dloss_dimage is null");
+// Adjoint of iterations [first, first + n) of one view or of every view (gradients summed over
+// the views): dloss holds dLoss/dImage of each view (nv images of res_x x res_y x 3).
+int AdjointImpl(pg3g_context *ctx, SceneSlot *S, uint32_t first_iter, uint32_t n_iter, const float *dloss,
+                const DevView *views, uint32_t nv, double *grad_scalars, double *grad_em, float *elapsed_ms) {
+    if (!dloss) return Fail(PG3G_ERR_INVALID_ARG, "dloss_dimage is null");
     if (n_iter == 0) return Fail(PG3G_ERR_INVALID_ARG, "n_iter must be >= 1");
     if ((uint64_t)first_iter + n_iter > 0xFFFFFFFFull) return Fail(PG3G_ERR_INVALID_ARG, "iteration range overflows");
     const SceneView &v = S->view;
     PG3_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const size_t npx = (size_t)v.resX * v.resY;
-    PG3_HIP(Upload(S->adjImg, dloss_dimage, npx * 3, st));
+    PG3_HIP(Upload(S->adjImg, dloss, npx * 3 * nv, st));
     PG3_HIP(S->gradScal.Alloc(sizeof(double) * kMaxDiffScalars));
-    PG3_HIP(S->adjWork.Alloc(sizeof(uint32_t)));
-    PG3_HIP(hipMemsetAsync(S->adjWork.p, 0, sizeof(uint32_t), st));
+    PG3_HIP(S->adjWork.Alloc(sizeof(uint32_t) * nv));                 // one item counter per view
+    PG3_HIP(hipMemsetAsync(S->adjWork.p, 0, sizeof(uint32_t) * nv, st));
     PG3_HIP(hipMemsetAsync(S->gradScal.p, 0, sizeof(double) * kMaxDiffScalars, st));
     const size_t emN = (size_t)v.em.w * v.em.h * 3;
     double *gEm = nullptr;
@@ -1514,12 +1524,12 @@ int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t
         PG3_HIP(hipMemsetAsync(S->gradEm.p, 0, sizeof(double) * emN, st));
         gEm = S->gradEm.as<double>();
     }
-    // radiance of the same samples kept by the last differentiable forward (same parameters);
-    // without it a forward pass computes it first (cheaper than interleaving a primal replay with
-    // the gradient replay in every wavefront)
+    // radiance of the same samples kept by the last differentiable forward (same parameters and
+    // views); without it a forward pass computes it first (cheaper than interleaving a primal
+    // replay with the gradient replay in every wavefront)
     bool stored = S->sampleValid && S->sampleFirst == first_iter && S->sampleCount == n_iter &&
-                  S->sampleEpoch == S->epoch;
-    const uint64_t ns = (uint64_t)n_iter * v.resX * v.resY;
+                  S->sampleEpoch == S->epoch && S->sampleViews == nv;
+    const uint64_t ns = (uint64_t)n_iter * nv * npx;
     bool primal = false;
     if (!stored && ns <= (1ull << 26)) {
         PG3_HIP(S->sampleL.Alloc(sizeof(float) * 3 * ns));
@@ -1532,11 +1542,12 @@ int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t
     hipError_t e = hipEventRecord(e0, st);
     if (e == hipSuccess && primal)
         e = LaunchDiff(v, S->viewDev.as<SceneView>(), first_iter, n_iter, nullptr, nullptr, nullptr, nullptr, 0,
-                       S->sampleL.as<float>(), nullptr, ctx->numCUs, ctx->counters.as<unsigned long long>(), st);
+                       S->sampleL.as<float>(), nullptr, ctx->numCUs, ctx->counters.as<unsigned long long>(), st, views,
+                       nv);
     if (e == hipSuccess)
         e = LaunchDiff(v, S->viewDev.as<SceneView>(), first_iter, n_iter, nullptr, S->adjImg.as<float>(), gEm,
                        S->gradScal.as<double>(), S->numScalars, stored ? S->sampleL.as<float>() : nullptr,
-                       S->adjWork.as<uint32_t>(), ctx->numCUs, ctx->counters.as<unsigned long long>(), st);
+                       S->adjWork.as<uint32_t>(), ctx->numCUs, ctx->counters.as<unsigned long long>(), st, views, nv);
     if (e == hipSuccess) e = hipEventRecord(e1, st);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.f;
@@ -1549,6 +1560,7 @@ int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t
         S->sampleFirst = first_iter;
         S->sampleCount = n_iter;
         S->sampleEpoch = S->epoch;
+        S->sampleViews = nv;
     }
     if (elapsed_ms) *elapsed_ms = ms;
     if (grad_scalars && S->numScalars > 0) {
@@ -1565,6 +1577,84 @@ int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t
         PG3_HIP(hipMemcpy(grad_em, S->gradEm.p, sizeof(double) * emN, hipMemcpyDeviceToHost));
     }
     return PG3G_OK;
+}
+
+int CheckViews(SceneSlot *S) {
+    if (S->numViews == 0) return Fail(PG3G_ERR_INVALID_ARG, "the slot has no views (pg3g_diff_set_views)");
+    return PG3G_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, float *elapsed_ms) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (int rc = RenderDiffImpl(ctx, S, first_iter, n_iter, nullptr, 1, S->fb.as<double>(), elapsed_ms)) return rc;
+    S->iterations += n_iter;
+    return PG3G_OK;
+}
+
+int pg3g_adjoint(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, const float *dloss_dimage,
+                 double *grad_scalars, double *grad_em, float *elapsed_ms) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    return AdjointImpl(ctx, S, first_iter, n_iter, dloss_dimage, nullptr, 1, grad_scalars, grad_em, elapsed_ms);
+}
+
+int pg3g_diff_set_views(pg3g_context *ctx, uint32_t slot, uint32_t num_views, const pg3g_camera_desc *cameras,
+                        const uint32_t *scene_ids) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (num_views > PG3G_MAX_DIFF_VIEWS) return Fail(PG3G_ERR_INVALID_ARG, "too many views");
+    if (num_views > 0 && (!cameras || !scene_ids)) return Fail(PG3G_ERR_INVALID_ARG, "cameras or scene_ids is null");
+    std::vector<DevView> vs(num_views);
+    for (uint32_t i = 0; i < num_views; ++i) {
+        const pg3g_camera_desc &c = cameras[i];
+        if (c.res_x != S->view.resX || c.res_y != S->view.resY)
+            return Fail(PG3G_ERR_INVALID_ARG, "every view must have the slot's resolution");
+        std::memset(&vs[i], 0, sizeof(DevView));
+        CameraMatrix(c, vs[i].r2w);
+        vs[i].camPos = mk(c.position[0], c.position[1], c.position[2]);
+        vs[i].sceneId = scene_ids[i] & 0xFFFFu;
+    }
+    PG3_HIP(hipSetDevice(ctx->device));
+    if (num_views > 0) {
+        PG3_HIP(Upload(S->viewCams, vs.data(), vs.size(), ctx->stream));
+        PG3_HIP(S->fbViews.Alloc(sizeof(double) * 3 * (size_t)S->view.resX * S->view.resY * num_views));
+        PG3_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    S->numViews = num_views;
+    S->sampleValid = false;
+    return PG3G_OK;
+}
+
+int pg3g_render_diff_views(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, double *sums,
+                           float *elapsed_ms) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (int rc = CheckViews(S)) return rc;
+    PG3_HIP(hipSetDevice(ctx->device));
+    PG3_HIP(hipMemsetAsync(S->fbViews.p, 0, S->fbViews.bytes, ctx->stream));
+    if (int rc = RenderDiffImpl(ctx, S, first_iter, n_iter, S->viewCams.as<DevView>(), S->numViews,
+                                S->fbViews.as<double>(), elapsed_ms))
+        return rc;
+    if (sums) {
+        PG3_HIP(hipMemcpyAsync(sums, S->fbViews.p, sizeof(double) * 3 * (size_t)S->view.resX * S->view.resY * S->numViews,
+                               hipMemcpyDeviceToHost, ctx->stream));
+        PG3_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return PG3G_OK;
+}
+
+int pg3g_adjoint_views(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
+                       const float *dloss_dimages, double *grad_scalars, double *grad_em, float *elapsed_ms) {
+    SceneSlot *S = nullptr;
+    if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (int rc = CheckViews(S)) return rc;
+    return AdjointImpl(ctx, S, first_iter, n_iter, dloss_dimages, S->viewCams.as<DevView>(), S->numViews, grad_scalars,
+                       grad_em, elapsed_ms);
 }
 
 int pg3g_adjoint_noalloc(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter,
@@ -1589,7 +1679,7 @@ int pg3g_adjoint_noalloc(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, 
             if (a < 0.001f || a > 1.0f) zeroMask |= 1ull << (S->gradBaseHost[m] + 3);
         }
     const bool stored = S->sampleValid && S->sampleFirst == first_iter && S->sampleCount == n_iter &&
-                        S->sampleEpoch == S->epoch;
+                        S->sampleEpoch == S->epoch && S->sampleViews == 1;
     PG3_HIP(hipSetDevice(ctx->device));
     PG3_HIP(LaunchAdjointNoAlloc(ctx->device, v, S->viewDev.as<SceneView>(), first_iter, n_iter, dloss_dimage_dev,
                                  S->numScalars, zeroMask, stored ? S->sampleL.as<float>() : nullptr, grad_scalars_dev,

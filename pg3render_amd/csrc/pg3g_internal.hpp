@@ -55,11 +55,15 @@ hipError_t LaunchBatchPass(const BatchLaunch &L, hipStream_t stream);
 // scene has no EM) and gradScal (numScal <= kMaxDiffScalars).  sampleL (may be null): per-sample
 // radiance, nIter * res_x * res_y * 3 floats, written by the forward and read by the adjoint in
 // place of its primal replay.  The adjoint is a persistent kernel (numCUs workgroups) taking work
-// items from *work, which must be zero at launch.
+// items from work[0 .. numViews), which must be zero at launch.  views (device, numViews cameras; null with
+// numViews = 1: the scene's own camera): every view renders the same iterations; fb, adjImg and
+// sampleL then hold numViews images one after another (sampleL: iteration-major, numViews x
+// res_x x res_y samples per iteration), and the gradients are the sum over the views.
 constexpr uint32_t kMaxDiffScalars = 64;
 hipError_t LaunchDiff(const SceneView &S, const SceneView *Sdev, uint32_t firstIter, uint32_t nIter, double *fb,
                       const float *adjImg, double *gradEm, double *gradScal, uint32_t numScal, float *sampleL,
-                      uint32_t *work, int numCUs, unsigned long long *counters, hipStream_t stream);
+                      uint32_t *work, int numCUs, unsigned long long *counters, hipStream_t stream,
+                      const DevView *views = nullptr, uint32_t numViews = 1);
 
 // Allocation-free adjoint (noalloc.hip).  kNoAllocMaxEmChannels: FP64 EM-gradient channels the
 // fixed scratch array holds (an RGB EM of up to 256 x 128 texels).  InitNoAllocScratch looks up

@@ -1116,6 +1116,19 @@ PG3_DEV void CameraSample(const SceneView &S, uint32_t pixel, uint32_t iteration
     GenerateRay(S, float(x) + jx, float(y) + jy, org, dir);
 }
 
+// The same for camera `view` of a multi-view scene (null: the scene's own camera).
+PG3_DEV void CameraSample(const SceneView &S, const DevView *view, uint32_t pixel, uint32_t iteration, RngKey &key,
+                          F3 &org, F3 &dir) {
+    if (!view) {
+        CameraSample(S, pixel, iteration, key, org, dir);
+        return;
+    }
+    key = RngKey{S.k0, S.k1, pixel, iteration, view->sceneId << 16};
+    const float jx = RngGet(key, 0, 0, 0), jy = RngGet(key, 0, 0, 1);
+    const uint32_t x = pixel % S.resX, y = pixel / S.resX;
+    GenerateRay(view->r2w, view->camPos, float(x) + jx, float(y) + jy, org, dir);
+}
+
 // Stages the inner levels of a steerable EM's triangle tree in `lds` (12 floats per node as F4v
 // triples, at most kMaxNodes nodes: the levels from the root down that fit, which are
 // contiguous at the end of nodeW), cooperatively by the workgroup; the caller synchronises the

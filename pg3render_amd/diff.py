@@ -82,8 +82,50 @@ class DiffScene:
         return out + (float(ms.value),) if timed else out
 
 
+class DiffViews:
+    """Several cameras of one DiffScene rendered and differentiated together
+    (pg3g_diff_set_views): one launch for the forward of every view and one for the adjoint, with
+    the gradients summed over the views.  `scenes`: SceneBuild objects of the same scene seen from
+    other cameras; each view takes its camera and its RNG scene id (so it renders the samples that
+    scene would render in a slot of its own)."""
+
+    def __init__(self, diff_scene, scenes):
+        self._lib = diff_scene._lib
+        self.ds = diff_scene
+        n = len(scenes)
+        cams = (_ffi.CameraDesc * n)(*[sb.desc.camera for sb in scenes])
+        ids = (C.c_uint32 * n)(*[int(sb.desc.params.scene_id) for sb in scenes])
+        check(self._lib.pg3g_diff_set_views(diff_scene.ctx.ptr, diff_scene.slot, n, cams, ids), "pg3g_diff_set_views")
+        self.num_views = n
+        w, h = diff_scene.resolution
+        self.shape = (n, h, w, 3)
+
+    def render(self, first_iter, n_iter, timed=False):
+        """Mean images (views, H, W, 3) FP64 of iterations [first_iter, first_iter + n_iter)."""
+        out = np.zeros(self.shape, np.float64)
+        ms = C.c_float()
+        check(self._lib.pg3g_render_diff_views(self.ds.ctx.ptr, self.ds.slot, first_iter, n_iter, _dp(out),
+                                               C.byref(ms) if timed else None), "pg3g_render_diff_views")
+        out /= max(n_iter, 1)
+        return (out, float(ms.value)) if timed else out
+
+    def adjoint(self, first_iter, n_iter, dloss_dimages, timed=False):
+        """Gradients (summed over the views) of a loss whose derivative with respect to the mean
+        images of the same iterations is `dloss_dimages` (views, H, W, 3)."""
+        adj = np.ascontiguousarray(dloss_dimages, np.float32).reshape(self.shape)
+        ds = self.ds
+        gs = np.zeros(max(ds.num_scalars, 1), np.float64)
+        ge = np.zeros(ds.em_shape, np.float64) if ds.em_shape else None
+        ms = C.c_float()
+        check(self._lib.pg3g_adjoint_views(ds.ctx.ptr, ds.slot, first_iter, n_iter, _fp(adj), _dp(gs),
+                                           _dp(ge) if ge is not None else None, C.byref(ms)), "pg3g_adjoint_views")
+        out = (gs[:ds.num_scalars], ge)
+        return out + (float(ms.value),) if timed else out
+
+
 def render_function(diff_scene, n_iter, decorrelate=False):
-    """A torch.autograd.Function for `diff_scene`: apply(scalars, em) -> image (H, W, 3) float64.
+    """A torch.autograd.Function for `diff_scene`: apply(scalars, em) -> image (H, W, 3) float64,
+    or, for a DiffViews, the images of all its views (views, H, W, 3) from one launch.
 
     Each call renders the next `n_iter` iterations (fresh samples every call); the parameters of
     the call become the sampling parameters.  Backward replays exactly those samples, or, with
@@ -96,13 +138,16 @@ def render_function(diff_scene, n_iter, decorrelate=False):
     state = {"next": 0}
     span = 2 * n_iter if decorrelate else n_iter
 
+    # a DiffViews renders and differentiates all its views at once; a DiffScene its own camera
+    scene = diff_scene.ds if isinstance(diff_scene, DiffViews) else diff_scene
+
     class RenderFn(torch.autograd.Function):
         @staticmethod
         def forward(ctx, scalars, em):
             first = state["next"]
             state["next"] += span
-            diff_scene.set_params(scalars.detach().cpu().numpy(),
-                                  None if em is None else em.detach().cpu().numpy(), refresh_sampling=True)
+            scene.set_params(scalars.detach().cpu().numpy(),
+                             None if em is None else em.detach().cpu().numpy(), refresh_sampling=True)
             img = diff_scene.render(first, n_iter)
             ctx.first = first
             ctx.has_em = em is not None

@@ -7,7 +7,8 @@ truth; the optimisation starts from alpha = 0.3 and a constant grey EM at the gr
 radiance, and runs Adam (FP32 parameters) on the logs of alpha and of the EM texels (both stay
 positive and move by comparable relative amounts per step), each step rendering every view with
 fresh samples and back-propagating through the path-replay adjoint (FP64 EM-gradient accumulation
-on the GPU), via the torch.autograd.Function of each view.
+on the GPU).  The 8 views are one slot with 8 cameras (DiffViews): each step renders all of them in
+one launch and back-propagates through one adjoint launch, via one torch.autograd.Function.
 
 Defaults (chosen by a sweep on this workload): steps of 0.02 (alpha) and 0.1 (EM) under a cosine
 decay to a tenth, a smoothness prior of weight 0.1 on the log EM, and gradients from samples
@@ -21,7 +22,7 @@ import time
 
 import numpy as np
 
-from .diff import DiffScene, render_function
+from .diff import DiffScene, DiffViews, render_function
 from .renderer import SceneBuild
 
 SPHERE_MATS = [3, 4, 5, 6]      # workload 5: floor materials 0..2, then the four spheres
@@ -38,15 +39,14 @@ def inverse_rendering(ctx, steps=200, res=256, spp=16, target_spp=64, lr=0.02, l
     import torch
 
     scenes = [SceneBuild.workload(5, seed=seed, variant=v, res=res) for v in range(views)]
-    diffs = []
-    for v, sb in enumerate(scenes):
-        ctx.upload(sb, v)
-        diffs.append(DiffScene(ctx, sb, SPHERE_MATS, slot=v))
-    theta_gt, em_gt = diffs[0].get_params()
+    ctx.upload(scenes[0], 0)
+    diff = DiffScene(ctx, scenes[0], SPHERE_MATS, slot=0)
+    multi = DiffViews(diff, scenes)       # view v: the camera and RNG streams of scenes[v]
+    theta_gt, em_gt = diff.get_params()
     alpha_gt = theta_gt[3::4].copy()
-    targets = [torch.from_numpy(d.render(1 << 20, target_spp)) for d in diffs]
+    targets = torch.from_numpy(multi.render(1 << 20, target_spp))
     # noise floor of the loss: the ground truth rendered at the optimisation's spp
-    loss_floor = float(np.mean([np.mean((d.render(1 << 21, spp) - t.numpy()) ** 2) for d, t in zip(diffs, targets)]))
+    loss_floor = float(np.mean((multi.render(1 << 21, spp) - targets.numpy()) ** 2))
 
     tints = torch.tensor(theta_gt.reshape(4, 4)[:, :3])             # known, not optimised
     alpha_log = torch.full((4,), float(np.log(0.3)), dtype=torch.float32, requires_grad=True)
@@ -58,7 +58,7 @@ def inverse_rendering(ctx, steps=200, res=256, spp=16, target_spp=64, lr=0.02, l
     # every view, so only these are recoverable
     upper = np.zeros(em_gt.shape[:2], bool)
     upper[: em_gt.shape[0] // 2] = True
-    fns = [render_function(d, spp, decorrelate) for d in diffs]
+    fn = render_function(multi, spp, decorrelate)
 
     def rel_rmse(em, mask=None):
         d, g = (em - em_gt), em_gt
@@ -82,10 +82,8 @@ def inverse_rendering(ctx, steps=200, res=256, spp=16, target_spp=64, lr=0.02, l
         opt.zero_grad()
         scal = torch.cat([tints, torch.exp(alpha_log)[:, None]], dim=1).reshape(-1)
         em = torch.exp(em_log)
-        loss = 0.0
-        for fn, tgt in zip(fns, targets):
-            img = fn.apply(scal, em)
-            loss = loss + torch.mean((img - tgt) ** 2) / len(fns)
+        imgs = fn.apply(scal, em)
+        loss = torch.mean((imgs - targets) ** 2)     # the mean over the views of each view's L2 loss
         total = loss
         if em_smooth > 0.0:
             dx = em_log - torch.roll(em_log, 1, dims=1)

@@ -157,3 +157,56 @@ def test_decorrelated_autograd_function(setup):
         # an independent block: not the gradient of the image's own samples
         gs_own, _ = ds.adjoint(2 * call * SPP, SPP, dl)
         assert not np.allclose(gs_own, gs, rtol=1e-3)
+
+
+def test_multi_view_forward_and_adjoint_equal_per_view_slots():
+    """Eight cameras of the inverse-rendering workload as one slot (pg3g_diff_set_views): one
+    launch renders every view and one adjoint launch returns the gradients summed over the views.
+    View v takes the camera and RNG scene id of the scene built for view v, so its image equals
+    that scene's own slot render (up to the order of FP64 additions), and the summed gradients
+    equal the sum of the eight per-slot adjoints of the same samples."""
+    from pg3render_amd import DiffViews
+    ctx = Context(0)
+    try:
+        res, spp, views = 64, 4, 8
+        scenes = [SceneBuild.workload(5, seed=7, variant=v, res=res) for v in range(views)]
+        mats = [3, 4, 5, 6]
+        per = []
+        for v, sb in enumerate(scenes):
+            ctx.upload(sb, 1 + v)
+            per.append(DiffScene(ctx, sb, mats, slot=1 + v))
+        ctx.upload(scenes[0], 0)
+        base = DiffScene(ctx, scenes[0], mats, slot=0)
+        multi = DiffViews(base, scenes)
+        theta, em = base.get_params()
+        rng = np.random.default_rng(3)
+        theta2 = theta * rng.uniform(0.8, 1.2, theta.shape).astype(np.float32)
+        em2 = em * rng.uniform(0.7, 1.3, em.shape).astype(np.float32)
+        for d in [base] + per:
+            d.set_params(theta2, em2, refresh_sampling=True)
+        imgs = multi.render(40, spp)
+        singles = np.stack([d.render(40, spp) for d in per])
+        rel = np.abs(imgs - singles) / np.maximum(np.abs(singles), 1e-6)
+        print(f"  images: max relative difference {rel.max():.2e}")
+        assert imgs.shape == (views, res, res, 3)
+        assert rel.max() < 1e-9
+        adj = rng.standard_normal(imgs.shape).astype(np.float32)
+        gs, ge = multi.adjoint(40, spp, adj)                 # reuses the forward's per-sample radiance
+        gs_sum = np.zeros_like(gs)
+        ge_sum = np.zeros_like(ge)
+        for v, d in enumerate(per):
+            g1, e1 = d.adjoint(40, spp, adj[v])
+            gs_sum += g1
+            ge_sum += e1
+        es = np.abs(gs - gs_sum) / np.maximum(np.abs(gs_sum), 1e-12)
+        ee = np.abs(ge - ge_sum).max() / np.abs(ge_sum).max()
+        print(f"  gradients: scalars max relative difference {es.max():.2e}, EM max difference / max {ee:.2e}")
+        assert np.abs(gs_sum).max() > 0 and np.abs(ge_sum).max() > 0
+        assert es.max() < 1e-6 and ee < 1e-6
+        # an adjoint of other iterations replays its own primal pass first: same gradients as the
+        # per-slot adjoints of those iterations
+        gs3, _ = multi.adjoint(80, spp, adj)
+        gs3_sum = sum(d.adjoint(80, spp, adj[v])[0] for v, d in enumerate(per))
+        assert np.max(np.abs(gs3 - gs3_sum) / np.maximum(np.abs(gs3_sum), 1e-12)) < 1e-6
+    finally:
+        ctx.close()
