@@ -7,6 +7,7 @@
 // of the same iterations produce the same per-pixel sums.
 #include <hip/hip_runtime.h>
 
+#include "pg3g.h"
 #include "pg3g_internal.hpp"
 #include "pt_kernels.hpp"
 
@@ -30,8 +31,11 @@ __device__ __forceinline__ uint32_t SceneOfItem(const BatchScene *__restrict__ i
 // entering a scene takes a slot holding that scene, or claims a slot no wavefront uses and stages
 // the scene's tables into it with its own lanes; each takes one compare-and-swap, and a wavefront
 // that gets no slot reads the scene's tables from memory.  No wavefront ever waits for another.
+constexpr uint32_t kBatchBlockWaves = 16;   // wavefronts of the largest batch workgroup (1024 lanes)
 constexpr uint32_t kSlotStaging = 1u << 31;
 constexpr uint32_t kSlotSlots = 2;
+static_assert(PG3G_MAX_SCENES + 1 < (1u << 15), "scene + 1 fits the slot word's 15 scene bits");
+static_assert(kBatchBlockWaves < (1u << 16), "the slot word's user count holds every wavefront of a workgroup");
 struct BatchSlots {
     PG3_DEV static F4v *Tables(uint32_t k) {
         __shared__ F4v tables[kSlotSlots][kTreeLdsNodes * 3];
