@@ -760,6 +760,134 @@ def test_steerable_refinement_stays_near_a_bright_texel():
     assert ang[extra].max() < 45.0
 
 
+def _refine_reference(img, max_err, min_level, max_level):
+    """The adaptive triangulation of the steerable sampler restated in float64 from the reference
+    (GenerateInitialEmTriangulation / UnitIcosahedron, em_steerable_sampler.hxx:2993-3025 and
+    geom.hxx:291-345; RefineEmTriangulation with TriangleHasToBeSubdivided(Impl),
+    SubdivTestSamplesPerDim and IsEstimationErrorTooLarge, em_steerable_sampler.hxx:3090-3433, with
+    the BuildParameters defaults :36-56: at most 50 test samples per dimension, oversampling 0.7,
+    span 1.1; TriangleNode::EvaluateLuminanceApprox :1475-1500; Geom::Triangle::BarycentricCoords
+    and MapCartToBary, geom.hxx:634-674, 820-833; DirToLatLong geom.hxx:80-96 and the tent filter
+    of _tent_reference, vectorised).  Returns the leaf triangles as (n, 3, 3) directions."""
+    import collections
+    h, w = img.shape[:2]
+    lum_img = img.astype(np.float64) @ np.array([0.212671, 0.715160, 0.072169])
+
+    def em_lum(d):
+        d = np.atleast_2d(d)
+        phi = -np.arctan2(d[:, 1], d[:, 0])
+        u = np.clip(0.5 + phi * 0.5 / np.pi, 0.0, 1.0)
+        v = np.clip(np.arccos(np.clip(d[:, 2], -1.0, 1.0)) / np.pi, 0.0, 1.0)
+        xf, yf = u * w, v * h
+        cx, cy = np.floor(xf + 0.5).astype(int), np.floor(yf + 0.5).astype(int)
+        x0, y0 = cx - 1, cy - 1
+        xl, yl = xf - (x0 + 0.5), yf - (y0 + 0.5)
+
+        def norm(x, y):
+            over, under = y >= h, y < 0
+            x = np.where(over | under, x + w // 2, x)
+            y = np.where(over, h - 1, np.where(under, 0, y))
+            return x % w, y
+        (ax, ay), (bx, by) = norm(x0, y0), norm(cx, cy)
+        top = (1 - xl) * lum_img[ay, ax] + xl * lum_img[ay, bx]
+        bot = (1 - xl) * lum_img[by, ax] + xl * lum_img[by, bx]
+        return (1 - yl) * top + yl * bot
+
+    def unit(v):
+        return v / np.linalg.norm(v)
+
+    def sin_of(d):
+        return np.sqrt(max(0.0, 1.0 - d[2] ** 2))
+
+    def error_too_large(whole, s0, s1, s2, spd):
+        i, j = np.meshgrid(np.arange(spd + 1), np.arange(spd + 1), indexing="ij")
+        x, y = (i.ravel() / spd) ** 2, j.ravel() / spd
+        xs = np.sqrt(x)
+        b0, b1 = 1 - xs, y * xs
+        p = b0[:, None] * s0 + b1[:, None] * s1 + (1 - b0 - b1)[:, None] * s2
+        W0, W1, W2 = whole
+        N = np.cross(W1 - W0, W2 - W0)
+        den = N @ N
+        bx = np.cross(W2 - W1, p - W1) @ N / den
+        by = np.cross(W0 - W2, p - W2) @ N / den
+        bx, by = np.clip(bx, 0, 1), np.clip(by, 0, 1)
+        l0, l1, l2 = em_lum(W0)[0], em_lum(W1)[0], em_lum(W2)[0]
+        approx = bx * l0 + by * l1 + np.clip(1 - bx - by, 0, 1) * l2
+        em = em_lum(p / np.linalg.norm(p, axis=1, keepdims=True))
+        mn, mx = np.minimum(em, approx), np.maximum(em, approx)
+        return bool(np.any((mx + 1e-4) > (mn + 1e-4) * (1 + max_err)))
+
+    def impl(v0, s0, v1, s1, v2, s2, whole):
+        cen = (v0 + v1 + v2) / 3
+        e01, e12, e20, cd = unit((v0 + v1) / 2), unit((v1 + v2) / 2), unit((v2 + v0) / 2), unit(cen)
+        se01, se12, se20 = sin_of(e01), sin_of(e12), sin_of(e20)
+        sins = [s0, s1, s2, se01, se12, se20, sin_of(cd)]
+        pole = np.sin(0.5 * (np.pi / 2) / h)
+        lo, hi = max(min(sins), pole), max(max(sins), pole)
+        ang = np.minimum(np.array([min(np.pi / h, lo * 2 * np.pi / w), min(np.pi / h, hi * 2 * np.pi / w)]) / 2,
+                         np.pi / 2 - 0.1)
+        planar = np.tan(ang) * np.linalg.norm(cen)
+        avg = (np.sum((v0 - v1) ** 2) + np.sum((v1 - v2) ** 2) + np.sum((v2 - v0) ** 2)) / 3
+        spd = np.sqrt(avg / (planar ** 2 / 2) / 2) * 0.7
+        max_spd, min_spd = spd[0], spd[1]
+        spd_i = int(np.ceil(max_spd))
+        if 50 < spd_i:
+            return True
+        if max_spd / min_spd >= 1.1 and max_spd > 32.0:
+            return (impl(v0, s0, e01, se01, e20, se20, whole) or impl(v1, s1, e12, se12, e01, se01, whole) or
+                    impl(v2, s2, e20, se20, e12, se12, whole) or impl(e01, se01, e12, se12, e20, se20, whole))
+        return error_too_large(whole, v0, v1, v2, spd_i)
+
+    t = (1 + np.sqrt(5.0)) / 2
+    V = np.array([[t, 1, 0], [-t, 1, 0], [t, -1, 0], [-t, -1, 0], [1, 0, t], [1, 0, -t], [-1, 0, t], [-1, 0, -t],
+                  [0, t, 1], [0, -t, 1], [0, t, -1], [0, -t, -1]]) / np.sqrt(1 + t * t)
+    F = [(0, 8, 4), (1, 10, 7), (2, 9, 11), (7, 3, 1), (0, 5, 10), (3, 9, 6), (3, 11, 9), (8, 6, 4), (2, 4, 9),
+         (3, 7, 11), (4, 2, 0), (9, 4, 6), (2, 11, 5), (0, 10, 8), (5, 0, 2), (10, 5, 7), (1, 6, 8), (1, 8, 10),
+         (6, 1, 3), (11, 7, 5)]
+    todo = collections.deque((V[a], V[b], V[c], 0) for a, b, c in F)
+    leaves = []
+    while todo:
+        a, b, c, lvl = todo.popleft()
+        if lvl < min_level:
+            split = True
+        elif lvl >= max_level:
+            split = False
+        else:
+            split = impl(a, sin_of(a), b, sin_of(b), c, sin_of(c), (a, b, c))
+        if split:
+            n0, n1, n2 = unit((a + b) / 2), unit((b + c) / 2), unit((c + a) / 2)
+            todo.extendleft([(n0, n1, n2, lvl + 1), (a, n0, n2, lvl + 1), (n0, b, n1, lvl + 1), (n1, c, n2, lvl + 1)])
+        else:
+            leaves.append((a, b, c))
+    return np.array(leaves)
+
+
+def test_steerable_refinement_matches_a_float64_restatement():
+    """The refinement decision at intermediate error thresholds (which of the reference's test
+    samples exceeds 1 + maxError against the triangle's linear approximation, with its
+    samples-per-dimension rule and sub-triangle split, em_steerable_sampler.hxx:3160-3433): the
+    oracle's leaves equal those of a float64 restatement written from the reference (above) on a
+    noisy map and a map of narrow bright lobes, at five thresholds.  A planted mistake in the
+    oracle's test (the two sample coordinates transposed, an oversampling factor of 0.75 for 0.7)
+    changes the leaf set; the clamp of the barycentric coordinates does not matter here (every test
+    sample lies inside the tested triangle, so it only absorbs rounding)."""
+    rng = np.random.default_rng(31)
+    h, w = 16, 32
+    noisy = (0.2 + rng.random((h, w, 3)) ** 2 * 4).astype(np.float32)
+    yy, xx = np.meshgrid(np.arange(h) + 0.5, np.arange(w) + 0.5, indexing="ij")
+    lobes = np.full((h, w, 3), 0.3, np.float32)
+    for cx, cy, amp in ((8.0, 4.0, 40.0), (22.0, 11.0, 15.0)):
+        lobes += (amp * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / 1.5))[..., None].astype(np.float32)
+    for name, img, levels in (("noisy", noisy, (1, 4)), ("lobes", lobes, (1, 5))):
+        for err in (0.05, 0.3, 1.0, 2.0, 5.0):
+            got, _, _ = O.OracleScene(_steer_scene(img, levels=levels, max_error=err)).steerable()
+            want = _refine_reference(img, err, *levels)
+            a, b = _leaf_set(got), _leaf_set(want.astype(np.float32))
+            print(f"  {name} max error {err}: oracle {len(got)} leaves, restatement {len(want)}, "
+                  f"differing {len(a ^ b)}")
+            assert len(got) == len(want) and a == b, (name, err, len(got), len(want), len(a ^ b))
+
+
 def _tent_reference(img, u, v):
     """EnvironmentMapImage::Evaluate with the tent filter (em_image.hxx:95-142) and its coordinate
     wrap (NormalizeImgCoords, :245-278) restated from the reference: a row beyond a pole wraps to
