@@ -163,8 +163,25 @@ def test_em_parity(ctx, sampler):
     # evaluation
     d = dirs(rng, n, True)
     eg, eo = ctx.em_eval(d), orc.em_eval(d)
-    tier1_report(f"sampler {sampler} EM eval", rel_err(eg, eo, 1e-3))
-    assert np.quantile(rel_err(eg, eo, 1e-3), 0.999) <= REL
+    e_eval = rel_err(eg, eo, 1e-3).max(axis=1)
+    tier1_report(f"sampler {sampler} EM eval", e_eval)
+    # Every sample: within 1e-5 plus what the oracle's own value moves when its lat-long
+    # coordinates move by up to 2 ulp (the GPU's atan2/acos and the host's differ by an ulp or so;
+    # next to the sun's steep gradient that ulp moves the tent-filtered value by more than 1e-5).
+    uv = np.array([O.dir2latlong(x) for x in d], np.float32)
+    env = np.zeros(n)
+    for axis in (0, 1):
+        for k in (-2, -1, 1, 2):
+            q = uv.copy()
+            q[:, axis] = np.nextafter(q[:, axis], np.float32(np.sign(k) * 2.0))
+            if abs(k) == 2:
+                q[:, axis] = np.nextafter(q[:, axis], np.float32(np.sign(k) * 2.0))
+            moved = rel_err(orc.em_eval_uv(np.clip(q, 0.0, 1.0)), eo, 1e-3).max(axis=1)
+            env = np.maximum(env, moved)
+    excess = e_eval - (REL + env)
+    print(f"sampler {sampler} EM eval: worst excess over 1e-5 + 2-ulp envelope {excess.max():.2e}, "
+          f"samples above 1e-5 {int(np.sum(e_eval > REL))}, all within the envelope")
+    assert np.all(excess <= 0.0), (e_eval[excess > 0], env[excess > 0])
     # sampling
     dg, pg, rg, okg = ctx.em_sample(normals, sides, u)
     do, po, ro, oko = orc.em_sample(normals, sides, u)
@@ -181,12 +198,15 @@ def test_em_parity(ctx, sampler):
     for e in (ep, er):
         assert np.quantile(e, 0.995) <= REL * 2, e.max()
         assert np.mean(e > 1e-3) < 2e-3
-    # pdf of arbitrary directions
+    # pdf of arbitrary directions.  The simple sampler's pdf is a difference of adjacent FP32 CDF
+    # entries times the row width; the GPU's parallel FP64 scan adds in another order than the
+    # oracle's sequential sum, so an entry occasionally rounds to the neighbouring float, and one
+    # ulp (6e-8) over an entry difference of ~1/256 is ~1.5e-5 of the pdf.  Printed: the tail.
     pg2 = ctx.em_pdf(d, normals, sides)
     po2 = orc.em_pdf(d, normals, sides)
     e = rel_err(pg2, po2, 1e-6)
     tier1_report(f"sampler {sampler} EM pdf", e)
-    assert np.quantile(e, 0.999) <= REL * 2 and np.mean(e > 1e-3) < 1e-3
+    assert np.quantile(e, 0.999) <= REL * 2 and np.mean(e > 1e-3) < 1e-3 and e.max() < 1e-4
 
 
 def test_steerable_tables_match_oracle_build(ctx):
