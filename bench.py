@@ -35,7 +35,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", type=int, default=2, help="2: steerable-EM box 512^2 x 64 spp")
     ap.add_argument("--kernel", choices=["megakernel", "wavefront"], default="megakernel")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
