@@ -109,6 +109,7 @@ _SIGNATURES = {
     "pg3g_synchronize": (C.c_int, [_P]),
     "pg3g_read_counters": (C.c_int, [_P, _U64P, _U64P]),
     "pg3g_read_ray_counters": (C.c_int, [_P, _U64P, _U64P]),
+    "pg3g_run_iteration": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_int]),
     "pg3g_scene_primitives": (C.c_int, [_P, C.c_uint32, _U32P, _U32P, _U32P]),
     "pg3g_adjoint_noalloc": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pg3g_reset_counters": (C.c_int, [_P]),

@@ -84,6 +84,7 @@ struct SceneSlot {
     std::vector<int32_t> gradBaseHost;
     uint32_t numScalars = 0;
     uint64_t iterations = 0;
+    uint32_t pendFirst = 0, pendCount = 0;    // iterations queued by pg3g_run_iteration, not launched
     SteerableTables steer;
     bool hasSteer = false;
 };
@@ -439,6 +440,7 @@ int UploadScene(pg3g_context *ctx, uint32_t slotIdx, const pg3g_scene_desc *d) {
     S.diffMats.clear();
     S.numScalars = 0;
     S.numViews = 0;
+    S.pendCount = 0;
     S.gradBaseHost.assign(d->num_materials, -1);
     std::vector<DevLight> lights(d->num_lights);
     v.bgLight = -1;
@@ -811,11 +813,47 @@ int pg3g_upload_scene_slot(pg3g_context *ctx, uint32_t slot, const pg3g_scene_de
     return UploadScene(ctx, slot, desc);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Launches iterations [first_iter, first_iter + n_iter) of a slot.
+int RenderSlot(pg3g_context *ctx, SceneSlot *S, uint32_t first_iter, uint32_t n_iter) {
+    if ((uint64_t)first_iter + n_iter > 0xFFFFFFFFull) return Fail(PG3G_ERR_INVALID_ARG, "iteration range overflows");
+    PG3_HIP(hipSetDevice(ctx->device));
+    if (ctx->kernelMode == PG3G_KERNEL_WAVEFRONT) {
+        if (!ctx->wavefront) ctx->wavefront = CreateWavefrontContext();
+        if (!ctx->wavefront) return Fail(PG3G_ERR_OUT_OF_MEMORY, "host allocation failed");
+        PG3_HIP(LaunchWavefront(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(),
+                                ctx->counters.as<unsigned long long>(), ctx->work.as<uint32_t>(), ctx->numCUs,
+                                ctx->wavefront, ctx->stream));
+    } else {
+        PG3_HIP(LaunchMegakernel(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(),
+                                 ctx->counters.as<unsigned long long>(), ctx->work.as<uint32_t>(), ctx->numCUs,
+                                 ctx->stream));
+    }
+    S->iterations += n_iter;
+    return PG3G_OK;
+}
+
+// Launches the iterations pg3g_run_iteration has queued for a slot.
+int FlushQueued(pg3g_context *ctx, SceneSlot *S) {
+    if (!S || S->pendCount == 0) return PG3G_OK;
+    const uint32_t first = S->pendFirst, n = S->pendCount;
+    S->pendCount = 0;
+    return RenderSlot(ctx, S, first, n);
+}
+
+}  // namespace
+
+extern "C" {
+
 int pg3g_set_params(pg3g_context *ctx, uint32_t slot, const pg3g_render_params *p) {
     SceneSlot *S = GetSlot(ctx, slot);
     if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
     if (!p) return Fail(PG3G_ERR_INVALID_ARG, "params is null");
     if (int rc = ValidateParams(*p)) return rc;
+    if (int rc = FlushQueued(ctx, S)) return rc;               // queued iterations use the old parameters
     S->params = *p;
     ApplyParams(S->view, *p);
     PG3_HIP(hipSetDevice(ctx->device));
@@ -834,20 +872,18 @@ int pg3g_render(pg3g_context *ctx, uint32_t first_iter, uint32_t n_iter) { retur
 int pg3g_render_slot(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter) {
     SceneSlot *S = GetSlot(ctx, slot);
     if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
-    if ((uint64_t)first_iter + n_iter > 0xFFFFFFFFull) return Fail(PG3G_ERR_INVALID_ARG, "iteration range overflows");
-    PG3_HIP(hipSetDevice(ctx->device));
-    if (ctx->kernelMode == PG3G_KERNEL_WAVEFRONT) {
-        if (!ctx->wavefront) ctx->wavefront = CreateWavefrontContext();
-        if (!ctx->wavefront) return Fail(PG3G_ERR_OUT_OF_MEMORY, "host allocation failed");
-        PG3_HIP(LaunchWavefront(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(),
-                                ctx->counters.as<unsigned long long>(), ctx->work.as<uint32_t>(), ctx->numCUs,
-                                ctx->wavefront, ctx->stream));
-    } else {
-        PG3_HIP(LaunchMegakernel(S->view, S->viewDev.as<SceneView>(), first_iter, n_iter, S->fb.as<double>(),
-                                 ctx->counters.as<unsigned long long>(), ctx->work.as<uint32_t>(), ctx->numCUs,
-                                 ctx->stream));
-    }
-    S->iterations += n_iter;
+    if (int rc = FlushQueued(ctx, S)) return rc;
+    return RenderSlot(ctx, S, first_iter, n_iter);
+}
+
+int pg3g_run_iteration(pg3g_context *ctx, uint32_t slot, uint32_t iteration, int launch_now) {
+    SceneSlot *S = GetSlot(ctx, slot);
+    if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
+    if (S->pendCount > 0 && (uint64_t)iteration != (uint64_t)S->pendFirst + S->pendCount)
+        if (int rc = FlushQueued(ctx, S)) return rc;            // not the next iteration: launch the run
+    if (S->pendCount == 0) S->pendFirst = iteration;
+    ++S->pendCount;
+    if (launch_now || S->pendCount >= PG3G_ITERATION_RUN) return FlushQueued(ctx, S);
     return PG3G_OK;
 }
 
@@ -878,6 +914,7 @@ int pg3g_read_framebuffer(pg3g_context *ctx, double *sum_rgb, uint64_t *iteratio
 int pg3g_read_framebuffer_slot(pg3g_context *ctx, uint32_t slot, double *sum_rgb, uint64_t *iterations) {
     SceneSlot *S = GetSlot(ctx, slot);
     if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
+    if (int rc = FlushQueued(ctx, S)) return rc;
     PG3_HIP(hipSetDevice(ctx->device));
     if (sum_rgb) PG3_HIP(hipMemcpyAsync(sum_rgb, S->fb.p, sizeof(double) * 3 * (size_t)S->view.resX * S->view.resY,
                                         hipMemcpyDeviceToHost, ctx->stream));
@@ -889,6 +926,7 @@ int pg3g_read_framebuffer_slot(pg3g_context *ctx, uint32_t slot, double *sum_rgb
 int pg3g_clear_framebuffer(pg3g_context *ctx, uint32_t slot) {
     SceneSlot *S = GetSlot(ctx, slot);
     if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
+    S->pendCount = 0;                                             // queued iterations are dropped
     PG3_HIP(hipSetDevice(ctx->device));
     PG3_HIP(hipMemsetAsync(S->fb.p, 0, S->fb.bytes, ctx->stream));
     S->iterations = 0;
@@ -899,12 +937,16 @@ int pg3g_framebuffer_device_ptr(pg3g_context *ctx, uint32_t slot, double **fb_de
     SceneSlot *S = GetSlot(ctx, slot);
     if (!S) return Fail(PG3G_ERR_NO_SCENE, "no scene in slot");
     if (!fb_dev) return Fail(PG3G_ERR_INVALID_ARG, "fb_dev is null");
+    if (int rc = FlushQueued(ctx, S)) return rc;
     *fb_dev = S->fb.as<double>();
     return PG3G_OK;
 }
 
 int pg3g_synchronize(pg3g_context *ctx) {
     if (!ctx) return Fail(PG3G_ERR_INVALID_ARG, "ctx is null");
+    for (auto &slot : ctx->slots)
+        if (slot && slot->valid)
+            if (int rc = FlushQueued(ctx, slot.get())) return rc;
     PG3_HIP(hipSetDevice(ctx->device));
     PG3_HIP(hipStreamSynchronize(ctx->stream));
     return PG3G_OK;
@@ -1308,13 +1350,24 @@ int pg3g_batch_get_config(const pg3g_batch *batch, pg3g_batch_config *config) {
     return PG3G_OK;
 }
 
+namespace {
+// Launches the iterations pg3g_run_iteration queued on the batch's scenes (they precede the pass).
+int FlushBatchSlots(pg3g_batch *b) {
+    for (uint32_t s : b->slots)
+        if (int rc = FlushQueued(b->ctx, GetSlot(b->ctx, s))) return rc;
+    return PG3G_OK;
+}
+}  // namespace
+
 int pg3g_batch_render(pg3g_batch *batch, uint32_t passes) {
     if (int rc = BatchCheck(batch)) return rc;
+    if (int rc = FlushBatchSlots(batch)) return rc;
     return BatchRenderPasses(batch, passes);
 }
 
 int pg3g_batch_render_timed(pg3g_batch *batch, uint32_t passes, float *elapsed_ms) {
     if (int rc = BatchCheck(batch)) return rc;
+    if (int rc = FlushBatchSlots(batch)) return rc;
     return BatchTimed(batch, passes, elapsed_ms);
 }
 
@@ -1396,6 +1449,7 @@ int pg3g_diff_set_params(pg3g_context *ctx, uint32_t slot, const float *scalars,
     SceneSlot *S = nullptr;
     if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
     if (em_texels && S->view.em.w == 0) return Fail(PG3G_ERR_INVALID_ARG, "the scene has no EM image");
+    if (int rc = FlushQueued(ctx, S)) return rc;                  // queued iterations use the old parameters
     PG3_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     SceneView &v = S->view;
@@ -1591,6 +1645,7 @@ extern "C" {
 int pg3g_render_diff(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter, float *elapsed_ms) {
     SceneSlot *S = nullptr;
     if (int rc = CheckDiffSlot(ctx, slot, S)) return rc;
+    if (int rc = FlushQueued(ctx, S)) return rc;
     if (int rc = RenderDiffImpl(ctx, S, first_iter, n_iter, nullptr, 1, S->fb.as<double>(), elapsed_ms)) return rc;
     S->iterations += n_iter;
     return PG3G_OK;

@@ -189,6 +189,10 @@ class Context:
     def render(self, first_iter, n_iter, slot=0):
         check(self._lib.pg3g_render_slot(self.ptr, slot, first_iter, n_iter), "pg3g_render_slot")
 
+    def run_iteration(self, iteration, slot=0, launch_now=False):
+        """Queues one RunIteration (pg3g_run_iteration): consecutive iterations launch together."""
+        check(self._lib.pg3g_run_iteration(self.ptr, slot, iteration, 1 if launch_now else 0), "pg3g_run_iteration")
+
     def render_timed(self, first_iter, n_iter, slot=0):
         """Renders and returns the GPU time (ms) between HIP events on the library's stream."""
         ms = C.c_float()
@@ -337,9 +341,11 @@ class GpuPathTracer:
         self.ctx.upload(scene, slot)
         self.iterations = 0
 
-    def run_iteration(self, iteration):
-        """One sample per pixel (PathTracerBase::RunIteration, src/path_tracer_base.hxx:57-90)."""
-        self.ctx.render(iteration, 1, self.slot)
+    def run_iteration(self, iteration, last=False):
+        """One sample per pixel (PathTracerBase::RunIteration, src/path_tracer_base.hxx:57-90),
+        queued: consecutive iterations launch together (pg3g_run_iteration); `last` launches what is
+        queued at once.  get_framebuffer launches whatever is still queued."""
+        self.ctx.run_iteration(iteration, self.slot, launch_now=last)
         self.iterations += 1
 
     def run_iterations(self, first, count):

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from pg3render_amd import Context, SceneBuild, _ffi
+from pg3render_amd import Context, GpuPathTracer, SceneBuild, _ffi
 from pg3render_amd.scene_desc import CustomScene, light, material
 
 pytestmark = pytest.mark.gpu
@@ -386,6 +386,51 @@ def test_iterations_accumulate_like_run_iteration(ctx):
     b, _ = ctx.framebuffer(0, sb.resolution)
     assert it == 8
     assert np.allclose(a, b, rtol=1e-12, atol=1e-12)
+
+
+def test_queued_run_iterations_equal_one_launch(ctx):
+    """pg3g_run_iteration, RunIteration called once per iteration (renderer.hxx:232-278): 100
+    consecutive calls (a run of 64 launched by the 64th, the other 36 by the last call) equal one
+    launch of iterations [0,100); a read launches what is still queued; an iteration that does not
+    follow the queued ones launches them first; a clear drops what is queued."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(1, seed=24, res=32)
+
+    def plain(ranges):
+        ctx.upload(sb, 7)
+        for first, n in ranges:
+            ctx.render(first, n, 7)
+        return ctx.framebuffer(7, sb.resolution)
+
+    ctx.upload(sb, 6)
+    for i in range(100):
+        ctx.run_iteration(i, 6, launch_now=(i == 99))
+    a, it = ctx.framebuffer(6, sb.resolution)
+    b, _ = plain([(0, 100)])
+    assert it == 100
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12)
+
+    ctx.clear(6)
+    for i in list(range(0, 5)) + list(range(10, 15)):      # a gap after iteration 4
+        ctx.run_iteration(i, 6)
+    a, it = ctx.framebuffer(6, sb.resolution)              # the read launches 10..14
+    b, _ = plain([(0, 5), (10, 5)])
+    assert it == 10
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12)
+
+    ctx.clear(6)
+    for i in range(3):
+        ctx.run_iteration(i, 6)
+    ctx.clear(6)                                           # drops the three queued iterations
+    a, it = ctx.framebuffer(6, sb.resolution)
+    assert it == 0 and not a.any()
+
+    tracer = GpuPathTracer(sb, context=ctx, slot=6)        # the AbstractRenderer-shaped loop
+    for i in range(70):
+        tracer.run_iteration(i)
+    img = tracer.get_framebuffer()
+    b, _ = plain([(0, 70)])
+    np.testing.assert_allclose(img, b / 70, rtol=1e-12, atol=1e-12)
 
 
 def test_unsupported_parameters_are_rejected(ctx):
