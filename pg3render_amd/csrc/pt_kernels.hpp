@@ -1429,9 +1429,20 @@ struct SplitPath {
         cold.Set(f + 1, __float_as_uint(v.y));
         cold.Set(f + 2, __float_as_uint(v.z));
     }
+    // A record: the vertex's A and the indirect weight W = attCos / den of its BSDF sample (0
+    // without an indirect term), as one 16-byte and one 8-byte access.
     struct Ring {
-        float r[kChainRing][7];  // A.xyz, attCos.xyz, den
+        float4 a[kChainRing];   // A.xyz, W.x
+        float2 b[kChainRing];   // W.yz
     };
+    PG3_DEV static F3 IndirectWeight(F3 attCos, float den) { return den > 0.f ? FastDiv(attCos, den) : mk(0.f); }
+    // R_d = A_d + the indirect estimate W_d R_{d+1}, clipped (path_tracer.hxx:380-404)
+    PG3_DEV static F3 FoldRecord(F3 A, F3 W, F3 R, float iic) {
+        if (IsZero(W) || IsZero(R)) return A;
+        F3 ind = W * R;
+        if (iic > 0.f) ind = ClipProportionally(ind, iic);
+        return A + ind;
+    }
 
     // BSDF samples of the first vertex (ComputeSplittingCounts, :440-446): the sub-samples per sample
     PG3_DEV static uint32_t Subs(const SceneView &S) {
@@ -1483,16 +1494,14 @@ struct SplitPath {
     // front; if the oldest records were overwritten, sets up the re-trace up to the oldest folded
     // depth and returns false, else adds the first vertex's estimate to L and returns true.
     PG3_DEV bool Fold(const SceneView &S, uint32_t t, uint32_t jStart, F3 rChild, const Ring &ring) {
-        F3 R = tA;
-        if (tDen > 0.f && !IsZero(rChild)) R += ClippedIndirect(tAttCos, rChild, tDen, S.iic);
+        F3 R = FoldRecord(tA, IndirectWeight(tAttCos, tDen), rChild, S.iic);
         const uint32_t np = nPush(), n = min(np, kChainRing);
 #pragma unroll 1
         for (uint32_t i = 0; i < n; ++i) {
-            const float *r = ring.r[(t - jStart - i) % kChainRing];
-            F3 Rk = mk(r[0], r[1], r[2]);
-            const float den = r[6];
-            if (den > 0.f && !IsZero(R)) Rk += ClippedIndirect(mk(r[3], r[4], r[5]), R, den, S.iic);
-            R = Rk;
+            const uint32_t slot = (t - jStart - i) % kChainRing;
+            const float4 a = ring.a[slot];
+            const float2 b = ring.b[slot];
+            R = FoldRecord(mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), R, S.iic);
         }
         if (np > kChainRing) {
             cold.Set(kTarget, np - kChainRing);
@@ -1517,14 +1526,9 @@ struct SplitPath {
     // Ring record of the parent vertex (its A is complete once the emission its BSDF sample finds
     // is added), pushed at step t.
     PG3_DEV void Push(uint32_t t, Ring &ring) {
-        float *r = ring.r[t % kChainRing];
-        r[0] = tA.x;
-        r[1] = tA.y;
-        r[2] = tA.z;
-        r[3] = tAttCos.x;
-        r[4] = tAttCos.y;
-        r[5] = tAttCos.z;
-        r[6] = tDen;
+        const F3 W = IndirectWeight(tAttCos, tDen);
+        ring.a[t % kChainRing] = make_float4(tA.x, tA.y, tA.z, W.x);
+        ring.b[t % kChainRing] = make_float2(W.y, W.z);
         cold.Set(kPush, nPush() + 1u);
     }
     // BSDF sample si of vertex len (:281-404) with nBv samples at the vertex: the indirect factors
