@@ -1392,6 +1392,10 @@ struct ChainPath {
 #define PG3_CHAIN_RING 32
 #endif
 constexpr uint32_t kChainRing = PG3_CHAIN_RING;
+#ifndef PG3_FOLD_BATCH
+#define PG3_FOLD_BATCH 4
+#endif
+constexpr uint32_t kFoldBatch = PG3_FOLD_BATCH;   // ring records loaded together by a fold
 static_assert(kChainRing >= 2 && (kChainRing & (kChainRing - 1)) == 0,
               "ring slots follow the step counter modulo a power of two, across its wrap-around");
 template <bool kMergeEm>
@@ -1496,12 +1500,21 @@ struct SplitPath {
     PG3_DEV bool Fold(const SceneView &S, uint32_t t, uint32_t jStart, F3 rChild, const Ring &ring) {
         F3 R = FoldRecord(tA, IndirectWeight(tAttCos, tDen), rChild, S.iic);
         const uint32_t np = nPush(), n = min(np, kChainRing);
+        // kFoldBatch records per memory round trip: their loads are issued together (a slot past
+        // the chain's oldest record is still inside the ring and its value is not used)
 #pragma unroll 1
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t slot = (t - jStart - i) % kChainRing;
-            const float4 a = ring.a[slot];
-            const float2 b = ring.b[slot];
-            R = FoldRecord(mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), R, S.iic);
+        for (uint32_t i = 0; i < n; i += kFoldBatch) {
+            float4 a[kFoldBatch];
+            float2 b[kFoldBatch];
+#pragma unroll
+            for (uint32_t j = 0; j < kFoldBatch; ++j) {
+                const uint32_t slot = (t - jStart - i - j) % kChainRing;
+                a[j] = ring.a[slot];
+                b[j] = ring.b[slot];
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kFoldBatch; ++j)
+                if (i + j < n) R = FoldRecord(mk(a[j].x, a[j].y, a[j].z), mk(a[j].w, b[j].x, b[j].y), R, S.iic);
         }
         if (np > kChainRing) {
             cold.Set(kTarget, np - kChainRing);
