@@ -22,13 +22,13 @@ TRI_FLOP, SPH_FLOP, SHADE_FLOP = 62, 35, 400
 BYTES_PER_SAMPLE, BYTES_PER_SEGMENT = 112, 304
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 KiB, factors from a 512 MiB copy): profiles/render_pt_box_steerable_em_512px_64spp.md
-MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 5772003 + 4087868) * 1024)}
+MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 5770405 + 4088079) * 1024)}
 # HBM bytes per frame of the wavefront pipeline on the default workload (all Wf* launches of a
 # frame, same calibration): profiles/wavefront_box_steerable_em_512px_64spp.md
 MEASURED_WF_TRAFFIC = int((2.0 * 16694492 + 17762542) * 1024)
 # HBM bytes per launch of the split kernel at the reference's defaults (same calibration and file)
-MEASURED_SPLIT_TRAFFIC = {"box_area_light_256px_16spp": int((2.0 * 1071797 + 1207018) * 1024),
-                          "box_steerable_em_512px_64spp": int((2.0 * 51649862 + 20672442) * 1024)}
+MEASURED_SPLIT_TRAFFIC = {"box_area_light_256px_16spp": int((2.0 * 1201257 + 1157614) * 1024),
+                          "box_steerable_em_512px_64spp": int((2.0 * 57084618 + 21762099) * 1024)}
 
 
 def parse():
