@@ -23,6 +23,11 @@ BYTES_PER_SAMPLE, BYTES_PER_SEGMENT = 112, 304
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 KiB, factors from a 512 MiB copy): profiles/render_pt_box_steerable_em_512px_64spp.md
 MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 5770405 + 4088079) * 1024)}
+# Wave-level VALU instructions per launch of the dominant kernel (rocprofv3 SQ_INSTS_VALU, same
+# file) and the issue peak they are priced against: 256 CUs x 2 FP32 wave-64 instructions per
+# cycle (four SIMDs) x 2.4 GHz
+MEASURED_VALU_INSTS = {(2, "megakernel"): 7059463088}
+VALU_ISSUE_PEAK = 256 * 2 * 2.4e9
 # HBM bytes per frame of the wavefront pipeline on the default workload (all Wf* launches of a
 # frame, same calibration): profiles/wavefront_box_steerable_em_512px_64spp.md
 MEASURED_WF_TRAFFIC = int((2.0 * 16694492 + 17762542) * 1024)
@@ -337,6 +342,12 @@ def main():
     measured = ({"hbm_measured_gbs": round(traffic / (kernel_ms_avg / 1000.0) / 1e9, 1),
                  "hbm_measured_frac": round(traffic / (kernel_ms_avg / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)}
                 if traffic else {})
+    insts = MEASURED_VALU_INSTS.get((args.workload, args.kernel))
+    if insts:   # what bounds the fused kernel: VALU issue, with every wavefront of a SIMD waiting otherwise
+        rate = insts / (kernel_ms_avg / 1000.0)
+        measured["valu_issue_measured"] = {"wave_instructions": insts, "achieved": round(rate / 1e9, 1),
+                                           "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instr/s",
+                                           "frac": round(rate / VALU_ISSUE_PEAK, 4)}
     roofline = {"bound": bound, "achieved": legs[bound]["achieved"], "peak": legs[bound]["peak"],
                 "unit": legs[bound]["unit"], "frac": legs[bound]["frac"], "traffic": traffic,
                 "legs": legs, "kernel_ms_per_step": round(kernel_ms_avg, 3),

@@ -215,3 +215,39 @@ def test_batch_is_invalidated_by_uploads_and_rejects_bad_input(ctx):
     with pytest.raises(Pg3gError) as e:
         Batch(ctx, [2], [1])
     assert e.value.code == _ffi.PG3G_ERR_UNSUPPORTED
+
+
+def test_batch_pass_and_parameter_change_launch_queued_run_iterations(ctx):
+    """Iterations queued by pg3g_run_iteration (RunIteration, renderer.hxx:232-278) are launched
+    before anything else acts on their slot: a batched pass of the slot adds to them, and a
+    parameter change leaves them rendered with the parameters they were queued under."""
+    scenes = c3_scenes(2)
+    for i, sb in enumerate(scenes):
+        ctx.upload(sb, i)
+    for i in range(len(scenes)):
+        for it in range(3):
+            ctx.run_iteration(it, i)                          # queued, not launched
+    b = Batch(ctx, range(len(scenes)), [sb.spp for sb in scenes])
+    b.set_config(iters_per_item=2, waves_per_cu=16, use_graph=True)
+    b.render(1)                                               # pass 0: iterations [0, spp)
+    got = [ctx.framebuffer(i, sb.resolution) for i, sb in enumerate(scenes)]
+    b.close()
+    for i, sb in enumerate(scenes):
+        ctx.upload(sb, 100 + i)
+        ctx.render(0, 3, 100 + i)
+        ctx.render(0, sb.spp, 100 + i)
+        assert got[i][1] == 3 + sb.spp
+        assert_same(got[i], ctx.framebuffer(100 + i, sb.resolution), f"scene {i}")
+
+    sb = scenes[0]
+    ctx.upload(sb, 0)
+    for it in range(5):
+        ctx.run_iteration(it, 0)
+    changed = type(sb.desc.params).from_buffer_copy(sb.desc.params)
+    changed.max_path_length = 2
+    ctx.set_params(0, changed)                                # launches iterations 0..4 first
+    img, n = ctx.framebuffer(0, sb.resolution)
+    ctx.upload(sb, 100)
+    ctx.render(0, 5, 100)
+    assert n == 5
+    assert_same((img, n), ctx.framebuffer(100, sb.resolution), "parameters of the queued iterations")
