@@ -28,9 +28,12 @@ __device__ __forceinline__ uint32_t SceneOfItem(const BatchScene *__restrict__ i
 // LDS cache of a batch workgroup's sampling tables: two slots, each holding the steerable tree's
 // inner levels (or the simple sampler's marginal CDF) of one scene.  Slot state word: bit 31 =
 // being staged, bits 16-30 = scene + 1 (0: empty), bits 0-15 = wavefronts using it.  A wavefront
-// entering a scene takes a slot holding that scene, or claims a slot no wavefront uses and stages
-// the scene's tables into it with its own lanes; each takes one compare-and-swap, and a wavefront
-// that gets no slot reads the scene's tables from memory.  No wavefront ever waits for another.
+// entering a scene takes a slot holding that scene; if a slot is being staged with that scene it
+// takes none for now; otherwise it claims a slot no wavefront uses and stages the scene's tables
+// into it with its own lanes.  Each takes one compare-and-swap, and a wavefront without a slot
+// reads the scene's tables from memory and, at each work item it takes, joins a slot that has
+// finished staging its scene (the copy equals the tables in memory, so paths in flight may switch
+// to it).  No wavefront ever waits for another.
 constexpr uint32_t kBatchBlockWaves = 16;   // wavefronts of the largest batch workgroup (1024 lanes)
 constexpr uint32_t kSlotStaging = 1u << 31;
 constexpr uint32_t kSlotSlots = 2;
@@ -68,7 +71,24 @@ struct BatchItems {
     int32_t slot;                      // its LDS slot (-1: tables read from memory)
     uint32_t minLevel, baseNode;       // its staged tree levels (TreeCache of StageTree)
     bool marg;                         // the slot holds the marginal CDF instead
+    bool want;                         // the scene has tables an LDS slot can hold
+    // Without a slot: join one that holds the entered scene and is no longer being staged.
+    PG3_DEV void Join() {
+        if (slot >= 0 || !want) return;
+        int32_t got = -1;
+        if (__lane_id() == 0) {
+            uint32_t *state = BatchSlots::State();
+            const uint32_t tag = (cur + 1u) << 16;
+            for (uint32_t k = 0; k < kSlotSlots && got < 0; ++k) {
+                const uint32_t o = __hip_atomic_load(&state[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if ((o & ~0xFFFFu) == tag && (o & 0xFFFFu) < 0xFFFFu && atomicCAS(&state[k], o, o + 1u) == o)
+                    got = (int32_t)k;
+            }
+        }
+        slot = __builtin_amdgcn_readfirstlane(__shfl(got, 0, 64));
+    }
     PG3_DEV bool Next(TileItem &t) {
+        Join();
         uint32_t item;
         for (;;) {
             const uint32_t part = (blockIdx.x + exhausted) % kBatchParts;
@@ -102,6 +122,7 @@ struct BatchItems {
         slot = -1;
         cur = s;
         marg = false;
+        want = false;
         const DevEm &E = views[s].em;
         const int kind = (kEmKind == kEmDynamic) ? (E.w > 0 ? E.sampler : kEmAbsent) : kEmKind;
         uint32_t count = 0;
@@ -115,16 +136,19 @@ struct BatchItems {
         } else {
             return;
         }
+        want = true;
         int32_t got = -1;
         uint32_t stage = 0;
         if (lead) {
             const uint32_t tag = (s + 1u) << 16;
+            bool staging = false;          // a slot is being staged with s: join it later
             for (uint32_t k = 0; k < kSlotSlots && got < 0; ++k) {
                 const uint32_t o = __hip_atomic_load(&state[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if ((o & ~0xFFFFu) == tag && (o & 0xFFFFu) < 0xFFFFu && atomicCAS(&state[k], o, o + 1u) == o)
-                    got = (int32_t)k;
+                if ((o & ~(kSlotStaging | 0xFFFFu)) != tag) continue;
+                if (o & kSlotStaging) staging = true;
+                else if ((o & 0xFFFFu) < 0xFFFFu && atomicCAS(&state[k], o, o + 1u) == o) got = (int32_t)k;
             }
-            for (uint32_t k = 0; k < kSlotSlots && got < 0; ++k) {
+            for (uint32_t k = 0; k < kSlotSlots && got < 0 && !staging; ++k) {
                 const uint32_t o = __hip_atomic_load(&state[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (!(o & kSlotStaging) && (o & 0xFFFFu) == 0u && atomicCAS(&state[k], o, kSlotStaging | tag | 1u) == o) {
                     got = (int32_t)k;
@@ -170,7 +194,7 @@ BatchKernel(const SceneView *__restrict__ views, const BatchScene *__restrict__ 
             uint32_t *__restrict__ work, unsigned long long *__restrict__ counters) {
     if (threadIdx.x < kSlotSlots) BatchSlots::State()[threadIdx.x] = 0u;
     __syncthreads();
-    BatchItems<kEmKind, kOneLight> src{views, info, numScenes, work[kBatchParts], work, 0u, ~0u, -1, 0u, 0u, false};
+    BatchItems<kEmKind, kOneLight> src{views, info, numScenes, work[kBatchParts], work, 0u, ~0u, -1, 0u, 0u, false, false};
     RayTally tally;
     TraceTasks<ChainPath<BatchItems<kEmKind, kOneLight>::kMergeEm>, true>(src, tally);
     FlushTally(tally, counters);
