@@ -35,26 +35,29 @@ struct GradSink {
     F3 adj;            // dLoss/dI(pixel) / N, per channel
     F3 total;          // the sample's radiance from the primal replay
     double *em;        // EM texel gradients (w*h*3), FP64
-    double *scal;      // the wavefront's LDS slice of scalar gradients
+    double *scal;      // the wavefront's LDS slice of scalar gradients (LDS)
     int32_t *cacheKeys;   // workgroup LDS cache (kEmCacheSlots keys, -1 = free), may be null
-    double *cacheVals;    // kEmCacheSlots x 3 FP64 sums
+    double *cacheVals;    // kEmCacheSlots x 3 FP64 sums (LDS)
 };
 
 PG3_DEV void EmGradAdd(const GradSink &g, uint32_t t, F3 v) {
     if (g.cacheKeys) {
         const uint32_t s = t & (kEmCacheSlots - 1u);
-        const int32_t old = atomicCAS(g.cacheKeys + s, -1, (int32_t)t);
+        int32_t old = -1;           // the slot's key before the exchange
+        __hip_atomic_compare_exchange_strong((__attribute__((address_space(3))) int32_t *)(g.cacheKeys + s), &old,
+                                             (int32_t)t, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == -1 || old == (int32_t)t) {
-            atomicAdd(g.cacheVals + s * 3 + 0, (double)v.x);
-            atomicAdd(g.cacheVals + s * 3 + 1, (double)v.y);
-            atomicAdd(g.cacheVals + s * 3 + 2, (double)v.z);
+            LdsAdd(g.cacheVals + s * 3 + 0, (double)v.x);
+            LdsAdd(g.cacheVals + s * 3 + 1, (double)v.y);
+            LdsAdd(g.cacheVals + s * 3 + 2, (double)v.z);
             return;
         }
     }
     double *p = g.em + (size_t)t * 3;
-    atomicAdd(p + 0, (double)v.x);
-    atomicAdd(p + 1, (double)v.y);
-    atomicAdd(p + 2, (double)v.z);
+    GlobalAdd(p + 0, (double)v.x);
+    GlobalAdd(p + 1, (double)v.y);
+    GlobalAdd(p + 2, (double)v.z);
 }
 
 PG3_DEV bool IsDiffMaterial(int32_t type) { return type == kMatLambert || type == kMatGgxCond; }
@@ -77,15 +80,15 @@ PG3_DEV void ScatterEm(const GradSink &g, const EmFoot &f, F3 W) {
 PG3_DEV void MatGrad(const GradSink &g, int32_t base, const DevMaterial &m, F3 wil, F3 wol, F3 X) {
     const F3 a = g.adj * X;
     if (m.type == kMatLambert) {
-        if (m.diffuse.x > 0.f) atomicAdd(g.scal + base + 0, (double)(a.x / m.diffuse.x));
-        if (m.diffuse.y > 0.f) atomicAdd(g.scal + base + 1, (double)(a.y / m.diffuse.y));
-        if (m.diffuse.z > 0.f) atomicAdd(g.scal + base + 2, (double)(a.z / m.diffuse.z));
+        if (m.diffuse.x > 0.f) LdsAdd(g.scal + base + 0, (double)(a.x / m.diffuse.x));
+        if (m.diffuse.y > 0.f) LdsAdd(g.scal + base + 1, (double)(a.y / m.diffuse.y));
+        if (m.diffuse.z > 0.f) LdsAdd(g.scal + base + 2, (double)(a.z / m.diffuse.z));
     } else if (m.type == kMatGgxCond) {
-        if (m.glossy.x > 0.f) atomicAdd(g.scal + base + 0, (double)(a.x / m.glossy.x));
-        if (m.glossy.y > 0.f) atomicAdd(g.scal + base + 1, (double)(a.y / m.glossy.y));
-        if (m.glossy.z > 0.f) atomicAdd(g.scal + base + 2, (double)(a.z / m.glossy.z));
+        if (m.glossy.x > 0.f) LdsAdd(g.scal + base + 0, (double)(a.x / m.glossy.x));
+        if (m.glossy.y > 0.f) LdsAdd(g.scal + base + 1, (double)(a.y / m.glossy.y));
+        if (m.glossy.z > 0.f) LdsAdd(g.scal + base + 2, (double)(a.z / m.glossy.z));
         const float s = a.x + a.y + a.z;
-        if (s != 0.f) atomicAdd(g.scal + base + 3, (double)(s * GgxCondDLogDAlpha(m, wil, wol)));
+        if (s != 0.f) LdsAdd(g.scal + base + 3, (double)(s * GgxCondDLogDAlpha(m, wil, wol)));
     }
 }
 

@@ -1205,6 +1205,18 @@ struct TileItem {
 //   traces exactly one extension ray (counted here) and counts its segments; a path with at most
 //   one shadow ray per step sets `shadowed` when it traced one (counted here), others count
 //   their own in the tally.
+// FP64 adds to global memory and to LDS.  The explicit address space keeps them global and LDS
+// atomics where the compiler cannot tell where a pointer points (a framebuffer pointer read from
+// memory, an LDS array passed through a struct): a flat atomic may address either, so every
+// later wait for an LDS access would also wait for it.
+PG3_DEV void GlobalAdd(double *p, double v) {
+    __hip_atomic_fetch_add((__attribute__((address_space(1))) double *)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+PG3_DEV void LdsAdd(double *p, double v) {
+    __hip_atomic_fetch_add((__attribute__((address_space(3))) double *)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 template <class Path, bool kMultiScene, class Src>
 PG3_DEV void TraceTasks(Src &src, RayTally &tally) {
     const uint32_t lane = __lane_id();
@@ -1264,9 +1276,9 @@ PG3_DEV void TraceTasks(Src &src, RayTally &tally) {
         if (busy && path.Step(S, spec, tally, shadowed, step, ring)) {
             const F3 L = path.Result();
             double *f = fb + (size_t)px * 3;
-            atomicAdd(f + 0, (double)L.x);
-            atomicAdd(f + 1, (double)L.y);
-            atomicAdd(f + 2, (double)L.z);
+            GlobalAdd(f + 0, (double)L.x);
+            GlobalAdd(f + 1, (double)L.y);
+            GlobalAdd(f + 2, (double)L.z);
             busy = false;
             counted = path.CountsSample();
         }

@@ -85,7 +85,10 @@ __device__ __forceinline__ void BlockAppend(const bool (&p)[kN], uint32_t *const
             k[w] = total;
             total += n;
         }
-        k[kWaves] = total ? atomicAdd(c[threadIdx.x], total) : 0u;
+        // the counters are in global memory (a global atomic: the pointer array hides it)
+        k[kWaves] = total ? __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t *)c[threadIdx.x], total,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          : 0u;
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
