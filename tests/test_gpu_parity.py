@@ -258,6 +258,38 @@ def test_workload2_image_parity(ctx, kernel):
     assert_image_parity(g, o, frac=0.03)
 
 
+FULL_SIZE = [pytest.param(1, {}, id="C1_box_area_light_256px_16spp"),
+             pytest.param(2, {}, id="C2_box_steerable_em_512px_64spp"),
+             pytest.param(2, dict(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0),
+                          id="C2_reference_defaults_sb4_iic16_rr")]
+
+
+@pytest.mark.parametrize("workload,params", FULL_SIZE)
+def test_full_size_frame_against_the_oracle(ctx, workload, params):
+    """The bench's workloads at their full sizes (C1 256x256 x 16 spp, C2 512x512 x 64 spp, and C2
+    at the reference's defaults): the whole frame through the fused kernel, and a band of 8 image
+    rows through the oracle at the same spp (RunIteration x spp, path_tracer_base.hxx:57-90; the
+    same RNG streams, so pixels agree to FP32 rounding except where rounding sends a path another
+    way).  The whole frame is finite."""
+    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+    sb = SceneBuild.workload(workload, seed=2024)
+    sb.set_params(**params)
+    ctx.upload(sb)
+    ctx.render(0, sb.spp)
+    img, it = ctx.framebuffer(0, sb.resolution)
+    w, h = sb.resolution
+    assert it == sb.spp and img.shape == (h, w, 3)
+    assert np.isfinite(img).all() and img.sum() > 0
+    r0 = h // 2 - 4
+    ref, _ = O.OracleScene(sb).render(0, sb.spp, threads=0, pixel_begin=r0 * w, pixel_end=(r0 + 8) * w)
+    g, o = img[r0:r0 + 8], ref[r0:r0 + 8]
+    lum_g, lum_o = g.sum(-1), o.sum(-1)
+    diff = np.abs(lum_g - lum_o) / np.maximum(np.abs(lum_o), 1e-3 * max(1e-6, lum_o.mean()))
+    print(f"{w}x{h} x {sb.spp} spp, rows {r0}..{r0 + 7}: pixels off by >1e-3 {np.mean(diff > 1e-3):.4f}, "
+          f"max {diff.max():.2e}, band sum rel {abs(lum_g.sum() - lum_o.sum()) / lum_o.sum():.2e}")
+    assert_image_parity(g, o, frac=0.03)
+
+
 @pytest.mark.parametrize("workload,sampler", [(2, None), (2, _ffi.EM_SIMPLE), (2, _ffi.EM_COSINE), (1, None), (3, None)],
                          ids=["steerable_em", "simple_em", "cosine_em", "area_lights", "lights_and_em"])
 def test_wavefront_equals_megakernel(ctx, workload, sampler):
