@@ -258,20 +258,21 @@ def test_workload2_image_parity(ctx, kernel):
     assert_image_parity(g, o, frac=0.03)
 
 
-FULL_SIZE = [pytest.param(1, {}, id="C1_box_area_light_256px_16spp"),
-             pytest.param(2, {}, id="C2_box_steerable_em_512px_64spp"),
+FULL_SIZE = [pytest.param(1, {}, _ffi.KERNEL_MEGAKERNEL, id="C1_box_area_light_256px_16spp"),
+             pytest.param(2, {}, _ffi.KERNEL_MEGAKERNEL, id="C2_box_steerable_em_512px_64spp"),
+             pytest.param(2, {}, _ffi.KERNEL_WAVEFRONT, id="C2_box_steerable_em_512px_64spp_wavefront"),
              pytest.param(2, dict(splitting_budget=4.0, indirect_clipping=16.0, max_path_length=0),
-                          id="C2_reference_defaults_sb4_iic16_rr")]
+                          _ffi.KERNEL_MEGAKERNEL, id="C2_reference_defaults_sb4_iic16_rr")]
 
 
-@pytest.mark.parametrize("workload,params", FULL_SIZE)
-def test_full_size_frame_against_the_oracle(ctx, workload, params):
-    """The bench's workloads at their full sizes (C1 256x256 x 16 spp, C2 512x512 x 64 spp, and C2
-    at the reference's defaults): the whole frame through the fused kernel, and a band of 8 image
-    rows through the oracle at the same spp (RunIteration x spp, path_tracer_base.hxx:57-90; the
-    same RNG streams, so pixels agree to FP32 rounding except where rounding sends a path another
-    way).  The whole frame is finite."""
-    ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
+@pytest.mark.parametrize("workload,params,kernel", FULL_SIZE)
+def test_full_size_frame_against_the_oracle(ctx, workload, params, kernel):
+    """The bench's workloads at their full sizes (C1 256x256 x 16 spp, C2 512x512 x 64 spp through
+    the fused kernel and the wavefront pipeline, and C2 at the reference's defaults): the whole
+    frame on the GPU, and a band of 8 image rows through the oracle at the same spp (RunIteration x
+    spp, path_tracer_base.hxx:57-90; the same RNG streams, so pixels agree to FP32 rounding except
+    where rounding sends a path another way).  The whole frame is finite."""
+    ctx.set_kernel_mode(kernel)
     sb = SceneBuild.workload(workload, seed=2024)
     sb.set_params(**params)
     ctx.upload(sb)
