@@ -172,8 +172,10 @@ def cpu_baseline(sb, spp, target_s):
 def batch_measure(ctx, tune_candidates):
     """64-scene batch workload: one persistent batched pass vs per-scene rendering, both of the
     same samples (each scene's spp iterations 0..spp-1, same RNG streams), so at equal accuracy.
-    Two per-scene baselines: one launch per scene rendering all its iterations (`speedup`), and
-    the drop-in RunIteration loop, one launch per iteration (`speedup_vs_run_iteration`)."""
+    Per-scene baselines: one launch per scene rendering all its iterations (`speedup`), the
+    RunIteration loop with one launch per iteration (`speedup_vs_run_iteration`), and the same loop
+    through pg3g_run_iteration, which launches consecutive iterations in runs of up to 64 (the
+    drop-in adapter's path, `queued_run_iteration_msamples_per_s`)."""
     from pg3render_amd import Batch, SceneBuild, _ffi
     scenes = [SceneBuild.workload(3, seed=2024, variant=v) for v in range(64)]
     slots = list(range(1, 65))
@@ -209,6 +211,17 @@ def batch_measure(ctx, tune_candidates):
         ctx.synchronize()
         return time.perf_counter() - t0
 
+    def queued_run_iteration_pass():
+        # the same loop through pg3g_run_iteration: the last iteration of each scene launches the
+        # run still queued, as the adapter does for the last iteration of a fixed count
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for s, n in zip(slots, spp):
+            for it in range(n):
+                ctx.run_iteration(it, s, launch_now=(it == n - 1))
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
     plain_pass()                                   # warm-up (code objects, caches)
     plain_s = min(plain_pass() for _ in range(2))
     # What batching can recover at most: a per-scene launch pays its ramp-up and drain (lanes idle
@@ -224,6 +237,7 @@ def batch_measure(ctx, tune_candidates):
                "drain_ms": round(sum(2 * a - b for a, b in zip(k1, k2)), 3), "steady_ms": round(steady_ms, 3),
                "max_speedup": round(plain_s * 1e3 / steady_ms, 3)}
     run_it_s = run_iteration_pass()
+    queued_s = queued_run_iteration_pass()
     batch = Batch(ctx, slots, spp)
     t0 = time.perf_counter()
     batch.autotune(tune_candidates)
@@ -242,6 +256,7 @@ def batch_measure(ctx, tune_candidates):
                         "side 64..512, spp in {4,16,64,256}, maxpl 8", "samples_per_pass": total,
             "plain_per_scene_msamples_per_s": round(total / plain_s / 1e6, 3),
             "run_iteration_msamples_per_s": round(total / run_it_s / 1e6, 3),
+            "queued_run_iteration_msamples_per_s": round(total / queued_s / 1e6, 3),
             "batched_msamples_per_s": round(total / batched_s / 1e6, 3),
             "speedup": round(plain_s / batched_s, 3), "speedup_vs_run_iteration": round(run_it_s / batched_s, 3),
             "batched_gpu_ms_per_pass": round(min(gpu_ms), 3),
