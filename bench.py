@@ -174,7 +174,7 @@ def batch_measure(ctx, tune_candidates):
     same samples (each scene's spp iterations 0..spp-1, same RNG streams), so at equal accuracy.
     Per-scene baselines: one launch per scene rendering all its iterations (`speedup`), the
     RunIteration loop with one launch per iteration (`speedup_vs_run_iteration`), and the same loop
-    through pg3g_run_iteration, which launches consecutive iterations in runs of up to 64 (the
+    through pg3g_run_iteration, which launches consecutive iterations in runs of up to 256 (the
     drop-in adapter's path, `queued_run_iteration_msamples_per_s`)."""
     from pg3render_amd import Batch, SceneBuild, _ffi
     scenes = [SceneBuild.workload(3, seed=2024, variant=v) for v in range(64)]

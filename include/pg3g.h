@@ -165,13 +165,14 @@ int pg3g_render(pg3g_context *ctx, uint32_t first_iter, uint32_t n_iter);
 /* PathTracerBase::RunIteration of the drop-in renderer (src/path_tracer_base.hxx:57-90) as
  * AbstractRenderer::RunIteration calls it (src/renderer.hxx:232-278): queues iteration `iteration`
  * of a slot.  Consecutive queued iterations are launched together, up to PG3G_ITERATION_RUN per
- * launch (one launch per iteration would pay the launch's ramp-up and drain every time);
+ * launch (one launch per iteration would pay the launch's ramp-up and drain every time; 256
+ * iterations of a 512x512 frame are about 40 ms of GPU work);
  * launch_now != 0 launches what is queued at once (the last iteration of a fixed count, or every
  * iteration of a time-limited render).  An iteration that does not follow the queued ones launches
  * them first.  Reading the framebuffer (pg3g_read_framebuffer*, pg3g_framebuffer_device_ptr),
  * pg3g_synchronize, pg3g_render* of the slot and pg3g_set_params launch what is queued before they
  * act; pg3g_clear_framebuffer and an upload to the slot drop it.  Asynchronous. */
-#define PG3G_ITERATION_RUN 64
+#define PG3G_ITERATION_RUN 256
 int pg3g_run_iteration(pg3g_context *ctx, uint32_t slot, uint32_t iteration, int launch_now);
 int pg3g_render_slot(pg3g_context *ctx, uint32_t slot, uint32_t first_iter, uint32_t n_iter);
 

@@ -422,10 +422,11 @@ def test_iterations_accumulate_like_run_iteration(ctx):
 
 
 def test_queued_run_iterations_equal_one_launch(ctx):
-    """pg3g_run_iteration, RunIteration called once per iteration (renderer.hxx:232-278): 100
-    consecutive calls (a run of 64 launched by the 64th, the other 36 by the last call) equal one
-    launch of iterations [0,100); a read launches what is still queued; an iteration that does not
-    follow the queued ones launches them first; a clear drops what is queued."""
+    """pg3g_run_iteration, RunIteration called once per iteration (renderer.hxx:232-278): 300
+    consecutive calls (a run of PG3G_ITERATION_RUN = 256 launched by the 256th, the other 44 by
+    the last call) equal one launch of iterations [0,300); a read launches what is still queued; an
+    iteration that does not follow the queued ones launches them first; a clear drops what is
+    queued."""
     ctx.set_kernel_mode(_ffi.KERNEL_MEGAKERNEL)
     sb = SceneBuild.workload(1, seed=24, res=32)
 
@@ -436,11 +437,11 @@ def test_queued_run_iterations_equal_one_launch(ctx):
         return ctx.framebuffer(7, sb.resolution)
 
     ctx.upload(sb, 6)
-    for i in range(100):
-        ctx.run_iteration(i, 6, launch_now=(i == 99))
+    for i in range(300):
+        ctx.run_iteration(i, 6, launch_now=(i == 299))
     a, it = ctx.framebuffer(6, sb.resolution)
-    b, _ = plain([(0, 100)])
-    assert it == 100
+    b, _ = plain([(0, 300)])
+    assert it == 300
     np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12)
 
     ctx.clear(6)
