@@ -31,7 +31,7 @@ __device__ __forceinline__ uint32_t SceneOfItem(const BatchScene *__restrict__ i
 // entering a scene takes a slot holding that scene; if a slot is being staged with that scene it
 // takes none for now; otherwise it claims a slot no wavefront uses and stages the scene's tables
 // into it with its own lanes.  Each takes one compare-and-swap, and a wavefront without a slot
-// reads the scene's tables from memory and, at each work item it takes, joins a slot that has
+// reads the scene's tables from memory and, at each step of its task loop, joins a slot that has
 // finished staging its scene (the copy equals the tables in memory, so paths in flight may switch
 // to it).  No wavefront ever waits for another.
 constexpr uint32_t kBatchBlockWaves = 16;   // wavefronts of the largest batch workgroup (1024 lanes)
@@ -88,7 +88,6 @@ struct BatchItems {
         slot = __builtin_amdgcn_readfirstlane(__shfl(got, 0, 64));
     }
     PG3_DEV bool Next(TileItem &t) {
-        Join();
         uint32_t item;
         for (;;) {
             const uint32_t part = (blockIdx.x + exhausted) % kBatchParts;

@@ -1245,6 +1245,7 @@ PG3_DEV void TraceTasks(Src &src, RayTally &tally) {
                 scene = T.scene;
                 src.Enter(scene);
             }
+            src.Join();                // a cached copy of the scene's tables, once one is staged
         }
         // the scene is looked up by index at each use: a pointer variable carried around the
         // loop would lose its address space (flat loads in place of scalar and LDS loads)
