@@ -68,13 +68,20 @@ struct BatchItems {
     uint32_t *work;                    // [p]: items handed out of part p; [kBatchParts + 1 + p]: its first item
     uint32_t exhausted;                // parts found used up: the wave takes from part w + exhausted
     uint32_t cur;                      // scene entered (~0u: none yet)
-    int32_t slot;                      // its LDS slot (-1: tables read from memory)
-    uint32_t minLevel, baseNode;       // its staged tree levels (TreeCache of StageTree)
-    bool marg;                         // the slot holds the marginal CDF instead
-    bool want;                         // the scene has tables an LDS slot can hold
+    // Its tables, one word (the kernel's scalar registers are all in use): bits 0-1 the LDS slot + 1
+    // (0: tables read from memory), bit 2 the slot holds the marginal CDF instead of tree levels,
+    // bit 3 the scene has tables a slot can hold, bits 4-8 the staged tree's lowest level and
+    // bits 9-31 its first node (TreeCache of StageTree; a tree refined to level 9 has 6.3 M nodes).
+    uint32_t tab;
+    PG3_DEV int32_t slot() const { return (int32_t)(tab & 3u) - 1; }
+    PG3_DEV void setSlot(int32_t k) { tab = (tab & ~3u) | (uint32_t)(k + 1); }
+    PG3_DEV bool marg() const { return (tab >> 2) & 1u; }
+    PG3_DEV bool want() const { return (tab >> 3) & 1u; }
+    PG3_DEV uint32_t minLevel() const { return (tab >> 4) & 31u; }
+    PG3_DEV uint32_t baseNode() const { return tab >> 9; }
     // Without a slot: join one that holds the entered scene and is no longer being staged.
     PG3_DEV void Join() {
-        if (slot >= 0 || !want) return;
+        if (slot() >= 0 || !want()) return;
         int32_t got = -1;
         if (__lane_id() == 0) {
             uint32_t *state = BatchSlots::State();
@@ -85,7 +92,7 @@ struct BatchItems {
                     got = (int32_t)k;
             }
         }
-        slot = __builtin_amdgcn_readfirstlane(__shfl(got, 0, 64));
+        setSlot(__builtin_amdgcn_readfirstlane(__shfl(got, 0, 64)));
     }
     PG3_DEV bool Next(TileItem &t) {
         uint32_t item;
@@ -117,25 +124,23 @@ struct BatchItems {
     PG3_DEV void Enter(uint32_t s) {
         uint32_t *state = BatchSlots::State();
         const bool lead = __lane_id() == 0;
-        if (slot >= 0 && lead) atomicSub(&state[slot], 1u);
-        slot = -1;
+        if (slot() >= 0 && lead) atomicSub(&state[slot()], 1u);
+        tab = 0u;
         cur = s;
-        marg = false;
-        want = false;
         const DevEm &E = views[s].em;
         const int kind = (kEmKind == kEmDynamic) ? (E.w > 0 ? E.sampler : kEmAbsent) : kEmKind;
         uint32_t count = 0;
         if (kind == kEmSteerable) {
-            minLevel = TreeLevels(E, count);
-            if (minLevel >= E.numLevels) return;
-            baseNode = E.levelBase[minLevel];
+            const uint32_t minLevel = TreeLevels(E, count);
+            if (minLevel >= E.numLevels || E.levelBase[minLevel] >= (1u << 23)) return;   // memory tables
+            tab = (E.levelBase[minLevel] << 9) | (minLevel << 4);
         } else if (kind == kEmSimple) {
             if (E.h + 1u > kMargLdsMax) return;
-            marg = true;
+            tab = 4u;
         } else {
             return;
         }
-        want = true;
+        tab |= 8u;
         int32_t got = -1;
         uint32_t stage = 0;
         if (lead) {
@@ -160,25 +165,25 @@ struct BatchItems {
         if (got < 0) return;
         if (stage) {
             TreeCache tc;
-            if (marg) StageMarginal(E, reinterpret_cast<float *>(BatchSlots::Tables((uint32_t)got)), tc, __lane_id(), 64u);
+            if (marg()) StageMarginal(E, reinterpret_cast<float *>(BatchSlots::Tables((uint32_t)got)), tc, __lane_id(), 64u);
             else StageTree(E, BatchSlots::Tables((uint32_t)got), __lane_id(), 64u);
             __threadfence_block();   // the tables are in LDS before the slot is published
             if (lead) atomicAnd(&state[got], ~kSlotStaging);
         }
-        slot = got;
+        setSlot(got);
     }
     PG3_DEV const SceneView *View(uint32_t s) const { return views + s; }
     PG3_DEV LightSpec Spec(uint32_t s) const {
         const SceneView &S = views[s];
         LightSpec l = (kEmKind == kEmDynamic) ? SceneLightSpec(S) : LightSpec{kEmKind, kOneLight ? 1u : S.numLights};
-        if (slot >= 0 && s == cur) {
-            const float *t = reinterpret_cast<const float *>(BatchSlots::Tables((uint32_t)slot));
-            if (marg) {
+        if (slot() >= 0 && s == cur) {
+            const float *t = reinterpret_cast<const float *>(BatchSlots::Tables((uint32_t)slot()));
+            if (marg()) {
                 l.tree.marg = t;
             } else {
                 l.tree.nodes = t;
-                l.tree.minLevel = minLevel;
-                l.tree.baseNode = baseNode;
+                l.tree.minLevel = minLevel();
+                l.tree.baseNode = baseNode();
             }
         }
         return l;
@@ -193,7 +198,7 @@ BatchKernel(const SceneView *__restrict__ views, const BatchScene *__restrict__ 
             uint32_t *__restrict__ work, unsigned long long *__restrict__ counters) {
     if (threadIdx.x < kSlotSlots) BatchSlots::State()[threadIdx.x] = 0u;
     __syncthreads();
-    BatchItems<kEmKind, kOneLight> src{views, info, numScenes, work[kBatchParts], work, 0u, ~0u, -1, 0u, 0u, false, false};
+    BatchItems<kEmKind, kOneLight> src{views, info, numScenes, work[kBatchParts], work, 0u, ~0u, 0u};
     RayTally tally;
     TraceTasks<ChainPath<BatchItems<kEmKind, kOneLight>::kMergeEm>, true>(src, tally);
     FlushTally(tally, counters);
