@@ -172,18 +172,20 @@ PG3_HD void Dir2LatLong(F3 d, float &u, float &v) {                             
 }
 
 // sampling.hxx warps
-PG3_HD F3 SampleCosHemisphereW(float u0, float u1, float *pdf) {                   // :289-307
+// The pdf of a sample w is w.z / pi (the reference's oPdfW output); callers that need it compute
+// it from the returned direction (an optional pdf pointer, null at some call sites, kept the
+// BSDF record of the shading code in scratch memory after the compiler merged the calls).
+PG3_HD F3 SampleCosHemisphereW(float u0, float u1) {                               // :289-307
     const float t1 = 2.f * kPiF * u0;
     const float t2 = sqrtf(1.f - u1);
     float s, c;
     sincosf(t1, &s, &c);
-    const F3 r = mk(c * t2, s * t2, sqrtf(u1));
-    if (pdf) *pdf = r.z * kPiInvF;
-    return r;
+    return mk(c * t2, s * t2, sqrtf(u1));
 }
 PG3_HD float CosHemispherePdfW(float z) { return fmaxf(0.f, z) * kPiInvF; }        // :309-312
 PG3_HD F3 SampleCosSphereParamPdfW(float u0, float u1, float u2, bool up, bool low, float &pdf) {  // :326-345
-    F3 w = SampleCosHemisphereW(u0, u1, &pdf);
+    F3 w = SampleCosHemisphereW(u0, u1);
+    pdf = w.z * kPiInvF;
     if (up && low) { if (u2 < 0.5f) w = w * -1.0f; pdf *= 0.5f; }
     else if (low) w = w * -1.0f;
     return w;
@@ -407,28 +409,32 @@ PG3_HD float DielPdf(const DevMaterial &m, const DielCtx &c) {                  
 
 // AbstractMaterial::EvalBsdf with kOptSamplingProbs (pdfW, compProb)
 PG3_HD void EvalBsdf(const DevMaterial &m, F3 wil, F3 wol, BsdfRec &r) {
-    r.wil = wil;
+    // the record is written once at each exit: its fields stay in registers (a record written
+    // field by field in every case of the switch was kept in scratch memory)
+    F3 o_wil = mk(0.f), o_att = mk(0.f);
+    float o_pdf = 0.f, o_comp = 0.f;
+    o_wil = wil;
     switch (m.type) {
     case kMatLambert:                                                                 // :268-345
-        r.att = (wil.z <= 0.f || wol.z <= 0.f) ? mk(0.f) : m.diffuse / kPiF;
-        r.pdfW = CosHemispherePdfW(wil.z);
-        r.compProb = 1.f;
+        o_att = (wil.z <= 0.f || wol.z <= 0.f) ? mk(0.f) : m.diffuse / kPiF;
+        o_pdf = CosHemispherePdfW(wil.z);
+        o_comp = 1.f;
         break;
     case kMatPhong: {                                                                 // :424-437, 579-596
-        r.att = PhongEval(m, wil, wol);
+        o_att = PhongEval(m, wil, wol);
         const float dEst = Luminance(m.diffuse);
         const float gEst = Luminance(m.glossy) * (0.5f + 0.5f * fmaxf(wol.z, 0.f));
-        r.pdfW = PhongPdf(m, wol, wil, dEst, gEst);
-        r.compProb = 1.0f;
+        o_pdf = PhongPdf(m, wol, wil, dEst, gEst);
+        o_comp = 1.0f;
         break;
     }
     case kMatGgxCond: {                                                               // :808-816, 889-959
         const F3 h = HalfwayReflection(wil, wol);
         const float D = DistributionGgx(h, m.alpha);
         const float F = FresnelConductor(Dot(h, wol), m.eta, m.k);
-        r.att = GgxCondEval(m, wil, wol, h, D, F);
-        r.compProb = 1.0f;
-        r.pdfW = (wol.z < 0.f) ? 0.0f : GgxVndfPdf(wol, h, D, m.alpha) * ReflectionJacobian(wol, h);
+        o_att = GgxCondEval(m, wil, wol, h, D, F);
+        o_comp = 1.0f;
+        o_pdf = (wol.z < 0.f) ? 0.0f : GgxVndfPdf(wol, h, D, m.alpha) * ReflectionJacobian(wol, h);
         break;
     }
     case kMatGgxDiel: {                                                               // :1023-1031, 1134-1167
@@ -443,57 +449,65 @@ PG3_HD void EvalBsdf(const DevMaterial &m, F3 wil, F3 wol, BsdfRec &r) {
         c.h = c.refl ? HalfwayReflection(c.wilS, c.wolS) : HalfwayRefraction(c.wilS, c.wolS, c.etaInvS);
         c.D = DistributionGgx(c.h, m.alpha);
         c.F = FresnelDielectric(Dot(c.h, c.wolS), c.etaS);
-        r.att = DielEval(m, c);
-        r.pdfW = DielPdf(m, c);
-        r.compProb = 1.0f;
+        o_att = DielEval(m, c);
+        o_pdf = DielPdf(m, c);
+        o_comp = 1.0f;
         break;
     }
     default:                                                                          // :612-654
-        r.att = mk(0.f);
-        r.pdfW = 0.f;
-        r.compProb = 0.f;
+        o_att = mk(0.f);
+        o_pdf = 0.f;
+        o_comp = 0.f;
         break;
     }
+    r = BsdfRec{o_wil, o_att, o_pdf, o_comp};
 }
 
 // AbstractMaterial::SampleBsdf with explicit uniforms (u0,u1 direction, u2 component choice)
 PG3_HD void SampleBsdf(const DevMaterial &m, F3 wol, float u0, float u1, float u2, BsdfRec &r) {
+    // the record is written once at each exit: its fields stay in registers (a record written
+    // field by field in every case of the switch was kept in scratch memory)
+    F3 o_wil = mk(0.f), o_att = mk(0.f);
+    float o_pdf = 0.f, o_comp = 0.f;
     switch (m.type) {
     case kMatLambert:                                                                 // :285-292
-        r.wil = SampleCosHemisphereW(u0, u1, &r.pdfW);
-        r.att = (r.wil.z <= 0.f || wol.z <= 0.f) ? mk(0.f) : m.diffuse / kPiF;
-        r.compProb = 1.f;
+        o_wil = SampleCosHemisphereW(u0, u1);
+        o_pdf = o_wil.z * kPiInvF;
+        o_att = (o_wil.z <= 0.f || wol.z <= 0.f) ? mk(0.f) : m.diffuse / kPiF;
+        o_comp = 1.f;
         break;
     case kMatPhong: {                                                                 // :442-504
         const float dEst = Luminance(m.diffuse);
         const float gEst = Luminance(m.glossy) * (0.5f + 0.5f * fmaxf(wol.z, 0.f));
         const float tot = dEst + gEst;
         if (tot < kBlockerEps) {
-            r.att = mk(0.f);
-            r.wil = SampleCosHemisphereW(u0, u1, &r.pdfW);
-            r.compProb = 1.f;
+            o_att = mk(0.f);
+            o_wil = SampleCosHemisphereW(u0, u1);
+            o_pdf = o_wil.z * kPiInvF;
+            o_comp = 1.f;
+            r = BsdfRec{o_wil, o_att, o_pdf, o_comp};
             return;
         }
         if (u2 * tot < dEst) {
-            r.wil = SampleCosHemisphereW(u0, u1, nullptr);
+            o_wil = SampleCosHemisphereW(u0, u1);
         } else {
             const F3 cs = SamplePowerCosHemisphereW(u0, u1, m.phongExp);
-            r.wil = ToWorld(FrameFromZ(ReflectLocal(wol)), cs);
+            o_wil = ToWorld(FrameFromZ(ReflectLocal(wol)), cs);
         }
-        r.compProb = 1.f;
-        r.pdfW = PhongPdf(m, wol, r.wil, dEst, gEst);
-        r.att = (r.wil.z > 0.0f) ? PhongEval(m, r.wil, wol) : mk(0.f);
+        o_comp = 1.f;
+        o_pdf = PhongPdf(m, wol, o_wil, dEst, gEst);
+        o_att = (o_wil.z > 0.0f) ? PhongEval(m, o_wil, wol) : mk(0.f);
         break;
     }
     case kMatGgxCond: {                                                               // :819-849
         const F3 h = SampleGgxVisibleNormals(wol, m.alpha, u0, u1);
         const float D = DistributionGgx(h, m.alpha);
         bool above;
-        r.wil = Reflect(wol, h, above);
+        o_wil = Reflect(wol, h, above);
         const float F = FresnelConductor(Dot(h, wol), m.eta, m.k);
-        r.compProb = 1.0f;
-        r.pdfW = (wol.z < 0.f) ? 0.0f : GgxVndfPdf(wol, h, D, m.alpha) * ReflectionJacobian(wol, h);
-        r.att = (!above || r.wil.z < 0.0f) ? mk(0.f) : GgxCondEval(m, r.wil, wol, h, D, F);
+        o_comp = 1.0f;
+        o_pdf = (wol.z < 0.f) ? 0.0f : GgxVndfPdf(wol, h, D, m.alpha) * ReflectionJacobian(wol, h);
+        o_att = (!above || o_wil.z < 0.0f) ? mk(0.f) : GgxCondEval(m, o_wil, wol, h, D, F);
         break;
     }
     case kMatGgxDiel: {                                                               // :1034-1089
@@ -517,36 +531,37 @@ PG3_HD void SampleBsdf(const DevMaterial &m, F3 wol, float u0, float u1, float u
             cosAbs = -c.wilS.z;
             c.refl = false;
         }
-        r.wil = c.wilS * s;
-        r.pdfW = DielPdf(m, c);
-        r.compProb = 1.0f;
-        r.att = (!above || cosAbs < 0.0f) ? mk(0.f) : DielEval(m, c);
+        o_wil = c.wilS * s;
+        o_pdf = DielPdf(m, c);
+        o_comp = 1.0f;
+        o_att = (!above || cosAbs < 0.0f) ? mk(0.f) : DielEval(m, c);
         break;
     }
     case kMatSmoothCond:                                                              // :672-687
-        r.wil = ReflectLocal(wol);
-        r.pdfW = kInf;
-        r.compProb = 1.f;
-        r.att = mk(FresnelConductor(r.wil.z, m.eta, m.k));
+        o_wil = ReflectLocal(wol);
+        o_pdf = kInf;
+        o_comp = 1.f;
+        o_att = mk(FresnelConductor(o_wil.z, m.eta, m.k));
         break;
     default: {                                                                        // :730-766
         const float F = FresnelDielectric(wol.z, m.eta);
         float a;
         if (u2 <= F) {
-            r.wil = ReflectLocal(wol);
+            o_wil = ReflectLocal(wol);
             a = F;
-            r.compProb = F;
+            o_comp = F;
         } else {
             bool inAbove;
-            r.wil = Refract(wol, mk(0.f, 0.f, 1.f), m.eta, inAbove);
+            o_wil = Refract(wol, mk(0.f, 0.f, 1.f), m.eta, inAbove);
             a = (1.0f - F) * (inAbove ? Sqr(m.eta) : Sqr(m.etaInv));
-            r.compProb = 1.0f - F;
+            o_comp = 1.0f - F;
         }
-        r.att = mk(a);
-        r.pdfW = kInf;
+        o_att = mk(a);
+        o_pdf = kInf;
         break;
     }
     }
+    r = BsdfRec{o_wil, o_att, o_pdf, o_comp};
 }
 
 PG3_HD float RRProb(const DevMaterial &m, F3 wol) {
