@@ -22,18 +22,18 @@ TRI_FLOP, SPH_FLOP, SHADE_FLOP = 62, 35, 400
 BYTES_PER_SAMPLE, BYTES_PER_SEGMENT = 112, 304
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 KiB, factors from a 512 MiB copy): profiles/render_pt_box_steerable_em_512px_64spp.md
-MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 5770405 + 4088079) * 1024)}
+MEASURED_TRAFFIC = {(2, "megakernel"): int((2.0 * 3855186 + 1467896) * 1024)}
 # Wave-level VALU instructions per launch of the dominant kernel (rocprofv3 SQ_INSTS_VALU, same
 # file) and the issue peak they are priced against: 256 CUs x 2 FP32 wave-64 instructions per
 # cycle (four SIMDs) x 2.4 GHz
-MEASURED_VALU_INSTS = {(2, "megakernel"): 7059463088}
+MEASURED_VALU_INSTS = {(2, "megakernel"): 7047553485}
 VALU_ISSUE_PEAK = 256 * 2 * 2.4e9
 # HBM bytes per frame of the wavefront pipeline on the default workload (all Wf* launches of a
 # frame, same calibration): profiles/wavefront_box_steerable_em_512px_64spp.md
-MEASURED_WF_TRAFFIC = int((2.0 * 16694492 + 17762542) * 1024)
+MEASURED_WF_TRAFFIC = int((2.0 * 16145826 + 15518939) * 1024)
 # HBM bytes per launch of the split kernel at the reference's defaults (same calibration and file)
-MEASURED_SPLIT_TRAFFIC = {"box_area_light_256px_16spp": int((2.0 * 1201257 + 1157614) * 1024),
-                          "box_steerable_em_512px_64spp": int((2.0 * 57084618 + 21762099) * 1024)}
+MEASURED_SPLIT_TRAFFIC = {"box_area_light_256px_16spp": int((2.0 * 887732 + 696239) * 1024),
+                          "box_steerable_em_512px_64spp": int((2.0 * 52301999 + 13773262) * 1024)}
 
 
 def parse():
